@@ -1,0 +1,334 @@
+// Bonded terms and the 1-4 / exclusion-correction pairs.
+//
+// Physics follows CudaBondedForce (bond :84-146, angle :222-300 with cos
+// clamped to +-0.999, dihedral/improper :386-684 of src/CudaBondedForce.cu)
+// and calc_14_force_kernel (src/CudaDirectForceKernels.cu:640-870): no cutoff
+// and no periodic image for these lists (ishift 13 in the reference).
+// One thread per term, FP32 (or FP64) math, int64 2^-40 atomics, per-block
+// FP64 energy/virial partials reduced later in fixed order.
+#include <cmath>
+
+#include "system.hpp"
+
+namespace apo {
+
+__device__ __forceinline__ unsigned long long fxr(float v) {
+  return (unsigned long long)(long long)__builtin_rintf(v * 1099511627776.0f);
+}
+__device__ __forceinline__ unsigned long long fxr(double v) {
+  return (unsigned long long)(long long)__builtin_rint(v * 1099511627776.0);
+}
+
+template <typename CT>
+__device__ __forceinline__ void add_grad(unsigned long long *f, int stride, int i, CT gx, CT gy,
+                                         CT gz) {
+  atomicAdd(&f[i], fxr(gx));
+  atomicAdd(&f[(size_t)stride + i], fxr(gy));
+  atomicAdd(&f[2 * (size_t)stride + i], fxr(gz));
+}
+
+template <int NT>
+__device__ __forceinline__ void block_partials(double *vals, int nv, double *sh, double *out) {
+  for (int k = 0; k < nv; k++) {
+    double v = vals[k];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0) sh[(threadIdx.x >> 6) * 16 + k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < nv) {
+    double s = 0;
+    for (int w = 0; w < NT / 64; w++) s += sh[w * 16 + threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+}
+
+// W -= d (x) g, symmetric part kept as xx xy xz yy yz zz
+template <typename CT>
+__device__ __forceinline__ void vir_add(double *v, CT dx, CT dy, CT dz, CT gx, CT gy, CT gz) {
+  v[0] -= (double)(dx * gx);
+  v[1] -= (double)(dx * gy);
+  v[2] -= (double)(dx * gz);
+  v[3] -= (double)(dy * gy);
+  v[4] -= (double)(dy * gz);
+  v[5] -= (double)(dz * gz);
+}
+
+template <typename CT> __device__ __forceinline__ CT sqrt_ct(CT x);
+template <> __device__ __forceinline__ float sqrt_ct<float>(float x) { return sqrtf(x); }
+template <> __device__ __forceinline__ double sqrt_ct<double>(double x) { return sqrt(x); }
+template <typename CT> __device__ __forceinline__ CT acos_ct(CT x);
+template <> __device__ __forceinline__ float acos_ct<float>(float x) { return acosf(x); }
+template <> __device__ __forceinline__ double acos_ct<double>(double x) { return acos(x); }
+template <typename CT> __device__ __forceinline__ CT asin_ct(CT x);
+template <> __device__ __forceinline__ float asin_ct<float>(float x) { return asinf(x); }
+template <> __device__ __forceinline__ double asin_ct<double>(double x) { return asin(x); }
+
+struct BondedArgs {
+  const float4 *xyzq;
+  const BondT *bonds, *ureyb;
+  const AngleT *angles;
+  const DiheT *dihe, *imdihe;
+  int nb, nub, na, nd, ni;
+  int stride;
+};
+
+// Dihedral gradient from dE/dphi (F, G, H, A = F x G, B = H x G convention of
+// src/CudaBondedForce.cu:515-580): returns cos/sin of phi via (ct, st).
+template <typename CT>
+__device__ __forceinline__ void dihedral_geom(const float4 *x, int i, int j, int k, int l, CT &ct,
+                                              CT &st, CT (&F)[3], CT (&G)[3], CT (&H)[3],
+                                              CT (&A)[3], CT (&B)[3], CT &ra2, CT &rb2, CT &rg) {
+  const float4 xi = x[i], xj = x[j], xk = x[k], xl = x[l];
+  F[0] = (CT)xi.x - (CT)xj.x; F[1] = (CT)xi.y - (CT)xj.y; F[2] = (CT)xi.z - (CT)xj.z;
+  G[0] = (CT)xj.x - (CT)xk.x; G[1] = (CT)xj.y - (CT)xk.y; G[2] = (CT)xj.z - (CT)xk.z;
+  H[0] = (CT)xl.x - (CT)xk.x; H[1] = (CT)xl.y - (CT)xk.y; H[2] = (CT)xl.z - (CT)xk.z;
+  A[0] = F[1] * G[2] - F[2] * G[1]; A[1] = F[2] * G[0] - F[0] * G[2]; A[2] = F[0] * G[1] - F[1] * G[0];
+  B[0] = H[1] * G[2] - H[2] * G[1]; B[1] = H[2] * G[0] - H[0] * G[2]; B[2] = H[0] * G[1] - H[1] * G[0];
+  ra2 = A[0] * A[0] + A[1] * A[1] + A[2] * A[2];
+  rb2 = B[0] * B[0] + B[1] * B[1] + B[2] * B[2];
+  rg = sqrt_ct<CT>(G[0] * G[0] + G[1] * G[1] + G[2] * G[2]);
+  const CT rab = (CT)1 / sqrt_ct<CT>(ra2 * rb2);
+  ct = (A[0] * B[0] + A[1] * B[1] + A[2] * B[2]) * rab;
+  st = rg * rab * (A[0] * H[0] + A[1] * H[1] + A[2] * H[2]);
+}
+
+template <typename CT, bool CALC_E, bool CALC_V>
+__global__ __launch_bounds__(256) void k_bonded(BondedArgs a, unsigned long long *__restrict__ f,
+                                                double *__restrict__ part) {
+  __shared__ double sh[4 * 16];
+  double acc[kBondPart];
+#pragma unroll
+  for (int k = 0; k < kBondPart; k++) acc[k] = 0.0;
+  int idx = blockIdx.x * 256 + threadIdx.x;
+  const float4 *x = a.xyzq;
+  if (idx < a.nb + a.nub) {
+    const bool ub = idx >= a.nb;
+    const BondT b = ub ? a.ureyb[idx - a.nb] : a.bonds[idx];
+    const float4 xi = x[b.i], xj = x[b.j];
+    const CT dx = (CT)xi.x - (CT)xj.x, dy = (CT)xi.y - (CT)xj.y, dz = (CT)xi.z - (CT)xj.z;
+    const CT r = sqrt_ct<CT>(dx * dx + dy * dy + dz * dz);
+    const CT db = r - (CT)b.b0;
+    CT fij = db * (CT)b.kb;
+    if (CALC_E) acc[ub ? 2 : 0] += (double)(fij * db);
+    fij *= (CT)2 / r;
+    add_grad<CT>(f, a.stride, b.i, fij * dx, fij * dy, fij * dz);
+    add_grad<CT>(f, a.stride, b.j, -fij * dx, -fij * dy, -fij * dz);
+    if (CALC_V) vir_add<CT>(acc + 5, dx, dy, dz, fij * dx, fij * dy, fij * dz);
+  } else if ((idx -= a.nb + a.nub) < a.na) {
+    const AngleT g = a.angles[idx];
+    const float4 xi = x[g.i], xj = x[g.j], xk = x[g.k];
+    const CT dxij = (CT)xi.x - (CT)xj.x, dyij = (CT)xi.y - (CT)xj.y, dzij = (CT)xi.z - (CT)xj.z;
+    const CT dxkj = (CT)xk.x - (CT)xj.x, dykj = (CT)xk.y - (CT)xj.y, dzkj = (CT)xk.z - (CT)xj.z;
+    const CT rij = sqrt_ct<CT>(dxij * dxij + dyij * dyij + dzij * dzij);
+    const CT rkj = sqrt_ct<CT>(dxkj * dxkj + dykj * dykj + dzkj * dzkj);
+    const CT rij_inv = (CT)1 / rij, rkj_inv = (CT)1 / rkj;
+    const CT uxi = dxij * rij_inv, uyi = dyij * rij_inv, uzi = dzij * rij_inv;
+    const CT uxk = dxkj * rkj_inv, uyk = dykj * rkj_inv, uzk = dzkj * rkj_inv;
+    CT cst = uxi * uxk + uyi * uyk + uzi * uzk;
+    cst = fmin(fmax(cst, (CT)-0.999), (CT)0.999);
+    const CT da = acos_ct<CT>(cst) - (CT)g.t0;
+    CT df = (CT)g.kt * da;
+    if (CALC_E) acc[1] += (double)(df * da);
+    df = -(CT)2 * df / sqrt_ct<CT>((CT)1 - cst * cst);
+    // gradient on i and k: -dE/dtheta / sin * d cos / dx
+    const CT gix = df * rij_inv * (uxk - cst * uxi), giy = df * rij_inv * (uyk - cst * uyi),
+             giz = df * rij_inv * (uzk - cst * uzi);
+    const CT gkx = df * rkj_inv * (uxi - cst * uxk), gky = df * rkj_inv * (uyi - cst * uyk),
+             gkz = df * rkj_inv * (uzi - cst * uzk);
+    add_grad<CT>(f, a.stride, g.i, gix, giy, giz);
+    add_grad<CT>(f, a.stride, g.k, gkx, gky, gkz);
+    add_grad<CT>(f, a.stride, g.j, -gix - gkx, -giy - gky, -giz - gkz);
+    if (CALC_V) {
+      vir_add<CT>(acc + 5, dxij, dyij, dzij, gix, giy, giz);
+      vir_add<CT>(acc + 5, dxkj, dykj, dzkj, gkx, gky, gkz);
+    }
+  } else if ((idx -= a.na) < a.nd + a.ni) {
+    const bool imp = idx >= a.nd;
+    const DiheT d = imp ? a.imdihe[idx - a.nd] : a.dihe[idx];
+    CT ct, st, F[3], G[3], H[3], A[3], B[3], ra2, rb2, rg;
+    dihedral_geom<CT>(x, d.i, d.j, d.k, d.l, ct, st, F, G, H, A, B, ra2, rb2, rg);
+    const int nper = (int)d.n;
+    CT dEdphi, e;
+    if (!imp && nper == 0) {
+      // multiplicity 0: constant k, no force (src/CudaBondedForce.cu:419-421)
+      e = (CT)d.kc;
+      dEdphi = 0;
+    } else if (nper != 0) {
+      // k (1 + cos(n phi - delta)) via the cos/sin recurrence
+      CT c1 = 1, s1 = 0;
+      for (int p = 0; p < (nper < 0 ? -nper : nper); p++) {
+        CT c2 = c1 * ct - s1 * st;
+        s1 = c1 * st + s1 * ct;
+        c1 = c2;
+      }
+      const int na_ = nper < 0 ? -nper : nper;
+      e = (CT)d.kc * ((CT)1 + c1 * (CT)d.cd + s1 * (CT)d.sd);
+      dEdphi = -(CT)d.kc * (CT)na_ * (s1 * (CT)d.cd - c1 * (CT)d.sd);
+    } else {
+      // harmonic improper on phi - phi0 wrapped to (-pi, pi]
+      const CT ca = ct * (CT)d.cd + st * (CT)d.sd;
+      const CT sa = st * (CT)d.cd - ct * (CT)d.sd;
+      CT ap;
+      if (ca > (CT)0.1)
+        ap = asin_ct<CT>(sa);
+      else {
+        ap = acos_ct<CT>(fmax(ca, (CT)-1));
+        ap = sa > (CT)0 ? ap : -ap;
+      }
+      e = (CT)d.kc * ap * ap;
+      dEdphi = (CT)2 * (CT)d.kc * ap;
+    }
+    if (CALC_E) acc[imp ? 4 : 3] += (double)e;
+    // dphi/dri = -|G|/A^2 A, dphi/drl = |G|/B^2 B, cross terms on j and k
+    const CT fg = F[0] * G[0] + F[1] * G[1] + F[2] * G[2];
+    const CT hg = H[0] * G[0] + H[1] * G[1] + H[2] * G[2];
+    const CT ia2 = dEdphi / ra2, ib2 = dEdphi / rb2;
+    CT gi[3], gj[3], gk[3], gl[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const CT di = -rg * ia2 * A[c];
+      const CT dl = rg * ib2 * B[c];
+      const CT cr = fg * ia2 / rg * A[c] - hg * ib2 / rg * B[c];
+      gi[c] = di;
+      gj[c] = -di + cr;
+      gk[c] = -dl - cr;
+      gl[c] = dl;
+    }
+    add_grad<CT>(f, a.stride, d.i, gi[0], gi[1], gi[2]);
+    add_grad<CT>(f, a.stride, d.j, gj[0], gj[1], gj[2]);
+    add_grad<CT>(f, a.stride, d.k, gk[0], gk[1], gk[2]);
+    add_grad<CT>(f, a.stride, d.l, gl[0], gl[1], gl[2]);
+    if (CALC_V) {
+      vir_add<CT>(acc + 5, F[0], F[1], F[2], gi[0], gi[1], gi[2]);
+      vir_add<CT>(acc + 5, -G[0], -G[1], -G[2], gk[0], gk[1], gk[2]);
+      vir_add<CT>(acc + 5, H[0] - G[0], H[1] - G[1], H[2] - G[2], gl[0], gl[1], gl[2]);
+    }
+  }
+  if (CALC_E || CALC_V) block_partials<256>(acc, kBondPart, sh, part + blockIdx.x * kBondPart);
+}
+
+// 1-4 pairs and 1-2/1-3 Ewald exclusion corrections.
+struct P14Args {
+  const float4 *xyzq;
+  const int *vdwtype;
+  const int2 *in14, *ex14;
+  int nin, nex, stride;
+  int ntypes;
+  double e14fac;
+};
+
+template <typename CT> struct Tab2;
+template <> struct Tab2<float> { using T = float2; };
+template <> struct Tab2<double> { using T = double2; };
+
+template <typename CT> __device__ __forceinline__ CT erfc_ct(CT x);
+template <> __device__ __forceinline__ float erfc_ct<float>(float x) { return erfcf(x); }
+template <> __device__ __forceinline__ double erfc_ct<double>(double x) { return erfc(x); }
+template <typename CT> __device__ __forceinline__ CT exp_ct(CT x);
+template <> __device__ __forceinline__ float exp_ct<float>(float x) { return expf(x); }
+template <> __device__ __forceinline__ double exp_ct<double>(double x) { return exp(x); }
+
+template <typename CT, bool CALC_E, bool CALC_V>
+__global__ __launch_bounds__(256) void k_pairs14(P14Args a, const PairConsts<CT> P,
+                                                 const typename Tab2<CT>::T *__restrict__ tab14,
+                                                 unsigned long long *__restrict__ f,
+                                                 double *__restrict__ part) {
+  __shared__ double sh[4 * 16];
+  double acc[kP14Part];
+#pragma unroll
+  for (int k = 0; k < kP14Part; k++) acc[k] = 0.0;
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx < a.nin + a.nex) {
+    const bool ex = idx >= a.nin;
+    const int2 pr = ex ? a.ex14[idx - a.nin] : a.in14[idx];
+    const float4 xi = a.xyzq[pr.x], xj = a.xyzq[pr.y];
+    const CT dx = (CT)xi.x - (CT)xj.x, dy = (CT)xi.y - (CT)xj.y, dz = (CT)xi.z - (CT)xj.z;
+    const CT r2 = dx * dx + dy * dy + dz * dz;
+    const CT r = sqrt_ct<CT>(r2), rinv = (CT)1 / r;
+    const CT qq = (CT)332.0716 * (CT)xi.w * (CT)xj.w;
+    // Ewald with e14fac: qq (erfc + e14fac - 1)/r; exclusions use e14fac = 0
+    const CT efac = ex ? (CT)0 : (CT)a.e14fac;
+    const CT erfcv = erfc_ct<CT>(P.kappa * r);
+    const CT u_el = qq * (erfcv + efac - (CT)1) * rinv;
+    CT fr = -qq * P.two_kappa_sqrtpi * exp_ct<CT>(-P.kappa2 * r2) - u_el;
+    if (!ex) {
+      const int ti = a.vdwtype[pr.x], tj = a.vdwtype[pr.y];
+      const int aa = ti > tj ? ti : tj, bb = ti > tj ? tj : ti;
+      const typename Tab2<CT>::T c = tab14[aa * (aa + 1) / 2 + bb];
+      const CT rinv2 = rinv * rinv, rinv3 = rinv * rinv2, rinv6 = rinv3 * rinv3;
+      const bool outer = r2 > P.ron2;
+      const CT A6 = outer ? P.k6 : (CT)1, B6 = outer ? P.roffinv3 : (CT)0;
+      const CT A12 = outer ? P.k12 : (CT)1, B12 = outer ? P.roffinv6 : (CT)0;
+      const CT t6 = rinv3 - B6, t12 = rinv6 - B12;
+      fr += c.x * A6 * t6 * rinv3 - c.y * A12 * t12 * rinv6;
+      if (CALC_E) {
+        const CT C6 = outer ? (CT)0 : P.dv6, C12 = outer ? (CT)0 : P.dv12;
+        acc[1] += (double)(c.y * ((CT)1 / (CT)12) * (A12 * t12 * t12 + C12) -
+                           c.x * ((CT)1 / (CT)6) * (A6 * t6 * t6 + C6));
+      }
+    }
+    if (CALC_E) acc[ex ? 2 : 0] += (double)u_el;
+    const CT fij = fr * rinv * rinv;
+    add_grad<CT>(f, a.stride, pr.x, fij * dx, fij * dy, fij * dz);
+    add_grad<CT>(f, a.stride, pr.y, -fij * dx, -fij * dy, -fij * dz);
+    if (CALC_V) vir_add<CT>(acc + 3, dx, dy, dz, fij * dx, fij * dy, fij * dz);
+  }
+  if (CALC_E || CALC_V) block_partials<256>(acc, kP14Part, sh, part + blockIdx.x * kP14Part);
+}
+
+template <typename CT, typename F>
+static void dispatch_ev(bool E, bool V, F &&fn) {
+  if (E && V) fn(std::integral_constant<int, 3>());
+  else if (E) fn(std::integral_constant<int, 1>());
+  else if (V) fn(std::integral_constant<int, 2>());
+  else fn(std::integral_constant<int, 0>());
+}
+
+void bonded_launch(NbSystem &S, const float4 *x, int64_t *force, int stride, bool E, bool V,
+                   hipStream_t s) {
+  BondedArgs a{x, S.bonds.get(), S.ureyb.get(), S.angles.get(), S.dihe.get(), S.imdihe.get(),
+               S.nbond, S.nub, S.nangle, S.ndihe, S.nimdihe, stride};
+  const int nt = S.nbond + S.nub + S.nangle + S.ndihe + S.nimdihe;
+  auto *f = reinterpret_cast<unsigned long long *>(force);
+  double *part = S.parts.get() + S.part.bond_off;
+  const int G = S.part.bond_n;
+  if (nt == 0) {
+    if (E || V) launch_fill_f64(part, (size_t)G * kBondPart, s);
+    return;
+  }
+  auto go = [&](auto ev) {
+    constexpr int m = decltype(ev)::value;
+    if (S.precision == 1)
+      k_bonded<double, (m & 1) != 0, (m & 2) != 0><<<G, 256, 0, s>>>(a, f, part);
+    else
+      k_bonded<float, (m & 1) != 0, (m & 2) != 0><<<G, 256, 0, s>>>(a, f, part);
+  };
+  dispatch_ev<float>(E, V, go);
+  APO_LAUNCH_CHECK();
+}
+
+void pairs14_launch(NbSystem &S, const float4 *x, int64_t *force, int stride, bool E, bool V,
+                    hipStream_t s) {
+  P14Args a{x, S.vdwtype.get(), S.in14.get(), S.ex14.get(), S.nin14, S.nex14, stride,
+            S.ntypes, S.e14fac};
+  auto *f = reinterpret_cast<unsigned long long *>(force);
+  double *part = S.parts.get() + S.part.p14_off;
+  const int G = S.part.p14_n;
+  if (S.nin14 + S.nex14 == 0) {
+    if (E || V) launch_fill_f64(part, (size_t)G * kP14Part, s);
+    return;
+  }
+  auto go = [&](auto ev) {
+    constexpr int m = decltype(ev)::value;
+    if (S.precision == 1)
+      k_pairs14<double, (m & 1) != 0, (m & 2) != 0>
+          <<<G, 256, 0, s>>>(a, S.pc_d, S.vdwtab14_d.get(), f, part);
+    else
+      k_pairs14<float, (m & 1) != 0, (m & 2) != 0>
+          <<<G, 256, 0, s>>>(a, S.pc_f, S.vdwtab14.get(), f, part);
+  };
+  dispatch_ev<float>(E, V, go);
+  APO_LAUNCH_CHECK();
+}
+
+}  // namespace apo

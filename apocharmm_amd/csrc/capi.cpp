@@ -1,0 +1,184 @@
+// C ABI entry points (include/apocharmm_amd.h).  Error handling mirrors the
+// reference's C ABI: every body runs inside guard(), which maps exceptions to
+// apo_status and keeps the message in a thread-local string
+// (capi/private_include/apocharmm_c/detail/ErrorInternal.h:27-40,
+// capi/src/apocharmm_c/Error.cpp:18-22).
+#include <cstring>
+#include <string>
+
+#include "../../include/apocharmm_amd.h"
+#include "system.hpp"
+
+namespace apo {
+NbSystem *create_system(const apo_nb_params &p, const apo_nb_topology &t);
+void set_box(NbSystem &S, const double *b);
+void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int stride,
+                double *energy, double *virial, hipStream_t s);
+}  // namespace apo
+
+struct apo_nb_system {
+  apo::NbSystem *sys;
+};
+
+namespace {
+thread_local std::string g_last_error;
+
+template <typename F> apo_status guard(F &&f) {
+  try {
+    f();
+    g_last_error.clear();
+    return APO_OK;
+  } catch (const apo::InvalidArgument &e) {
+    g_last_error = e.what();
+    return APO_INVALID_ARGUMENT;
+  } catch (const apo::GpuError &e) {
+    g_last_error = e.what();
+    return APO_GPU_ERROR;
+  } catch (const std::invalid_argument &e) {
+    g_last_error = e.what();
+    return APO_INVALID_ARGUMENT;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return APO_RUNTIME_ERROR;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return APO_RUNTIME_ERROR;
+  }
+}
+
+apo::NbSystem &get(apo_nb_system *h) {
+  if (!h || !h->sys) throw apo::InvalidArgument("null system handle");
+  return *h->sys;
+}
+const apo::NbSystem &get(const apo_nb_system *h) {
+  if (!h || !h->sys) throw apo::InvalidArgument("null system handle");
+  return *h->sys;
+}
+}  // namespace
+
+extern "C" {
+
+APO_API const char *apo_last_error(void) { return g_last_error.c_str(); }
+
+APO_API const char *apo_version(void) { return "apocharmm_amd 0.1 (gfx950)"; }
+
+APO_API apo_status apo_nb_create(const apo_nb_params *params, const apo_nb_topology *topology,
+                                 apo_nb_system **out) {
+  return guard([&] {
+    if (!params || !topology || !out) throw apo::InvalidArgument("null argument");
+    *out = nullptr;
+    apo::NbSystem *S = apo::create_system(*params, *topology);
+    *out = new apo_nb_system{S};
+  });
+}
+
+APO_API apo_status apo_nb_destroy(apo_nb_system *sys) {
+  return guard([&] {
+    if (!sys) return;
+    delete sys->sys;
+    delete sys;
+  });
+}
+
+APO_API apo_status apo_nb_set_box(apo_nb_system *sys, const double *box3) {
+  return guard([&] {
+    if (!box3) throw apo::InvalidArgument("null box");
+    apo::set_box(get(sys), box3);
+  });
+}
+
+APO_API apo_status apo_nb_reset_list(apo_nb_system *sys, const float *d_xyzq, void *stream) {
+  return guard([&] {
+    if (!d_xyzq) throw apo::InvalidArgument("null xyzq");
+    apo::nblist_build(get(sys), reinterpret_cast<const float4 *>(d_xyzq),
+                      reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+APO_API apo_status apo_nb_calc_force(apo_nb_system *sys, const float *d_xyzq, int32_t flags,
+                                     int64_t *d_force, int32_t stride, double *d_energy,
+                                     double *d_virial, void *stream) {
+  return guard([&] {
+    apo::calc_force(get(sys), d_xyzq, flags, d_force, stride, d_energy, d_virial,
+                    reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+APO_API apo_status apo_nb_force_stride(const apo_nb_system *sys, int32_t *stride) {
+  return guard([&] {
+    if (!stride) throw apo::InvalidArgument("null stride");
+    const int n = get(sys).natom;
+    *stride = ((n + 31) * 8 + 255) / 256 * 256 / 8;
+  });
+}
+
+APO_API apo_status apo_nb_num_atoms(const apo_nb_system *sys, int32_t *natom) {
+  return guard([&] {
+    if (!natom) throw apo::InvalidArgument("null natom");
+    *natom = get(sys).natom;
+  });
+}
+
+APO_API apo_status apo_nb_force_to_double(const int64_t *d_force, int32_t natom, int32_t stride,
+                                          double *d_out, void *stream) {
+  return guard([&] {
+    if (!d_force || !d_out || natom < 0 || stride < natom)
+      throw apo::InvalidArgument("bad force conversion arguments");
+    if (natom)
+      apo::launch_force_to_double(d_force, natom, stride, d_out,
+                                  reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+APO_API apo_status apo_nb_list_stats(apo_nb_system *sys, int64_t *out6, void *stream) {
+  return guard([&] {
+    if (!out6) throw apo::InvalidArgument("null output");
+    apo::nblist_stats(get(sys), out6, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+APO_API apo_status apo_nb_profile(apo_nb_system *sys, int32_t enable) {
+  return guard([&] {
+    apo::NbSystem &S = get(sys);
+    S.profile_flush();
+    S.prof_on = enable != 0;
+    for (int k = 0; k < 8; k++) {
+      S.prof_ms[k] = 0;
+      S.prof_cnt[k] = 0;
+    }
+  });
+}
+
+APO_API apo_status apo_nb_profile_read(apo_nb_system *sys, double *ms8, int64_t *count8) {
+  return guard([&] {
+    if (!ms8 || !count8) throw apo::InvalidArgument("null output");
+    apo::NbSystem &S = get(sys);
+    S.profile_flush();
+    for (int k = 0; k < 8; k++) {
+      ms8[k] = S.prof_ms[k];
+      count8[k] = S.prof_cnt[k];
+    }
+  });
+}
+
+APO_API apo_status apo_topology_exclusions(int32_t natom, int32_t nbond, const int32_t *bonds,
+                                           int32_t *counts3, int32_t *excl_ptr,
+                                           int32_t *excl_idx, int32_t *in14, int32_t *ex14) {
+  return guard([&] {
+    if (natom < 0 || nbond < 0 || !counts3 || (nbond > 0 && !bonds))
+      throw apo::InvalidArgument("bad topology arguments");
+    for (int k = 0; k < 2 * nbond; k++)
+      if (bonds[k] < 0 || bonds[k] >= natom) throw apo::InvalidArgument("bond index out of range");
+    std::vector<int32_t> p, x, i14, e14;
+    apo::build_exclusions(natom, nbond, bonds, p, x, i14, e14);
+    counts3[0] = (int32_t)x.size();
+    counts3[1] = (int32_t)i14.size() / 2;
+    counts3[2] = (int32_t)e14.size() / 2;
+    if (excl_ptr) std::memcpy(excl_ptr, p.data(), p.size() * sizeof(int32_t));
+    if (excl_idx) std::memcpy(excl_idx, x.data(), x.size() * sizeof(int32_t));
+    if (in14) std::memcpy(in14, i14.data(), i14.size() * sizeof(int32_t));
+    if (ex14) std::memcpy(ex14, e14.data(), e14.size() * sizeof(int32_t));
+  });
+}
+
+}  // extern "C"

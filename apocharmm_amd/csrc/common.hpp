@@ -1,0 +1,141 @@
+// Shared definitions of the MI355X (gfx950) CHARMM force path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace apo {
+
+// Gradient fixed point: value * 2^40 in int64 (include/gpu_utils.h:21-23).
+constexpr double kForceScale = 1099511627776.0;  // 2^40
+constexpr double kInvForceScale = 1.0 / 1099511627776.0;
+// PME charge grid fixed point (int64, exact, order independent).
+constexpr double kGridScale = 1099511627776.0;  // 2^40
+constexpr double kCcelec = 332.0716;            // src/CudaDirectForceKernels.cu:59
+constexpr double kPi = 3.14159265358979323846;
+// Cell (tile edge) size and tiles per pair-kernel work item.
+constexpr int kCell = 32;
+constexpr int kItemTiles = 16;
+// Largest xy column population the LDS column sort handles.
+constexpr int kMaxColumn = 2048;
+// Pair-kernel per-block partials: elec, vdw, virial xx xy xz yy yz zz.
+constexpr int kPairPart = 8;
+// 1-4/exclusion partials: elec, vdw, ewex, virial(6).
+constexpr int kP14Part = 9;
+// Bonded partials: bond, angle, ureyb, dihe, imdihe, virial(6).
+constexpr int kBondPart = 11;
+// Reciprocal partials: ewks, virial(6).
+constexpr int kRecipPart = 7;
+
+struct InvalidArgument : std::invalid_argument {
+  using std::invalid_argument::invalid_argument;
+};
+struct RuntimeError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct GpuError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define APO_HIP(call)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      throw ::apo::GpuError(std::string(#call) + ": " +                        \
+                            hipGetErrorString(e_) + " (" __FILE__ ":" +        \
+                            std::to_string(__LINE__) + ")");                   \
+  } while (0)
+
+#define APO_LAUNCH_CHECK() APO_HIP(hipGetLastError())
+
+// Owning device buffer (no copies; moves allowed).
+template <typename T> struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  // (Re)allocates to hold at least `count` elements; contents are lost.
+  void reserve(size_t count) {
+    if (count <= n && p) return;
+    release();
+    size_t c = count ? count : 1;
+    APO_HIP(hipMalloc(&p, c * sizeof(T)));
+    n = c;
+  }
+  void upload(const T *h, size_t count, hipStream_t s) {
+    reserve(count);
+    if (count) APO_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void upload(const std::vector<T> &v, hipStream_t s) { upload(v.data(), v.size(), s); }
+  T *get() const { return p; }
+};
+
+// Pair-kernel constants (VFSW + Ewald), CT = float (mixed) or double (FP64).
+// Switching constants follow src/CudaPMEDirectForce.cu:321-331.
+template <typename CT> struct PairConsts {
+  CT sx, sy, sz;          // box / 2^32: fixed-point coordinate unit
+  CT roff2, ron2, rlist2;
+  CT k6, k12, dv6, dv12, roffinv3, roffinv6;
+  CT kappa, kappa2, k2log2e, two_kappa_sqrtpi;
+  CT erfc_p;              // t = 1/(1 + erfc_p * kappa r)
+  CT erfc_c[9];           // erfc(x) ~ exp(-x^2) * sum_k c_k t^k
+};
+
+// Offsets of per-block partial regions in the partials buffer.
+struct PartLayout {
+  int pair_off = 0, pair_n = 0;
+  int p14_off = 0, p14_n = 0;
+  int bond_off = 0, bond_n = 0;
+  int recip_off = 0, recip_n = 0;
+  int self_off = 0, self_n = 0;
+  int total = 0;
+};
+
+// ---- kernel launchers (defined in the .hip translation units) -------------
+struct NbSystem;
+
+void launch_fill_i64(int64_t *p, size_t n, hipStream_t s);
+void launch_fill_f64(double *p, size_t n, hipStream_t s);
+void launch_fill_i32(int *p, size_t n, hipStream_t s);
+void launch_scan(const int *in, int *out, const int *n_dev, int n_max,
+                 hipStream_t s);
+void launch_force_to_double(const int64_t *f, int natom, int stride,
+                            double *out, hipStream_t s);
+
+void nblist_build(NbSystem &sys, const float4 *d_xyzq, hipStream_t s);
+void nblist_sort_coords(NbSystem &sys, const float4 *d_xyzq, hipStream_t s);
+void nblist_stats(NbSystem &sys, int64_t *out6, hipStream_t s);
+
+void direct_launch(NbSystem &sys, bool energy, bool virial, hipStream_t s);
+void direct_finalize(NbSystem &sys, int64_t *d_force, int stride,
+                     hipStream_t s);
+void pairs14_launch(NbSystem &sys, const float4 *d_xyzq, int64_t *d_force,
+                    int stride, bool energy, bool virial, hipStream_t s);
+void bonded_launch(NbSystem &sys, const float4 *d_xyzq, int64_t *d_force,
+                   int stride, bool energy, bool virial, hipStream_t s);
+void recip_launch(NbSystem &sys, const float4 *d_xyzq, int64_t *d_force,
+                  int stride, bool energy, bool virial, hipStream_t s);
+void self_launch(NbSystem &sys, const float4 *d_xyzq, hipStream_t s);
+void reduce_energy_launch(NbSystem &sys, int term_mask, bool energy,
+                          bool virial, double *d_energy, double *d_virial,
+                          hipStream_t s);
+
+// Host topology helper shared by the C ABI.
+void build_exclusions(int natom, int nbond, const int32_t *bonds,
+                      std::vector<int32_t> &excl_ptr,
+                      std::vector<int32_t> &excl_idx,
+                      std::vector<int32_t> &in14, std::vector<int32_t> &ex14);
+
+}  // namespace apo

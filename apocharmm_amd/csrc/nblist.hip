@@ -1,0 +1,463 @@
+// Neighbor list: xy-column binning, deterministic z sort per column in LDS,
+// 32-atom cells with bounding boxes, cell-pair tiles inside the list cutoff,
+// per-tile 32x32 exclusion/prune masks and pair-kernel work items.
+//
+// Reference behaviour (re-designed, not translated):
+//   CudaNeighborListSort   src/CudaNeighborListSort.cu:31-716 (columns of edge
+//                          (V*32/N)^(1/3), z-sorted 32-atom cells, bbox)
+//   CudaNeighborListBuild  src/CudaNeighborListBuild.cu:336-969 (cell pairs
+//                          within the list cutoff, 32x32 exclusion masks,
+//                          diagonal tile keeps i<j only)
+// Differences by design: coordinates are 32-bit fixed point per box edge, so
+// every pair uses its minimum image through integer wrap-around and a tile is
+// one cell pair (no 27 image shifts); the column sort orders (z, atom index)
+// so the list -- and every result -- is bitwise reproducible.
+#include <algorithm>
+#include <cmath>
+
+#include "system.hpp"
+
+namespace apo {
+
+__device__ __forceinline__ uint32_t to_fixed(double x, double inv_len) {
+  double f = x * inv_len;
+  f -= floor(f);
+  // [0,1) -> [0, 2^32); 2^32 wraps to 0 which is the same periodic point
+  return (uint32_t)(int64_t)llrint(f * 4294967296.0);
+}
+
+// 1. column of each atom (wrapped xy) and its slot in the column.
+__global__ void k_assign(const float4 *__restrict__ xyzq, int n, double3 inv_len, double3 len,
+                         int ncx, int ncy, int *__restrict__ col_count, int *__restrict__ atom_col,
+                         int *__restrict__ atom_slot, float *__restrict__ zkey,
+                         int *__restrict__ meta) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 p = xyzq[i];
+  double fx = (double)p.x * inv_len.x, fy = (double)p.y * inv_len.y, fz = (double)p.z * inv_len.z;
+  fx -= floor(fx);
+  fy -= floor(fy);
+  fz -= floor(fz);
+  // non-finite input: flag it (the host raises) and keep every index in range
+  if (!(fx >= 0.0 && fx <= 1.0 && fy >= 0.0 && fy <= 1.0 && fz >= 0.0 && fz <= 1.0)) {
+    atomicOr(&meta[6], 1);
+    fx = fy = fz = 0.0;
+  }
+  int cx = max(0, min((int)(fx * ncx), ncx - 1)), cy = max(0, min((int)(fy * ncy), ncy - 1));
+  int col = cx + ncx * cy;
+  atom_col[i] = col;
+  atom_slot[i] = atomicAdd(&col_count[col], 1);
+  zkey[i] = (float)(fz * len.z);
+}
+
+// 2. cells per column; largest column.
+__global__ void k_col_prep(const int *__restrict__ col_count, int ncol, int *__restrict__ col_ncell,
+                           int *__restrict__ meta) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  int cnt = col_count[c];
+  col_ncell[c] = (cnt + kCell - 1) / kCell;
+  atomicMax(&meta[4], cnt);
+}
+
+__global__ void k_copy_total(const int *__restrict__ scan, const int *__restrict__ nptr, int nfix,
+                             int *__restrict__ dst) {
+  if (threadIdx.x == 0) *dst = scan[nptr ? *nptr : nfix];
+}
+
+// 3. scatter atoms into column order (slot order is arbitrary here; step 4
+// makes it deterministic).
+__global__ void k_scatter(int n, const int *__restrict__ atom_col, const int *__restrict__ atom_slot,
+                          const int *__restrict__ col_start, const float *__restrict__ zkey,
+                          unsigned long long *__restrict__ keys) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int pos = col_start[atom_col[i]] + atom_slot[i];
+  // (z, atom) packed so that an unsigned compare orders by z then index;
+  // z >= 0, so its float bits are monotone as unsigned.
+  keys[pos] = ((unsigned long long)__float_as_uint(zkey[i]) << 32) | (unsigned)i;
+}
+
+// 4. per-column bitonic sort in LDS, then cell layout.
+__global__ __launch_bounds__(256) void k_colsort(const unsigned long long *__restrict__ keys,
+                                                 const int *__restrict__ col_count,
+                                                 const int *__restrict__ col_start,
+                                                 const int *__restrict__ col_cellstart,
+                                                 int *__restrict__ loc2glo, int *__restrict__ glo2loc,
+                                                 int *__restrict__ meta) {
+  __shared__ unsigned long long sh[kMaxColumn];
+  const int c = blockIdx.x;
+  const int cnt = col_count[c], st = col_start[c];
+  if (cnt > kMaxColumn) {
+    // leave the column's slots empty (the host raises after the build)
+    if (threadIdx.x == 0) atomicOr(&meta[5], 1);
+    const int cs0 = col_cellstart[c], ncl0 = (cnt + kCell - 1) / kCell;
+    for (int k = threadIdx.x; k < ncl0 * kCell; k += blockDim.x) loc2glo[cs0 * kCell + k] = -1;
+    return;
+  }
+  int P = 1;
+  while (P < cnt) P <<= 1;
+  for (int k = threadIdx.x; k < P; k += blockDim.x) sh[k] = k < cnt ? keys[st + k] : ~0ull;
+  __syncthreads();
+  for (int size = 2; size <= P; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int k = threadIdx.x; k < P; k += blockDim.x) {
+        int partner = k ^ stride;
+        if (partner > k) {
+          bool up = (k & size) == 0;
+          unsigned long long a = sh[k], b = sh[partner];
+          if ((a > b) == up) {
+            sh[k] = b;
+            sh[partner] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const int cs = col_cellstart[c];
+  const int ncl = (cnt + kCell - 1) / kCell;
+  for (int k = threadIdx.x; k < ncl * kCell; k += blockDim.x) {
+    int loc = cs * kCell + k;
+    if (k < cnt) {
+      int a = (int)(unsigned)(sh[k] & 0xffffffffull);
+      loc2glo[loc] = a;
+      glo2loc[a] = loc;
+    } else {
+      loc2glo[loc] = -1;
+    }
+  }
+}
+
+// Per-step gather of xyzq into cell order as 32-bit fixed-point coordinates
+// (setSortedCoords, src/CudaPMEDirectForce.cu:1389-1400), fused with the
+// clear of the sorted force buffer.
+__global__ void k_sort_coords(const float4 *__restrict__ xyzq, const int *__restrict__ vdwtype,
+                              const int *__restrict__ loc2glo, const int *__restrict__ meta,
+                              double3 inv_len, float qscale, int4 *__restrict__ posq,
+                              int *__restrict__ type_s, int64_t *__restrict__ fsort, int ss) {
+  int loc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (loc >= meta[0] * kCell) return;
+  int g = loc2glo[loc];
+  int4 o = make_int4(0, 0, 0, 0);
+  int ty = 0;
+  if (g >= 0) {
+    float4 p = xyzq[g];
+    o.x = (int)to_fixed((double)p.x, inv_len.x);
+    o.y = (int)to_fixed((double)p.y, inv_len.y);
+    o.z = (int)to_fixed((double)p.z, inv_len.z);
+    o.w = __float_as_int(p.w * qscale);
+    ty = vdwtype[g];
+  }
+  posq[loc] = o;
+  type_s[loc] = ty;
+  fsort[loc] = 0;
+  fsort[ss + loc] = 0;
+  fsort[2 * (size_t)ss + loc] = 0;
+}
+
+// 5. cell bounding boxes from the fixed-point coordinates (two cells per wave).
+__global__ __launch_bounds__(64) void k_cellbb(const int4 *__restrict__ posq,
+                                               const int *__restrict__ loc2glo,
+                                               const int *__restrict__ meta, float3 unit,
+                                               float4 *__restrict__ bb) {
+  const int lane = threadIdx.x;
+  const int cell = blockIdx.x * 2 + (lane >> 5);
+  const bool cv = cell < meta[0];
+  const int loc = cell * kCell + (lane & 31);
+  bool valid = cv && loc2glo[loc] >= 0;
+  float x = 0, y = 0, z = 0;
+  if (valid) {
+    int4 p = posq[loc];
+    x = (float)(uint32_t)p.x * unit.x;
+    y = (float)(uint32_t)p.y * unit.y;
+    z = (float)(uint32_t)p.z * unit.z;
+  }
+  float mnx = valid ? x : 3e38f, mny = valid ? y : 3e38f, mnz = valid ? z : 3e38f;
+  float mxx = valid ? x : -3e38f, mxy = valid ? y : -3e38f, mxz = valid ? z : -3e38f;
+  for (int off = 16; off > 0; off >>= 1) {
+    mnx = fminf(mnx, __shfl_xor(mnx, off));
+    mny = fminf(mny, __shfl_xor(mny, off));
+    mnz = fminf(mnz, __shfl_xor(mnz, off));
+    mxx = fmaxf(mxx, __shfl_xor(mxx, off));
+    mxy = fmaxf(mxy, __shfl_xor(mxy, off));
+    mxz = fmaxf(mxz, __shfl_xor(mxz, off));
+  }
+  if (cv && (lane & 31) == 0) {
+    bb[2 * cell] = make_float4(0.5f * (mnx + mxx), 0.5f * (mny + mxy), 0.5f * (mnz + mxz), 0.f);
+    bb[2 * cell + 1] = make_float4(0.5f * (mxx - mnx), 0.5f * (mxy - mny), 0.5f * (mxz - mnz), 0.f);
+  }
+}
+
+__device__ __forceinline__ float bb_gap2(float4 ci, float4 hi, float4 cj, float4 hj, float3 L) {
+  float d2 = 0.f;
+  float d, g;
+  d = ci.x - cj.x; d -= L.x * rintf(d / L.x); g = fmaxf(fabsf(d) - hi.x - hj.x, 0.f); d2 += g * g;
+  d = ci.y - cj.y; d -= L.y * rintf(d / L.y); g = fmaxf(fabsf(d) - hi.y - hj.y, 0.f); d2 += g * g;
+  d = ci.z - cj.z; d -= L.z * rintf(d / L.z); g = fmaxf(fabsf(d) - hi.z - hj.z, 0.f); d2 += g * g;
+  return d2;
+}
+
+// 6. candidate tiles: cell pairs (i <= j) whose boxes are within the list
+// cutoff.  emit = 0 counts per i-cell, emit = 1 writes in j order.
+__global__ __launch_bounds__(64) void k_tiles(const float4 *__restrict__ bb,
+                                              const int *__restrict__ meta, float3 L, float rl2,
+                                              int *__restrict__ icell_cnt,
+                                              const int *__restrict__ icell_start,
+                                              int *__restrict__ cand_i, int *__restrict__ cand_j,
+                                              int emit) {
+  const int icell = blockIdx.x;
+  const int ncell = meta[0];
+  if (icell >= ncell) return;
+  const int lane = threadIdx.x;
+  const float4 ci = bb[2 * icell], hi = bb[2 * icell + 1];
+  int count = 0;
+  const int base = emit ? icell_start[icell] : 0;
+  for (int j0 = icell; j0 < ncell; j0 += 64) {
+    int j = j0 + lane;
+    bool hit = false;
+    if (j < ncell) hit = bb_gap2(ci, hi, bb[2 * j], bb[2 * j + 1], L) < rl2;
+    unsigned long long m = __ballot(hit);
+    if (emit && hit) {
+      int pos = base + count + __popcll(m & ((1ull << lane) - 1ull));
+      cand_i[pos] = icell;
+      cand_j[pos] = j;
+    }
+    count += __popcll(m);
+  }
+  if (!emit && lane == 0) icell_cnt[icell] = count;
+}
+
+// 7. 32x32 masks: bit t of lane l set <=> pair (i-cell atom t, j-cell atom l)
+// is skipped: padding, diagonal i >= j, topological exclusion (1-2/1-3/1-4),
+// or farther than the list cutoff at build time.
+__global__ __launch_bounds__(64) void k_tile_mask(
+    const int *__restrict__ cand_i, const int *__restrict__ cand_j, int ncand,
+    const int4 *__restrict__ posq, const int *__restrict__ loc2glo, const int *__restrict__ glo2loc,
+    const int *__restrict__ excl_ptr, const int *__restrict__ excl_idx, float3 unit, float rl2,
+    uint32_t *__restrict__ cand_mask, int *__restrict__ cand_active) {
+  const int lane = threadIdx.x, l = lane & 31;
+  const int tile = blockIdx.x * 2 + (lane >> 5);
+  const bool tv = tile < ncand;
+  uint32_t m = 0xffffffffu;
+  if (tv) {
+    const int ic = cand_i[tile], jc = cand_j[tile];
+    const int jl = jc * kCell + l;
+    const int gj = loc2glo[jl];
+    if (gj >= 0) {
+      m = 0u;
+      const int4 pj = posq[jl];
+      for (int t = 0; t < kCell; t++) {
+        const int il = ic * kCell + t;
+        bool bad = loc2glo[il] < 0 || (ic == jc && t >= l);
+        if (!bad) {
+          const int4 pi = posq[il];
+          float dx = (float)(int)(pi.x - pj.x) * unit.x;
+          float dy = (float)(int)(pi.y - pj.y) * unit.y;
+          float dz = (float)(int)(pi.z - pj.z) * unit.z;
+          bad = dx * dx + dy * dy + dz * dz > rl2;
+        }
+        if (bad) m |= 1u << t;
+      }
+      for (int e = excl_ptr[gj]; e < excl_ptr[gj + 1]; e++) {
+        int lp = glo2loc[excl_idx[e]];
+        if ((lp >> 5) == ic) m |= 1u << (lp & 31);
+      }
+    }
+    cand_mask[(size_t)tile * kCell + l] = m;
+  }
+  unsigned long long b = __ballot(tv && m != 0xffffffffu);
+  unsigned hb = (lane < 32) ? (unsigned)b : (unsigned)(b >> 32);
+  if (tv && l == 0) cand_active[tile] = hb != 0u;
+}
+
+// 8. active tiles per i-cell and work items of <= kItemTiles tiles.
+__global__ __launch_bounds__(64) void k_active_count(const int *__restrict__ meta,
+                                                     const int *__restrict__ icell_start,
+                                                     const int *__restrict__ cand_active,
+                                                     int *__restrict__ icell_act,
+                                                     int *__restrict__ icell_nitem) {
+  const int icell = blockIdx.x;
+  if (icell >= meta[0]) return;
+  const int s = icell_start[icell], e = icell_start[icell + 1];
+  int c = 0;
+  for (int k = s + threadIdx.x; k < e; k += 64) c += cand_active[k];
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  if (threadIdx.x == 0) {
+    icell_act[icell] = c;
+    icell_nitem[icell] = (c + kItemTiles - 1) / kItemTiles;
+  }
+}
+
+// 9. compaction into the final tile arrays and items (deterministic order).
+__global__ __launch_bounds__(64) void k_compact(
+    const int *__restrict__ meta, const int *__restrict__ icell_start,
+    const int *__restrict__ icell_actstart, const int *__restrict__ icell_itemstart,
+    const int *__restrict__ cand_j, const uint32_t *__restrict__ cand_mask,
+    const int *__restrict__ cand_active, int *__restrict__ tile_j, uint32_t *__restrict__ tile_mask,
+    int4 *__restrict__ items) {
+  const int icell = blockIdx.x;
+  if (icell >= meta[0]) return;
+  const int lane = threadIdx.x;
+  const int s = icell_start[icell], e = icell_start[icell + 1];
+  const int a0 = icell_actstart[icell], a1 = icell_actstart[icell + 1];
+  int count = 0;
+  for (int k0 = s; k0 < e; k0 += 64) {
+    int k = k0 + lane;
+    bool act = k < e && cand_active[k];
+    unsigned long long m = __ballot(act);
+    if (act) {
+      int pos = a0 + count + __popcll(m & ((1ull << lane) - 1ull));
+      tile_j[pos] = cand_j[k];
+      for (int w = 0; w < kCell; w++)
+        tile_mask[(size_t)pos * kCell + w] = cand_mask[(size_t)k * kCell + w];
+    }
+    count += __popcll(m);
+  }
+  const int i0 = icell_itemstart[icell], i1 = icell_itemstart[icell + 1];
+  for (int it = i0 + lane; it < i1; it += 64) {
+    int ts = a0 + (it - i0) * kItemTiles;
+    items[it] = make_int4(icell, ts, min(ts + kItemTiles, a1), 0);
+  }
+}
+
+static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
+
+void nblist_build(NbSystem &S, const float4 *x, hipStream_t s) {
+  const int n = S.natom;
+  const double3 inv = make_double3(1.0 / S.box[0], 1.0 / S.box[1], 1.0 / S.box[2]);
+  const double3 len = make_double3(S.box[0], S.box[1], S.box[2]);
+  const float3 unit = make_float3((float)(S.box[0] / 4294967296.0), (float)(S.box[1] / 4294967296.0),
+                                  (float)(S.box[2] / 4294967296.0));
+  const float3 Lf = make_float3((float)S.box[0], (float)S.box[1], (float)S.box[2]);
+  const float rl2 = (float)(S.cutnb * S.cutnb);
+  int *meta = S.meta.get();
+
+  launch_fill_i32(meta, 16, s);
+  launch_fill_i32(S.col_count.get(), S.ncol + 1, s);
+  k_assign<<<nblk(n, 256), 256, 0, s>>>(x, n, inv, len, S.ncx, S.ncy, S.col_count.get(),
+                                         S.atom_col.get(), S.atom_slot.get(), S.zkey.get(), meta);
+  APO_LAUNCH_CHECK();
+  launch_scan(S.col_count.get(), S.col_start.get(), nullptr, S.ncol, s);
+  k_col_prep<<<nblk(S.ncol, 256), 256, 0, s>>>(S.col_count.get(), S.ncol, S.col_ncell.get(), meta);
+  APO_LAUNCH_CHECK();
+  launch_scan(S.col_ncell.get(), S.col_cellstart.get(), nullptr, S.ncol, s);
+  k_copy_total<<<1, 64, 0, s>>>(S.col_cellstart.get(), nullptr, S.ncol, meta + 0);
+  APO_LAUNCH_CHECK();
+  unsigned long long *kb = S.colkeys.get();
+  k_scatter<<<nblk(n, 256), 256, 0, s>>>(n, S.atom_col.get(), S.atom_slot.get(), S.col_start.get(),
+                                          S.zkey.get(), kb);
+  APO_LAUNCH_CHECK();
+  k_colsort<<<S.ncol, 256, 0, s>>>(kb, S.col_count.get(), S.col_start.get(), S.col_cellstart.get(),
+                                   S.loc2glo.get(), S.glo2loc.get(), meta);
+  APO_LAUNCH_CHECK();
+  nblist_sort_coords(S, x, s);
+  k_cellbb<<<nblk(S.ncell_cap, 2), 64, 0, s>>>(S.posq.get(), S.loc2glo.get(), meta, unit,
+                                                S.cell_bb.get());
+  APO_LAUNCH_CHECK();
+  k_tiles<<<S.ncell_cap, 64, 0, s>>>(S.cell_bb.get(), meta, Lf, rl2, S.icell_cnt.get(), nullptr,
+                                     nullptr, nullptr, 0);
+  APO_LAUNCH_CHECK();
+  launch_scan(S.icell_cnt.get(), S.icell_start.get(), meta + 0, S.ncell_cap, s);
+  k_copy_total<<<1, 64, 0, s>>>(S.icell_start.get(), meta + 0, 0, meta + 1);
+  APO_LAUNCH_CHECK();
+  int hmeta[8];
+  APO_HIP(hipMemcpyAsync(hmeta, meta, sizeof(hmeta), hipMemcpyDeviceToHost, s));
+  APO_HIP(hipStreamSynchronize(s));
+  if (hmeta[6]) {
+    S.list_valid = false;
+    throw RuntimeError("non-finite coordinates in xyzq");
+  }
+  if (hmeta[5]) {
+    S.list_valid = false;
+    throw RuntimeError("neighbor list: an xy column exceeds the LDS sort capacity");
+  }
+  S.host_ncell = hmeta[0];
+  S.host_ncand = hmeta[1];
+  if ((size_t)S.host_ncand > S.cand_cap) {
+    S.cand_cap = (size_t)(S.host_ncand * 1.25) + 64;
+    S.cand_i.reserve(S.cand_cap);
+    S.cand_j.reserve(S.cand_cap);
+    S.cand_mask.reserve(S.cand_cap * kCell);
+    S.cand_active.reserve(S.cand_cap);
+  }
+  k_tiles<<<S.ncell_cap, 64, 0, s>>>(S.cell_bb.get(), meta, Lf, rl2, nullptr, S.icell_start.get(),
+                                     S.cand_i.get(), S.cand_j.get(), 1);
+  APO_LAUNCH_CHECK();
+  if (S.host_ncand > 0) {
+    k_tile_mask<<<nblk(S.host_ncand, 2), 64, 0, s>>>(
+        S.cand_i.get(), S.cand_j.get(), S.host_ncand, S.posq.get(), S.loc2glo.get(),
+        S.glo2loc.get(), S.excl_ptr.get(), S.excl_idx.get(), unit, rl2, S.cand_mask.get(),
+        S.cand_active.get());
+    APO_LAUNCH_CHECK();
+  }
+  k_active_count<<<S.ncell_cap, 64, 0, s>>>(meta, S.icell_start.get(), S.cand_active.get(),
+                                            S.icell_act.get(), S.icell_nitem.get());
+  APO_LAUNCH_CHECK();
+  launch_scan(S.icell_act.get(), S.icell_actstart.get(), meta + 0, S.ncell_cap, s);
+  launch_scan(S.icell_nitem.get(), S.icell_itemstart.get(), meta + 0, S.ncell_cap, s);
+  k_copy_total<<<1, 64, 0, s>>>(S.icell_actstart.get(), meta + 0, 0, meta + 2);
+  k_copy_total<<<1, 64, 0, s>>>(S.icell_itemstart.get(), meta + 0, 0, meta + 3);
+  APO_LAUNCH_CHECK();
+  APO_HIP(hipMemcpyAsync(hmeta, meta, sizeof(hmeta), hipMemcpyDeviceToHost, s));
+  APO_HIP(hipStreamSynchronize(s));
+  S.host_ntile = hmeta[2];
+  S.host_nitem = hmeta[3];
+  if ((size_t)S.host_ntile > S.tile_cap) {
+    S.tile_cap = (size_t)(S.host_ntile * 1.25) + 64;
+    S.tile_j.reserve(S.tile_cap);
+    S.tile_mask.reserve(S.tile_cap * kCell);
+  }
+  if ((size_t)S.host_nitem > S.item_cap) {
+    S.item_cap = (size_t)(S.host_nitem * 1.25) + 64;
+    S.items.reserve(S.item_cap);
+  }
+  k_compact<<<S.ncell_cap, 64, 0, s>>>(meta, S.icell_start.get(), S.icell_actstart.get(),
+                                       S.icell_itemstart.get(), S.cand_j.get(), S.cand_mask.get(),
+                                       S.cand_active.get(), S.tile_j.get(), S.tile_mask.get(),
+                                       S.items.get());
+  APO_LAUNCH_CHECK();
+  S.list_valid = true;
+}
+
+void nblist_sort_coords(NbSystem &S, const float4 *x, hipStream_t s) {
+  const double3 inv = make_double3(1.0 / S.box[0], 1.0 / S.box[1], 1.0 / S.box[2]);
+  float qscale = S.precision == 1 ? 1.0f : (float)std::sqrt(kCcelec);
+  k_sort_coords<<<nblk(S.ssort, 256), 256, 0, s>>>(x, S.vdwtype.get(), S.loc2glo.get(),
+                                                    S.meta.get(), inv, qscale, S.posq.get(),
+                                                    S.type_s.get(), S.fsort.get(), S.ssort);
+  APO_LAUNCH_CHECK();
+}
+
+// Pairs inside the list cutoff covered by the tiles (unmasked bits).
+__global__ void k_count_pairs(const uint32_t *__restrict__ tile_mask, int ntile,
+                              unsigned long long *__restrict__ out) {
+  size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  unsigned c = 0;
+  if (k < (size_t)ntile * kCell) c = __popc(~tile_mask[k]);
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, (unsigned long long)c);
+}
+
+void nblist_stats(NbSystem &S, int64_t *out6, hipStream_t s) {
+  DevBuf<unsigned long long> cnt;
+  cnt.reserve(1);
+  APO_HIP(hipMemsetAsync(cnt.get(), 0, sizeof(unsigned long long), s));
+  if (S.list_valid && S.host_ntile > 0)
+    k_count_pairs<<<nblk((long)S.host_ntile * kCell, 256), 256, 0, s>>>(S.tile_mask.get(),
+                                                                       S.host_ntile, cnt.get());
+  APO_LAUNCH_CHECK();
+  unsigned long long pairs = 0;
+  int hmeta[8];
+  APO_HIP(hipMemcpyAsync(&pairs, cnt.get(), sizeof(pairs), hipMemcpyDeviceToHost, s));
+  APO_HIP(hipMemcpyAsync(hmeta, S.meta.get(), sizeof(hmeta), hipMemcpyDeviceToHost, s));
+  APO_HIP(hipStreamSynchronize(s));
+  out6[0] = hmeta[0];
+  out6[1] = hmeta[2];
+  out6[2] = hmeta[3];
+  out6[3] = (int64_t)pairs;
+  out6[4] = S.ncol;
+  out6[5] = hmeta[4];
+}
+
+}  // namespace apo

@@ -1,0 +1,555 @@
+// Smooth PME reciprocal space on gfx950.
+//
+// Physics and grid conventions of CudaPMERecip (spread
+// include/cuda_pme_spread.h:24-142, scalar_sum_ortho_kernel
+// src/CudaPMERecip.cu:673-929, gather include/cuda_pme_gather.h:18-320,
+// order-4 B-splines src/CudaPMERecip.cu:290-465):
+//   fractional coordinate fr = K (w - (floor(w + 1/2) - 1/2)), w = x/L + 2,
+//   grid point ix + t carries theta[t], x fastest in memory.
+// gfx950 design:
+//   * fr is formed in FP64 per atom (one conversion), B-splines in CT;
+//   * the charge grid is int64 fixed point (2^-40): order-independent sums;
+//   * the 3-D R2C/C2R FFT is five LDS passes of a hand-written mixed-radix
+//     (4, 2, 3, 5, 7, 11, 13) Stockham FFT: x (two real rows packed per complex
+//     transform), y, [z forward -> k-space convolution + energy/virial -> z
+//     inverse] fused in one kernel, y inverse, x inverse;
+//   * gather uses 16 lanes per atom and writes int64 2^-40 gradients.
+#include <algorithm>
+#include <cmath>
+
+#include "system.hpp"
+
+namespace apo {
+
+template <typename CT> struct Cx;
+template <> struct Cx<float> { using T = float2; };
+template <> struct Cx<double> { using T = double2; };
+
+struct FftPlan {
+  int n, nrad;
+  int rad[16];
+};
+
+static FftPlan plan_of(const FftDim &d) {
+  FftPlan p;
+  p.n = d.n;
+  p.nrad = d.nrad;
+  for (int i = 0; i < 16; i++) p.rad[i] = d.rad[i];
+  return p;
+}
+
+template <typename C2> __device__ __forceinline__ C2 cmul(C2 a, C2 b) {
+  C2 r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+template <typename C2> __device__ __forceinline__ C2 cadd(C2 a, C2 b) {
+  C2 r;
+  r.x = a.x + b.x;
+  r.y = a.y + b.y;
+  return r;
+}
+template <typename C2> __device__ __forceinline__ C2 csub(C2 a, C2 b) {
+  C2 r;
+  r.x = a.x - b.x;
+  r.y = a.y - b.y;
+  return r;
+}
+template <typename C2> __device__ __forceinline__ C2 twid(const C2 *tw, int e, bool inv) {
+  C2 w = tw[e];
+  if (inv) w.y = -w.y;
+  return w;
+}
+
+// Small DFT of length R on registers (forward: e^{-2 pi i r q / R}).
+template <typename C2, int R>
+__device__ __forceinline__ void small_dft(C2 (&v)[R], const C2 *tw, int n, bool inv) {
+  if constexpr (R == 2) {
+    C2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    C2 a = v[0], b = v[1], c = v[2], d = v[3];
+    C2 s0 = cadd(a, c), s1 = csub(a, c), s2 = cadd(b, d), s3 = csub(b, d);
+    // -i*s3 (forward) or +i*s3 (inverse)
+    C2 js3;
+    if (inv) { js3.x = -s3.y; js3.y = s3.x; } else { js3.x = s3.y; js3.y = -s3.x; }
+    v[0] = cadd(s0, s2);
+    v[2] = csub(s0, s2);
+    v[1] = cadd(s1, js3);
+    v[3] = csub(s1, js3);
+  } else {
+    C2 o[R];
+    const int st = n / R;
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+      C2 s = v[0];
+#pragma unroll
+      for (int r = 1; r < R; r++) s = cadd(s, cmul(v[r], twid(tw, ((r * q) % R) * st, inv)));
+      o[q] = s;
+    }
+#pragma unroll
+    for (int q = 0; q < R; q++) v[q] = o[q];
+  }
+}
+
+// One Stockham autosort stage of radix R over nrow rows of length n.
+template <typename C2, int R>
+__device__ __forceinline__ void fft_stage(const C2 *in, C2 *out, int n, int nrow, int Ns,
+                                          const C2 *tw, bool inv) {
+  const int nbf = n / R;
+  const int step = n / (Ns * R);
+  for (int w = threadIdx.x; w < nbf * nrow; w += blockDim.x) {
+    const int row = w / nbf, j = w - row * nbf;
+    const C2 *ri = in + row * n;
+    C2 *ro = out + row * n;
+    const int k = j % Ns;
+    C2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) v[r] = ri[j + r * nbf];
+#pragma unroll
+    for (int r = 1; r < R; r++) v[r] = cmul(v[r], twid(tw, r * k * step, inv));
+    small_dft<C2, R>(v, tw, n, inv);
+    const int idxD = (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; r++) ro[idxD + r * Ns] = v[r];
+  }
+}
+
+// nrow unnormalised FFTs of length p.n held in LDS buffer a (b = scratch).
+// Returns the buffer holding the result.
+template <typename C2>
+__device__ C2 *lds_fft(C2 *a, C2 *b, const FftPlan &p, int nrow, const C2 *tw, bool inv) {
+  int Ns = 1;
+  for (int s = 0; s < p.nrad; s++) {
+    const int R = p.rad[s];
+    switch (R) {
+      case 2: fft_stage<C2, 2>(a, b, p.n, nrow, Ns, tw, inv); break;
+      case 3: fft_stage<C2, 3>(a, b, p.n, nrow, Ns, tw, inv); break;
+      case 4: fft_stage<C2, 4>(a, b, p.n, nrow, Ns, tw, inv); break;
+      case 5: fft_stage<C2, 5>(a, b, p.n, nrow, Ns, tw, inv); break;
+      case 7: fft_stage<C2, 7>(a, b, p.n, nrow, Ns, tw, inv); break;
+      case 11: fft_stage<C2, 11>(a, b, p.n, nrow, Ns, tw, inv); break;
+      default: fft_stage<C2, 13>(a, b, p.n, nrow, Ns, tw, inv); break;
+    }
+    __syncthreads();
+    C2 *t = a;
+    a = b;
+    b = t;
+    Ns *= R;
+  }
+  return a;
+}
+
+// ---------------------------------------------------------------------------
+// B-splines and fractional coordinates
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void frac_coord(float x, double inv_len, int K, int &ix, double &w) {
+  const double ww = (double)x * inv_len + 2.0;
+  double fr = (double)K * (ww - (floor(ww + 0.5) - 0.5));
+  // fr = K is the periodic image of 0; non-finite input maps to 0 so every
+  // grid index stays in range
+  if (!(fr >= 0.0 && fr < (double)K)) fr = 0.0;
+  ix = (int)fr;
+  w = fr - (double)ix;
+}
+
+// Order-4 Cardinal B-spline weights (and derivatives) at offset w.
+template <typename CT>
+__device__ __forceinline__ void bspline4(CT w, CT (&th)[4], CT (&dth)[4]) {
+  th[3] = 0;
+  th[1] = w;
+  th[0] = (CT)1 - w;
+  th[2] = (CT)0.5 * w * th[1];
+  th[1] = (CT)0.5 * ((w + (CT)1) * th[0] + ((CT)2 - w) * th[1]);
+  th[0] = (CT)0.5 * ((CT)1 - w) * th[0];
+  dth[0] = -th[0];
+  dth[1] = th[0] - th[1];
+  dth[2] = th[1] - th[2];
+  dth[3] = th[2] - th[3];
+  const CT third = (CT)1 / (CT)3;
+  th[3] = third * w * th[2];
+  th[2] = third * ((w + (CT)1) * th[1] + ((CT)3 - w) * th[2]);
+  th[1] = third * ((w + (CT)2) * th[0] + ((CT)2 - w) * th[1]);
+  th[0] = third * ((CT)1 - w) * th[0];
+}
+
+struct GridGeom {
+  int K0, K1, K2;
+  double inv0, inv1, inv2;
+};
+
+// Spread: 16 lanes per atom, each lane one (y, z) row of 4 x points.
+template <typename CT>
+__global__ __launch_bounds__(256) void k_spread(const float4 *__restrict__ xyzq, int n, GridGeom g,
+                                                unsigned long long *__restrict__ grid) {
+  const int gt = blockIdx.x * 256 + threadIdx.x;
+  const int atom = gt >> 4, row = gt & 15;
+  if (atom >= n) return;
+  const float4 p = xyzq[atom];
+  int ix, iy, iz;
+  double wx, wy, wz;
+  frac_coord(p.x, g.inv0, g.K0, ix, wx);
+  frac_coord(p.y, g.inv1, g.K1, iy, wy);
+  frac_coord(p.z, g.inv2, g.K2, iz, wz);
+  CT tx[4], ty[4], tz[4], d[4];
+  bspline4<CT>((CT)wx, tx, d);
+  bspline4<CT>((CT)wy, ty, d);
+  bspline4<CT>((CT)wz, tz, d);
+  const int a = row & 3, b = row >> 2;
+  int y = iy + a, z = iz + b;
+  if (y >= g.K1) y -= g.K1;
+  if (z >= g.K2) z -= g.K2;
+  const CT qyz = (CT)p.w * ty[a] * tz[b];
+  const size_t base = (size_t)g.K0 * ((size_t)y + (size_t)g.K1 * z);
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    int x = ix + c;
+    if (x >= g.K0) x -= g.K0;
+    const CT v = qyz * tx[c];
+    atomicAdd(&grid[base + x],
+              (unsigned long long)(long long)__builtin_rint((double)v * kGridScale));
+  }
+}
+
+// x forward: int64 rows -> complex half rows, two real rows per complex FFT.
+template <typename CT>
+__global__ __launch_bounds__(256) void k_fftx_fwd(const long long *__restrict__ grid, int nrows,
+                                                  FftPlan p, int rb,
+                                                  const typename Cx<CT>::T *__restrict__ twg,
+                                                  typename Cx<CT>::T *__restrict__ out) {
+  using C2 = typename Cx<CT>::T;
+  extern __shared__ char smem[];
+  const int n = p.n, H = n / 2 + 1;
+  C2 *tw = reinterpret_cast<C2 *>(smem);
+  C2 *a = tw + n;
+  C2 *b = a + rb * n;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) tw[e] = twg[e];
+  const int r0 = blockIdx.x * rb;  // first row pair
+  for (int e = threadIdx.x; e < rb * n; e += blockDim.x) {
+    const int r = e / n, x = e - r * n;
+    const int ra = 2 * (r0 + r), rbw = ra + 1;
+    C2 v;
+    v.x = ra < nrows ? (CT)((double)grid[(size_t)ra * n + x] * (1.0 / kGridScale)) : (CT)0;
+    v.y = rbw < nrows ? (CT)((double)grid[(size_t)rbw * n + x] * (1.0 / kGridScale)) : (CT)0;
+    a[e] = v;
+  }
+  __syncthreads();
+  C2 *res = lds_fft<C2>(a, b, p, rb, tw, false);
+  for (int e = threadIdx.x; e < rb * H; e += blockDim.x) {
+    const int r = e / H, k = e - r * H;
+    const int ra = 2 * (r0 + r), rbw = ra + 1;
+    const C2 Z = res[r * n + k], Zm = res[r * n + (n - k) % n];
+    C2 A, B;
+    A.x = (CT)0.5 * (Z.x + Zm.x);
+    A.y = (CT)0.5 * (Z.y - Zm.y);
+    B.x = (CT)0.5 * (Z.y + Zm.y);
+    B.y = (CT)0.5 * (Zm.x - Z.x);
+    if (ra < nrows) out[(size_t)ra * H + k] = A;
+    if (rbw < nrows) out[(size_t)rbw * H + k] = B;
+  }
+}
+
+// x inverse: complex half rows -> real rows (C2R semantics: the imaginary
+// parts of the k = 0 and Nyquist bins are ignored).
+template <typename CT>
+__global__ __launch_bounds__(256) void k_fftx_inv(const typename Cx<CT>::T *__restrict__ in,
+                                                  int nrows, FftPlan p, int rb,
+                                                  const typename Cx<CT>::T *__restrict__ twg,
+                                                  CT *__restrict__ phi) {
+  using C2 = typename Cx<CT>::T;
+  extern __shared__ char smem[];
+  const int n = p.n, H = n / 2 + 1;
+  C2 *tw = reinterpret_cast<C2 *>(smem);
+  C2 *a = tw + n;
+  C2 *b = a + rb * n;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) tw[e] = twg[e];
+  const int r0 = blockIdx.x * rb;
+  for (int e = threadIdx.x; e < rb * n; e += blockDim.x) {
+    const int r = e / n, k = e - r * n;
+    const int ra = 2 * (r0 + r), rbw = ra + 1;
+    const bool mirror = k > n / 2;
+    const int kk = mirror ? n - k : k;
+    C2 A = {0, 0}, B = {0, 0};
+    if (ra < nrows) A = in[(size_t)ra * H + kk];
+    if (rbw < nrows) B = in[(size_t)rbw * H + kk];
+    if (kk == 0 || 2 * kk == n) {
+      A.y = 0;
+      B.y = 0;
+    }
+    if (mirror) {
+      A.y = -A.y;
+      B.y = -B.y;
+    }
+    C2 Z;
+    Z.x = A.x - B.y;
+    Z.y = A.y + B.x;
+    a[e] = Z;
+  }
+  __syncthreads();
+  C2 *res = lds_fft<C2>(a, b, p, rb, tw, true);
+  for (int e = threadIdx.x; e < rb * n; e += blockDim.x) {
+    const int r = e / n, x = e - r * n;
+    const int ra = 2 * (r0 + r), rbw = ra + 1;
+    if (ra < nrows) phi[(size_t)ra * n + x] = res[e].x;
+    if (rbw < nrows) phi[(size_t)rbw * n + x] = res[e].y;
+  }
+}
+
+// y pass (either direction): block = (z, chunk of W kx columns).
+template <typename CT>
+__global__ __launch_bounds__(256) void k_ffty(typename Cx<CT>::T *__restrict__ data, int H, int K1,
+                                              int W, FftPlan p,
+                                              const typename Cx<CT>::T *__restrict__ twg, int inv) {
+  using C2 = typename Cx<CT>::T;
+  extern __shared__ char smem[];
+  const int n = p.n;
+  C2 *tw = reinterpret_cast<C2 *>(smem);
+  C2 *a = tw + n;
+  C2 *b = a + W * n;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) tw[e] = twg[e];
+  const int nchunk = (H + W - 1) / W;
+  const int z = blockIdx.x / nchunk, kx0 = (blockIdx.x % nchunk) * W;
+  const int wc = min(W, H - kx0);
+  for (int e = threadIdx.x; e < W * n; e += blockDim.x) {
+    const int y = e / W, c = e - y * W;
+    if (c < wc) a[c * n + y] = data[((size_t)z * K1 + y) * H + kx0 + c];
+  }
+  __syncthreads();
+  C2 *res = lds_fft<C2>(a, b, p, wc, tw, inv != 0);
+  for (int e = threadIdx.x; e < W * n; e += blockDim.x) {
+    const int y = e / W, c = e - y * W;
+    if (c < wc) data[((size_t)z * K1 + y) * H + kx0 + c] = res[c * n + y];
+  }
+}
+
+struct ConvArgs {
+  int K0, K1, K2, H, W;
+  double r0, r1, r2;   // 1/L per axis
+  double fac, piv_inv;  // pi^2/kappa^2, 1/(pi V)
+  double half_ccelec;
+};
+
+template <typename CT> __device__ __forceinline__ CT exp_k(CT x);
+template <> __device__ __forceinline__ float exp_k<float>(float x) { return expf(x); }
+template <> __device__ __forceinline__ double exp_k<double>(double x) { return exp(x); }
+
+// z pass with the k-space convolution in the middle: block = (y, W columns).
+template <typename CT, bool CALC_EV>
+__global__ __launch_bounds__(256) void k_fftz_conv(typename Cx<CT>::T *__restrict__ data, ConvArgs c,
+                                                   FftPlan p,
+                                                   const typename Cx<CT>::T *__restrict__ twg,
+                                                   const CT *__restrict__ prefac,
+                                                   double *__restrict__ part) {
+  using C2 = typename Cx<CT>::T;
+  extern __shared__ char smem[];
+  const int n = p.n, H = c.H, W = c.W;
+  C2 *tw = reinterpret_cast<C2 *>(smem);
+  C2 *a = tw + n;
+  C2 *b = a + W * n;
+  CT *pre = reinterpret_cast<CT *>(b + W * n);
+  __shared__ double red[4 * 8];
+  for (int e = threadIdx.x; e < n; e += blockDim.x) tw[e] = twg[e];
+  for (int e = threadIdx.x; e < c.K0 + c.K1 + c.K2; e += blockDim.x) pre[e] = prefac[e];
+  const int nchunk = (H + W - 1) / W;
+  const int y = blockIdx.x / nchunk, kx0 = (blockIdx.x % nchunk) * W;
+  const int wc = min(W, H - kx0);
+  const size_t zs = (size_t)c.K1 * H;
+  for (int e = threadIdx.x; e < W * n; e += blockDim.x) {
+    const int z = e / W, col = e - z * W;
+    if (col < wc) a[col * n + z] = data[(size_t)z * zs + (size_t)y * H + kx0 + col];
+  }
+  __syncthreads();
+  C2 *res = lds_fft<C2>(a, b, p, wc, tw, false);
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  const int nf0 = c.K0 / 2 + (c.K0 % 2), nf1 = c.K1 / 2 + (c.K1 % 2), nf2 = c.K2 / 2 + (c.K2 % 2);
+  for (int e = threadIdx.x; e < wc * n; e += blockDim.x) {
+    const int col = e / n, z = e - col * n;
+    const int kx = kx0 + col;
+    C2 q = res[e];
+    if (kx == 0 && y == 0 && z == 0) {
+      q.x = 0;
+      q.y = 0;
+    } else {
+      const int m0 = kx >= nf0 ? kx - c.K0 : kx;
+      const int m1 = y >= nf1 ? y - c.K1 : y;
+      const int m2 = z >= nf2 ? z - c.K2 : z;
+      const CT mh0 = (CT)c.r0 * (CT)m0, mh1 = (CT)c.r1 * (CT)m1, mh2 = (CT)c.r2 * (CT)m2;
+      const CT msq = mh0 * mh0 + mh1 * mh1 + mh2 * mh2;
+      const CT msq_inv = (CT)1 / msq;
+      const CT eterm = exp_k<CT>(-(CT)c.fac * msq) * (CT)c.piv_inv * pre[kx] *
+                       pre[c.K0 + y] * pre[c.K0 + c.K1 + z] * msq_inv;
+      if (CALC_EV) {
+        // the x half grid stands for both Hermitian halves: every kx >= 1
+        // column counts twice (the reference's mirror bookkeeping)
+        const CT mult = kx >= 1 ? (CT)2 : (CT)1;
+        const CT t1 = mult * eterm * (q.x * q.x + q.y * q.y);
+        const CT vterm = (CT)2 * ((CT)c.fac + msq_inv);
+        const CT t2 = t1 * vterm;
+        acc[0] += (double)t1;
+        acc[1] += (double)(t1 * (vterm * mh0 * mh0 - (CT)1));
+        acc[2] += (double)(t2 * mh0 * mh1);
+        acc[3] += (double)(t2 * mh0 * mh2);
+        acc[4] += (double)(t1 * (vterm * mh1 * mh1 - (CT)1));
+        acc[5] += (double)(t2 * mh1 * mh2);
+        acc[6] += (double)(t1 * (vterm * mh2 * mh2 - (CT)1));
+      }
+      q.x *= eterm;
+      q.y *= eterm;
+    }
+    res[e] = q;
+  }
+  __syncthreads();
+  C2 *other = res == a ? b : a;
+  C2 *out = lds_fft<C2>(res, other, p, wc, tw, true);
+  for (int e = threadIdx.x; e < W * n; e += blockDim.x) {
+    const int z = e / W, col = e - z * W;
+    if (col < wc) data[(size_t)z * zs + (size_t)y * H + kx0 + col] = out[col * n + z];
+  }
+  if (CALC_EV) {
+    for (int k = 0; k < 7; k++) {
+      double v = acc[k];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+      if ((threadIdx.x & 63) == 0) red[(threadIdx.x >> 6) * 8 + k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 7) {
+      double s = 0;
+      for (int w = 0; w < 4; w++) s += red[w * 8 + threadIdx.x];
+      // energy +1/2 ccelec sum, virial -1/2 ccelec sum
+      part[blockIdx.x * kRecipPart + threadIdx.x] =
+          threadIdx.x == 0 ? s * c.half_ccelec : -s * c.half_ccelec;
+    }
+  }
+}
+
+// Gather: 16 lanes per atom; gradient q ccelec (K/L) sum dtheta theta theta phi.
+template <typename CT>
+__global__ __launch_bounds__(256) void k_gather(const float4 *__restrict__ xyzq, int n, GridGeom g,
+                                                const CT *__restrict__ phi,
+                                                unsigned long long *__restrict__ f, int stride) {
+  const int gt = blockIdx.x * 256 + threadIdx.x;
+  const int atom = gt >> 4, row = gt & 15;
+  const bool valid = atom < n;
+  float4 p = valid ? xyzq[atom] : make_float4(0, 0, 0, 0);
+  int ix, iy, iz;
+  double wx, wy, wz;
+  frac_coord(p.x, g.inv0, g.K0, ix, wx);
+  frac_coord(p.y, g.inv1, g.K1, iy, wy);
+  frac_coord(p.z, g.inv2, g.K2, iz, wz);
+  CT tx[4], ty[4], tz[4], dx[4], dy[4], dz[4];
+  bspline4<CT>((CT)wx, tx, dx);
+  bspline4<CT>((CT)wy, ty, dy);
+  bspline4<CT>((CT)wz, tz, dz);
+  const int a = row & 3, b = row >> 2;
+  int y = iy + a, z = iz + b;
+  if (y >= g.K1) y -= g.K1;
+  if (z >= g.K2) z -= g.K2;
+  CT f1 = 0, f2 = 0, f3 = 0;
+  if (valid) {
+    const size_t base = (size_t)g.K0 * ((size_t)y + (size_t)g.K1 * z);
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      int x = ix + c;
+      if (x >= g.K0) x -= g.K0;
+      const CT v = phi[base + x];
+      f1 += dx[c] * ty[a] * tz[b] * v;
+      f2 += tx[c] * dy[a] * tz[b] * v;
+      f3 += tx[c] * ty[a] * dz[b] * v;
+    }
+  }
+#pragma unroll
+  for (int off = 1; off < 16; off <<= 1) {
+    f1 += __shfl_xor(f1, off);
+    f2 += __shfl_xor(f2, off);
+    f3 += __shfl_xor(f3, off);
+  }
+  if (valid && row == 0) {
+    const double q = (double)p.w * kCcelec;
+    const double gx = q * g.inv0 * g.K0 * (double)f1;
+    const double gy = q * g.inv1 * g.K1 * (double)f2;
+    const double gz = q * g.inv2 * g.K2 * (double)f3;
+    atomicAdd(&f[atom], (unsigned long long)(long long)__builtin_rint(gx * kForceScale));
+    atomicAdd(&f[(size_t)stride + atom],
+              (unsigned long long)(long long)__builtin_rint(gy * kForceScale));
+    atomicAdd(&f[2 * (size_t)stride + atom],
+              (unsigned long long)(long long)__builtin_rint(gz * kForceScale));
+  }
+}
+
+template <typename CT>
+static void recip_t(NbSystem &S, const float4 *x, int64_t *force, int stride, bool EV,
+                    typename Cx<CT>::T *cgrid, CT *phi, const typename Cx<CT>::T *twx,
+                    const typename Cx<CT>::T *twy, const typename Cx<CT>::T *twz, const CT *pre,
+                    hipStream_t s) {
+  using C2 = typename Cx<CT>::T;
+  const int K0 = S.K[0], K1 = S.K[1], K2 = S.K[2], H = K0 / 2 + 1;
+  const size_t ng = (size_t)K0 * K1 * K2;
+  GridGeom g{K0, K1, K2, 1.0 / S.box[0], 1.0 / S.box[1], 1.0 / S.box[2]};
+  hipEvent_t t0 = nullptr;
+  S.tic(1, s, t0);
+  launch_fill_i64(S.qgrid.get(), ng, s);
+  k_spread<CT><<<(S.natom * 16 + 255) / 256, 256, 0, s>>>(
+      x, S.natom, g, reinterpret_cast<unsigned long long *>(S.qgrid.get()));
+  APO_LAUNCH_CHECK();
+  S.toc(1, s, t0);
+  S.tic(2, s, t0);
+  const size_t lds_cap = 64 * 1024;
+  // x forward
+  const int nrows = K1 * K2;
+  int rb = (int)std::max<size_t>(1, std::min<size_t>(16, (lds_cap / sizeof(C2) - K0) / (2 * K0)));
+  size_t shx = (size_t)(K0 + 2 * rb * K0) * sizeof(C2);
+  const int npair = (nrows + 1) / 2;
+  k_fftx_fwd<CT><<<(npair + rb - 1) / rb, 256, shx, s>>>(
+      reinterpret_cast<const long long *>(S.qgrid.get()), nrows, plan_of(S.fx), rb, twx, cgrid);
+  APO_LAUNCH_CHECK();
+  // y forward
+  const int W = 16;
+  const int wy = (int)std::max<size_t>(1, std::min<size_t>(W, (lds_cap / sizeof(C2) - K1) / (2 * K1)));
+  size_t shy = (size_t)(K1 + 2 * wy * K1) * sizeof(C2);
+  const int ncy = (H + wy - 1) / wy;
+  k_ffty<CT><<<K2 * ncy, 256, shy, s>>>(cgrid, H, K1, wy, plan_of(S.fy), twy, 0);
+  APO_LAUNCH_CHECK();
+  // z forward + convolution + z inverse
+  const int wz = (int)std::max<size_t>(
+      1, std::min<size_t>(W, (lds_cap / sizeof(C2) - K2 - (K0 + K1 + K2)) / (2 * K2)));
+  size_t shz = (size_t)(K2 + 2 * wz * K2) * sizeof(C2) + (size_t)(K0 + K1 + K2) * sizeof(CT);
+  const int ncz = (H + wz - 1) / wz;
+  ConvArgs c{K0, K1, K2, H, wz, 1.0 / S.box[0], 1.0 / S.box[1], 1.0 / S.box[2],
+             kPi * kPi / (S.kappa * S.kappa),
+             1.0 / (S.box[0] * S.box[1] * S.box[2]) / kPi, 0.5 * kCcelec};
+  if (K1 * ncz > S.part.recip_n) throw RuntimeError("reciprocal partial buffer too small");
+  double *part = S.parts.get() + S.part.recip_off;
+  if (EV) {
+    launch_fill_f64(part, (size_t)S.part.recip_n * kRecipPart, s);
+    k_fftz_conv<CT, true><<<K1 * ncz, 256, shz, s>>>(cgrid, c, plan_of(S.fz), twz, pre, part);
+  } else {
+    k_fftz_conv<CT, false><<<K1 * ncz, 256, shz, s>>>(cgrid, c, plan_of(S.fz), twz, pre, part);
+  }
+  APO_LAUNCH_CHECK();
+  // y inverse
+  k_ffty<CT><<<K2 * ncy, 256, shy, s>>>(cgrid, H, K1, wy, plan_of(S.fy), twy, 1);
+  APO_LAUNCH_CHECK();
+  // x inverse
+  k_fftx_inv<CT><<<(npair + rb - 1) / rb, 256, shx, s>>>(cgrid, nrows, plan_of(S.fx), rb, twx, phi);
+  APO_LAUNCH_CHECK();
+  S.toc(2, s, t0);
+  S.tic(3, s, t0);
+  k_gather<CT><<<(S.natom * 16 + 255) / 256, 256, 0, s>>>(
+      x, S.natom, g, phi, reinterpret_cast<unsigned long long *>(force), stride);
+  APO_LAUNCH_CHECK();
+  S.toc(3, s, t0);
+}
+
+void recip_launch(NbSystem &S, const float4 *x, int64_t *force, int stride, bool E, bool V,
+                  hipStream_t s) {
+  if (S.precision == 1)
+    recip_t<double>(S, x, force, stride, E || V, S.cgrid_d.get(), S.phi_d.get(), S.fx.tw64.get(),
+                    S.fy.tw64.get(), S.fz.tw64.get(), S.prefac_d.get(), s);
+  else
+    recip_t<float>(S, x, force, stride, E || V, S.cgrid.get(), S.phi.get(), S.fx.tw.get(),
+                   S.fy.tw.get(), S.fz.tw.get(), S.prefac.get(), s);
+}
+
+}  // namespace apo

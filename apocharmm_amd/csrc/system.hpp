@@ -1,0 +1,115 @@
+// One system's device state: the MI355X counterpart of what ForceManager
+// owns on the force path (CudaPMEDirectForce + CudaNeighborList,
+// CudaPMEReciprocalForce, CudaBondedForce, CudaEnergyVirial).
+#pragma once
+
+#include "common.hpp"
+
+namespace apo {
+
+struct BondT { int i, j; float b0, kb; };
+struct AngleT { int i, j, k; float t0, kt; };
+struct DiheT { int i, j, k, l; float n, kc, sd, cd; float delta; int pad[3]; };
+
+// FFT plan of one dimension: radices and twiddles exp(-2 pi i e / n).
+struct FftDim {
+  int n = 0;
+  int nrad = 0;
+  int rad[16] = {0};
+  DevBuf<float2> tw;
+  DevBuf<double2> tw64;
+};
+
+struct NbSystem {
+  // ---- parameters ----------------------------------------------------------
+  int natom = 0;
+  double box[3] = {0, 0, 0};
+  double kappa = 0, cutnb = 0, ctofnb = 0, ctonnb = 0, e14fac = 1;
+  int K[3] = {0, 0, 0};
+  int order = 4;
+  int precision = 0;
+  int device = 0;
+  int ncu = 256;
+
+  // ---- topology ------------------------------------------------------------
+  int ntypes = 0;
+  DevBuf<int> vdwtype;
+  DevBuf<float2> vdwtab;      // (6*c6, 12*c12)
+  DevBuf<float2> vdwtab14;
+  DevBuf<double2> vdwtab_d;   // FP64 instantiation
+  DevBuf<double2> vdwtab14_d;
+  DevBuf<int> excl_ptr, excl_idx;
+  int nin14 = 0, nex14 = 0;
+  DevBuf<int2> in14, ex14;
+  int nbond = 0, nub = 0, nangle = 0, ndihe = 0, nimdihe = 0;
+  DevBuf<BondT> bonds, ureyb;
+  DevBuf<AngleT> angles;
+  DevBuf<DiheT> dihe, imdihe;
+
+  // ---- neighbor list (cell sort + tiles) ------------------------------------
+  int ncx = 1, ncy = 1, ncol = 1;
+  int ncell_cap = 0;  // ceil(natom/32) + ncol + 1
+  bool list_valid = false;
+  DevBuf<int> meta;   // [0] ncell [1] ncand [2] ntile [3] nitem [4] maxcol
+  DevBuf<int> col_count, col_start, col_ncell, col_cellstart;
+  DevBuf<int> atom_col, atom_slot;
+  DevBuf<unsigned long long> colkeys;
+  DevBuf<float> zkey;
+  DevBuf<int> loc2glo, glo2loc;
+  DevBuf<float4> cell_bb;  // 2 per cell: center, half extent
+  DevBuf<int> icell_cnt, icell_start;
+  DevBuf<int> icell_act, icell_actstart, icell_nitem, icell_itemstart;
+  DevBuf<int> cand_i, cand_j;
+  DevBuf<uint32_t> cand_mask;
+  DevBuf<int> cand_active;
+  DevBuf<int> tile_j;
+  DevBuf<uint32_t> tile_mask;
+  DevBuf<int4> items;
+  size_t cand_cap = 0, tile_cap = 0, item_cap = 0;
+  int host_ncell = 0, host_ncand = 0, host_ntile = 0, host_nitem = 0;
+
+  // ---- per-step direct-space buffers -----------------------------------------
+  DevBuf<int4> posq;       // fixed-point coords (x,y,z as uint32) + scaled q
+  DevBuf<int> type_s;
+  DevBuf<int64_t> fsort;   // sorted-order gradient, 3 * ssort
+  int ssort = 0;           // stride of fsort (= ncell_cap*32)
+  int pair_blocks = 0;
+
+  // ---- PME ------------------------------------------------------------------
+  FftDim fx, fy, fz;
+  DevBuf<int64_t> qgrid;     // K0*K1*K2 fixed-point charge grid
+  DevBuf<float2> cgrid;      // (K0/2+1)*K1*K2 complex half grid
+  DevBuf<float> phi;         // K0*K1*K2 potential grid
+  DevBuf<double2> cgrid_d;   // FP64 instantiation
+  DevBuf<double> phi_d;
+  DevBuf<float> prefac;      // K0 + K1 + K2 (1/|b|^2 per axis)
+  DevBuf<double> prefac_d;
+  int zconv_blocks = 0;
+
+  // ---- energies / virial --------------------------------------------------------
+  PartLayout part;
+  DevBuf<double> parts;
+
+  // HIP-event kernel timers (apo_nb_profile)
+  struct TimerRec { int slot; hipEvent_t a, b; };
+  bool prof_on = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<TimerRec> prof_recs;
+  double prof_ms[8] = {0};
+  long long prof_cnt[8] = {0};
+  hipEvent_t prof_event();
+  void tic(int slot, hipStream_t s, hipEvent_t &a);
+  void toc(int slot, hipStream_t s, hipEvent_t a);
+  void profile_flush();
+  ~NbSystem();
+
+  PairConsts<float> pc_f;
+  PairConsts<double> pc_d;
+
+  void make_pair_consts();
+  void setup_pme();
+  void setup_columns();
+};
+
+}  // namespace apo

@@ -1,0 +1,259 @@
+// Small device utilities: fills, single-block scans, fixed-point conversion,
+// self energy and the final fixed-order energy/virial reduction.
+#include <algorithm>
+#include <set>
+
+#include "../../include/apocharmm_amd.h"
+#include "system.hpp"
+
+namespace apo {
+
+// Fills (kernels instead of hipMemsetAsync so the step stays graph-safe).
+__global__ void k_fill_i64(int64_t *p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+__global__ void k_fill_f64(double *p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+__global__ void k_fill_i32(int *p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+void launch_fill_i32(int *p, size_t n, hipStream_t s) {
+  if (!n) return;
+  int nb = (int)std::min<size_t>((n + 255) / 256, 4096);
+  k_fill_i32<<<nb, 256, 0, s>>>(p, n);
+  APO_LAUNCH_CHECK();
+}
+
+void launch_fill_i64(int64_t *p, size_t n, hipStream_t s) {
+  if (!n) return;
+  int nb = (int)std::min<size_t>((n + 255) / 256, 4096);
+  k_fill_i64<<<nb, 256, 0, s>>>(p, n);
+  APO_LAUNCH_CHECK();
+}
+void launch_fill_f64(double *p, size_t n, hipStream_t s) {
+  if (!n) return;
+  int nb = (int)std::min<size_t>((n + 255) / 256, 4096);
+  k_fill_f64<<<nb, 256, 0, s>>>(p, n);
+  APO_LAUNCH_CHECK();
+}
+
+// Exclusive scan of in[0..n) into out[0..n], out[n] = total, one block.
+// n = *n_dev (clamped to n_max) when n_dev is given, else n_max.
+__global__ __launch_bounds__(1024) void k_scan(const int *in, int *out, const int *n_dev,
+                                               int n_max) {
+  __shared__ int sh[1024];
+  __shared__ int carry;
+  int n = n_dev ? min(*n_dev, n_max) : n_max;
+  int tid = threadIdx.x;
+  if (tid == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < n; base += 1024) {
+    int i = base + tid;
+    int v = i < n ? in[i] : 0;
+    sh[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      int t = tid >= off ? sh[tid - off] : 0;
+      __syncthreads();
+      sh[tid] += t;
+      __syncthreads();
+    }
+    int c = carry;
+    if (i < n) out[i] = c + sh[tid] - v;
+    __syncthreads();
+    if (tid == 1023) carry = c + sh[1023];
+    __syncthreads();
+  }
+  if (tid == 0) out[n] = carry;
+}
+
+void launch_scan(const int *in, int *out, const int *n_dev, int n_max, hipStream_t s) {
+  k_scan<<<1, 1024, 0, s>>>(in, out, n_dev, n_max);
+  APO_LAUNCH_CHECK();
+}
+
+// Force<long long> -> double (src/reduce.cu:132-143).
+__global__ void k_force_to_double(const int64_t *f, int natom, int stride, double *out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= natom) return;
+  for (int c = 0; c < 3; c++)
+    out[(size_t)c * stride + i] = (double)f[(size_t)c * stride + i] * kInvForceScale;
+}
+
+void launch_force_to_double(const int64_t *f, int natom, int stride, double *out,
+                            hipStream_t s) {
+  k_force_to_double<<<(natom + 255) / 256, 256, 0, s>>>(f, natom, stride, out);
+  APO_LAUNCH_CHECK();
+}
+
+// Block-level fixed-order double reduction (tree over LDS).
+template <int NT>
+__device__ double block_sum(double v, double *sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = NT / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// Sum of q^2 per block (Ewald self energy, include/cuda_pme_selfe.h:14-53).
+__global__ __launch_bounds__(256) void k_self(const float4 *xyzq, int n, double *part) {
+  __shared__ double sh[256];
+  int i = blockIdx.x * 256 + threadIdx.x;
+  double q = 0.0;
+  if (i < n) q = (double)xyzq[i].w;
+  double s = block_sum<256>(q * q, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+void self_launch(NbSystem &S, const float4 *x, hipStream_t s) {
+  k_self<<<S.part.self_n, 256, 0, s>>>(x, S.natom, S.parts.get() + S.part.self_off);
+  APO_LAUNCH_CHECK();
+}
+
+// Final reduction: every component is summed over its block partials in a
+// fixed order (deterministic bits run to run).
+struct ReduceArgs {
+  PartLayout L;
+  int terms;
+  int E, V;
+  double self_fac;
+};
+
+__global__ __launch_bounds__(256) void k_reduce_energy(const double *parts, ReduceArgs a,
+                                                        double *energy, double *virial) {
+  __shared__ double sh[256];
+  const PartLayout &L = a.L;
+  auto region_sum = [&](int off, int nblk, int width, int comp) -> double {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += 256) s += parts[off + b * width + comp];
+    return block_sum<256>(s, sh);
+  };
+  bool dir = a.terms & APO_TERM_DIRECT, p14 = a.terms & APO_TERM_PAIRS14,
+       bnd = a.terms & APO_TERM_BONDED, rec = a.terms & APO_TERM_RECIP,
+       slf = a.terms & APO_TERM_SELF;
+  if (a.E) {
+    double e[APO_NUM_ENERGY] = {0};
+    for (int k = 0; k < 5; k++)
+      e[k] = bnd ? region_sum(L.bond_off, L.bond_n, kBondPart, k) : 0.0;
+    e[APO_E_EWKS] = rec ? region_sum(L.recip_off, L.recip_n, kRecipPart, 0) : 0.0;
+    e[APO_E_EWSE] = slf ? a.self_fac * region_sum(L.self_off, L.self_n, 1, 0) : 0.0;
+    e[APO_E_EWEX] = p14 ? region_sum(L.p14_off, L.p14_n, kP14Part, 2) : 0.0;
+    double el = 0, vd = 0;
+    if (dir) {
+      el += region_sum(L.pair_off, L.pair_n, kPairPart, 0);
+      vd += region_sum(L.pair_off, L.pair_n, kPairPart, 1);
+    }
+    if (p14) {
+      el += region_sum(L.p14_off, L.p14_n, kP14Part, 0);
+      vd += region_sum(L.p14_off, L.p14_n, kP14Part, 1);
+    }
+    e[APO_E_ELEC] = el;
+    e[APO_E_VDW] = vd;
+    double tot = 0;
+    for (int k = 0; k < APO_E_EPOT; k++) tot += e[k];
+    e[APO_E_EPOT] = tot;
+    if (threadIdx.x == 0)
+      for (int k = 0; k < APO_NUM_ENERGY; k++) energy[k] = e[k];
+  }
+  if (a.V) {
+    double v6[6] = {0};
+    for (int c = 0; c < 6; c++) {
+      double s = 0;
+      if (dir) s += region_sum(L.pair_off, L.pair_n, kPairPart, 2 + c);
+      if (p14) s += region_sum(L.p14_off, L.p14_n, kP14Part, 3 + c);
+      if (bnd) s += region_sum(L.bond_off, L.bond_n, kBondPart, 5 + c);
+      if (rec) s += region_sum(L.recip_off, L.recip_n, kRecipPart, 1 + c);
+      v6[c] = s;
+    }
+    if (threadIdx.x == 0) {
+      virial[0] = v6[0];
+      virial[1] = v6[1];
+      virial[2] = v6[2];
+      virial[3] = v6[1];
+      virial[4] = v6[3];
+      virial[5] = v6[4];
+      virial[6] = v6[2];
+      virial[7] = v6[4];
+      virial[8] = v6[5];
+    }
+  }
+}
+
+void reduce_energy_launch(NbSystem &S, int terms, bool E, bool V, double *energy,
+                          double *virial, hipStream_t s) {
+  ReduceArgs a;
+  a.L = S.part;
+  a.terms = terms;
+  a.E = E;
+  a.V = V;
+  a.self_fac = -S.kappa * kCcelec / std::sqrt(kPi);
+  k_reduce_energy<<<1, 256, 0, s>>>(S.parts.get(), a, energy, virial);
+  APO_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// Topology: 1-2/1-3/1-4 sets from the bond graph and the exclusion CSR /
+// in14 / ex14 lists (CharmmPSF::buildTopologicalExclusions and
+// getInclusionExclusionLists, src/CharmmPSF.cu:235-385).
+// ---------------------------------------------------------------------------
+void build_exclusions(int natom, int nbond, const int32_t *bonds,
+                      std::vector<int32_t> &excl_ptr, std::vector<int32_t> &excl_idx,
+                      std::vector<int32_t> &in14, std::vector<int32_t> &ex14) {
+  std::vector<std::set<int>> c12(natom), c13(natom), c14(natom);
+  for (int b = 0; b < nbond; b++) {
+    int i = bonds[2 * b], j = bonds[2 * b + 1];
+    c12[i].insert(j);
+    c12[j].insert(i);
+  }
+  for (int b = 0; b < nbond; b++) {
+    int i = bonds[2 * b], j = bonds[2 * b + 1];
+    for (int a : c12[j])
+      if (a != i) c13[i].insert(a);
+    for (int a : c12[i])
+      if (a != j) c13[j].insert(a);
+  }
+  for (int i = 0; i < natom; i++)
+    for (int j : c13[i])
+      for (int k : c12[j])
+        if (!c12[i].count(k)) c14[i].insert(k);
+  excl_ptr.assign(1, 0);
+  excl_idx.clear();
+  in14.clear();
+  ex14.clear();
+  for (int i = 0; i < natom; i++) {
+    std::set<int> all;
+    all.insert(c12[i].begin(), c12[i].end());
+    all.insert(c13[i].begin(), c13[i].end());
+    all.insert(c14[i].begin(), c14[i].end());
+    all.erase(i);
+    for (int j : all) excl_idx.push_back(j);
+    excl_ptr.push_back((int32_t)excl_idx.size());
+    for (int j : c14[i])
+      if (j > i && !c12[i].count(j) && !c13[i].count(j)) {
+        in14.push_back(i);
+        in14.push_back(j);
+      }
+    std::set<int> ex(c12[i].begin(), c12[i].end());
+    ex.insert(c13[i].begin(), c13[i].end());
+    for (int j : ex)
+      if (j > i) {
+        ex14.push_back(i);
+        ex14.push_back(j);
+      }
+  }
+}
+
+}  // namespace apo

@@ -1,0 +1,243 @@
+/*
+ * apocharmm_amd -- C ABI of the MI355X-native CHARMM force path.
+ *
+ * One handle (apo_nb_system) owns everything the reference's ForceManager
+ * owns for one system on the hot path: CudaPMEDirectForce (neighbor list +
+ * pair kernel + 1-4/exclusion kernel), CudaPMEReciprocalForce, CudaBondedForce
+ * and the energy/virial reduction.  Coordinates come in as the reference's
+ * xyzq float4 array; the result is the fixed-point (2^-40) gradient in the
+ * reference's strided SoA layout, per-term energies and the 3x3 virial.
+ *
+ * Conventions follow the reference's C ABI (capi/include/apocharmm_c/):
+ *  - every entry returns apo_status (Status.h); on failure the message is
+ *    kept in thread-local storage and read with apo_last_error() (Error.h,
+ *    capi/src/apocharmm_c/Error.cpp:18-22);
+ *  - handles are opaque and released with *_destroy.
+ * All device pointers are HBM addresses on the current HIP device; `stream`
+ * is a hipStream_t passed as void* (NULL = default stream).  Entry points
+ * that only enqueue work do not synchronise the host.
+ */
+#ifndef APOCHARMM_AMD_H
+#define APOCHARMM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(_WIN32)
+#define APO_API __declspec(dllexport)
+#else
+#define APO_API __attribute__((visibility("default")))
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: capi/include/apocharmm_c/Status.h (CUDA_ERROR -> GPU_ERROR). */
+typedef enum apo_status {
+  APO_OK = 0,
+  APO_INVALID_ARGUMENT = 1,
+  APO_RUNTIME_ERROR = 2,
+  APO_GPU_ERROR = 3,
+  APO_NOT_INITIALIZED = 4,
+  APO_NOT_IMPLEMENTED = 5
+} apo_status;
+
+/* Energy terms, in the order of the keys of ForceManager::getEnergyComponents
+ * (src/ForceManager.cu:278-295); index APO_E_EPOT holds their sum
+ * (UpdatePotentialEnergyKernel, src/ForceManager.cu:641-653). */
+enum {
+  APO_E_BOND = 0,
+  APO_E_ANGLE = 1,
+  APO_E_UREYB = 2,
+  APO_E_DIHE = 3,
+  APO_E_IMDIHE = 4,
+  APO_E_EWKS = 5,
+  APO_E_EWSE = 6,
+  APO_E_EWEX = 7,
+  APO_E_ELEC = 8,
+  APO_E_VDW = 9,
+  APO_E_EPOT = 10,
+  APO_NUM_ENERGY = 11
+};
+
+/* calc_force flags.  APO_CALC_ENERGY / APO_CALC_VIRIAL are the calcEnergy /
+ * calcVirial arguments of ForceManager::calcForce (include/ForceManager.h:424);
+ * APO_RESET_LIST is its `reset` argument (rebuild the neighbor list first).
+ * The APO_TERM_* bits select force groups (0 = all groups), which the
+ * reference does through its three force objects. */
+enum {
+  APO_CALC_ENERGY = 1 << 0,
+  APO_CALC_VIRIAL = 1 << 1,
+  APO_RESET_LIST = 1 << 2,
+  APO_TERM_BONDED = 1 << 8,
+  APO_TERM_DIRECT = 1 << 9,
+  APO_TERM_PAIRS14 = 1 << 10,
+  APO_TERM_RECIP = 1 << 11,
+  APO_TERM_SELF = 1 << 12,
+  APO_TERM_ALL = 0x1F00
+};
+
+/* Arithmetic of the pair and reciprocal kernels: the mixed path (FP32 math,
+ * int64 fixed-point accumulation) or the all-FP64 instantiation of the same
+ * kernels. */
+enum { APO_PRECISION_MIXED = 0, APO_PRECISION_FP64 = 1 };
+
+/* Simulation parameters.  Field meaning follows the ForceManager setters
+ * (include/ForceManager.h:200-231): cutnb = setCutoff (list cutoff),
+ * ctofnb = setCtofnb (r_off), ctonnb = setCtonnb (r_on), kappa = setKappa,
+ * fft_grid = setFFTGrid, pme_order = setPmeSplineOrder (4 only), box =
+ * setBoxDimensions (orthorhombic).  e14fac scales 1-4 electrostatics
+ * (CudaPMEDirectForce::setup, src/CudaPMEDirectForce.cu:275). */
+typedef struct apo_nb_params {
+  double box[3];
+  double kappa;
+  double cutnb;
+  double ctofnb;
+  double ctonnb;
+  double e14fac;
+  int32_t fft_grid[3];
+  int32_t pme_order;
+  int32_t precision;
+  int32_t reserved;
+} apo_nb_params;
+
+/* Topology and resolved force-field parameters (what ForceManager::initialize
+ * extracts from CharmmPSF + CharmmParameters, src/ForceManager.cu:463-527).
+ *  vdwtype[natom], vdwtab[ntypes*(ntypes+1)]: (c6, c12) per lower-triangular
+ *    type pair a*(a+1)/2+b (a >= b), U = c12/r^12 - c6/r^6 before switching
+ *    (CharmmParameters::getVdwParamsAndTypes, src/CharmmParameters.cu:718-844);
+ *    vdwtab14 the same for 1-4 pairs (may equal vdwtab);
+ *  bonds[2*nbond] + bond_par[2*nbond] (b0, kb): harmonic bonds; the bond list
+ *    is also the connectivity from which the 1-2/1-3/1-4 exclusion sets are
+ *    derived (CharmmPSF::buildTopologicalExclusions, src/CharmmPSF.cu:331-385);
+ *  ureyb[2*n] + ub_par[2*n] (s0, kub);  angles[3*n] + angle_par[2*n]
+ *    (theta0 [rad], ktheta);  dihe[4*n] + dihe_par[3*n] (multiplicity, k,
+ *    delta [rad]), one row per cosine term;  imdihe[4*n] + imdihe_par[3*n]
+ *    (multiplicity 0 = harmonic k (phi - delta)^2).
+ * All indices are 0-based.  Arrays are host memory, copied at creation. */
+typedef struct apo_nb_topology {
+  int32_t natom;
+  int32_t ntypes;
+  const int32_t *vdwtype;
+  const float *vdwtab;
+  const float *vdwtab14;
+  int32_t nbond;
+  const int32_t *bonds;
+  const float *bond_par;
+  int32_t nureyb;
+  const int32_t *ureyb;
+  const float *ub_par;
+  int32_t nangle;
+  const int32_t *angles;
+  const float *angle_par;
+  int32_t ndihe;
+  const int32_t *dihe;
+  const float *dihe_par;
+  int32_t nimdihe;
+  const int32_t *imdihe;
+  const float *imdihe_par;
+} apo_nb_topology;
+
+typedef struct apo_nb_system apo_nb_system;
+
+/* Thread-local message of the last failed call on this thread
+ * (apo_last_error, capi/src/apocharmm_c/Error.cpp:18-22). */
+APO_API const char *apo_last_error(void);
+
+/* Library version string. */
+APO_API const char *apo_version(void);
+
+/* Creates the force path for one system on the current HIP device
+ * (ForceManager(psf, prm) + ForceManager::initialize,
+ * src/ForceManager.cu:436-550).  Validates cutoffs against the box as the
+ * reference does (cutoff <= box/2, src/ForceManager.cu:443-447). */
+APO_API apo_status apo_nb_create(const apo_nb_params *params,
+                                 const apo_nb_topology *topology,
+                                 apo_nb_system **out);
+
+/* Releases the handle and its device memory (ForceManager::dealloc,
+ * src/ForceManager.cu:957-975). */
+APO_API apo_status apo_nb_destroy(apo_nb_system *sys);
+
+/* ForceManager::setBoxDimensions (include/ForceManager.h:200).  Takes effect
+ * on the next calc_force; the neighbor list is rebuilt on the next reset. */
+APO_API apo_status apo_nb_set_box(apo_nb_system *sys, const double *box3);
+
+/* ForceManager::resetNeighborList(xyzq) (src/ForceManager.cu,
+ * CudaPMEDirectForce::resetNeighborList src/CudaPMEDirectForce.cu:1347-1366):
+ * cell sort + tile list build from device xyzq[4*natom]. */
+APO_API apo_status apo_nb_reset_list(apo_nb_system *sys, const float *d_xyzq,
+                                     void *stream);
+
+/* ForceManager::calcForce(xyzq, reset, calcEnergy, calcVirial)
+ * (include/ForceManager.h:424, src/ForceManager.cu:821-827).
+ *  d_xyzq    device float[4*natom] (x, y, z, q), whole molecules as produced
+ *            by CharmmContext::imageCentering;
+ *  d_force   device int64[3*stride]: OVERWRITTEN with the total gradient
+ *            dU/dx in units of 2^-40 kcal/mol/A, x at [0,natom), y at
+ *            [stride, ...), z at [2*stride, ...) (Force<long long>,
+ *            include/Force.h:32-154);
+ *  d_energy  device double[APO_NUM_ENERGY] or NULL; written when
+ *            APO_CALC_ENERGY is set;
+ *  d_virial  device double[9] or NULL; written when APO_CALC_VIRIAL is set
+ *            (W = -sum x (x) g, row-major).
+ * Work is enqueued on `stream`; the host is not synchronised. */
+APO_API apo_status apo_nb_calc_force(apo_nb_system *sys, const float *d_xyzq,
+                                     int32_t flags, int64_t *d_force,
+                                     int32_t stride, double *d_energy,
+                                     double *d_virial, void *stream);
+
+/* ForceManager::getForceStride: the smallest valid stride for this system,
+ * ceil((natom+31)*8/256)*256/8 (src/Force.cu:344-352). */
+APO_API apo_status apo_nb_force_stride(const apo_nb_system *sys,
+                                       int32_t *stride);
+
+APO_API apo_status apo_nb_num_atoms(const apo_nb_system *sys, int32_t *natom);
+
+/* Force<long long>::convert / reduce_force<long long,double>
+ * (src/reduce.cu:132-143): d_out[k] = (double)d_force[k] * 2^-40 for
+ * k in the three strided rows of natom entries. */
+APO_API apo_status apo_nb_force_to_double(const int64_t *d_force,
+                                          int32_t natom, int32_t stride,
+                                          double *d_out, void *stream);
+
+/* Neighbor-list statistics of the last build (synchronises the stream):
+ * out[0] cells, out[1] tiles (32x32 cell pairs after pruning),
+ * out[2] work items, out[3] atom pairs inside the list cutoff covered by the
+ * tiles, out[4] columns, out[5] largest column population.  The pair count
+ * is the reference's debug cross-check quantity
+ * (src/CudaNeighborListBuild.cu:2215-2264). */
+APO_API apo_status apo_nb_list_stats(apo_nb_system *sys, int64_t *out6,
+                                     void *stream);
+
+/* Kernel timing with HIP events recorded on the calc_force stream (no
+ * counterpart in the reference, which times with NVTX ranges,
+ * src/ForceManager.cu:606-616).  enable != 0 starts recording and clears the
+ * accumulators.  apo_nb_profile_read synchronises the recorded events and
+ * returns, per slot, the summed milliseconds and the number of launches:
+ * slot 0 pair kernel, 1 PME spread, 2 PME FFT + convolution (5 passes),
+ * 3 PME gather, 4 neighbor-list rebuild, 5 whole calc_force, 6 bonded +
+ * 1-4/exclusions, 7 sort/finalize of the direct-space coordinates/forces. */
+enum { APO_PROFILE_SLOTS = 8 };
+APO_API apo_status apo_nb_profile(apo_nb_system *sys, int32_t enable);
+APO_API apo_status apo_nb_profile_read(apo_nb_system *sys, double *ms8,
+                                       int64_t *count8);
+
+/* Host-side topology helper (no device work): 1-2/1-3/1-4 exclusion CSR and
+ * the in14/ex14 pair lists derived from a bond list, exactly as
+ * apo_nb_create derives them (CharmmPSF::buildTopologicalExclusions +
+ * getInclusionExclusionLists, src/CharmmPSF.cu:235-385).  Pass NULL outputs
+ * to query sizes: counts[0] = excl entries, counts[1] = in14 pairs,
+ * counts[2] = ex14 pairs. */
+APO_API apo_status apo_topology_exclusions(int32_t natom, int32_t nbond,
+                                           const int32_t *bonds,
+                                           int32_t *counts3,
+                                           int32_t *excl_ptr,
+                                           int32_t *excl_idx, int32_t *in14,
+                                           int32_t *ex14);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
