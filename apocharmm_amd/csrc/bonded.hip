@@ -63,6 +63,49 @@ template <typename CT> __device__ __forceinline__ CT asin_ct(CT x);
 template <> __device__ __forceinline__ float asin_ct<float>(float x) { return asinf(x); }
 template <> __device__ __forceinline__ double asin_ct<double>(double x) { return asin(x); }
 
+// r - b0 for a stiff bond.  In FP32 the rounding of r (~3e-8 A) alone would
+// put 2 kb * 3e-8 ~ 3e-5 kcal/mol/A into the force, so the distance is formed
+// with error-free transformations (TwoSum / FMA products): dx = hi + lo exact,
+// r^2 as a float-float sum, r - b0 from the float-float sqrt.
+__device__ __forceinline__ void two_diff(float a, float b, float &s, float &e) {
+  s = a - b;
+  const float bb = s - a;
+  e = (a - (s - bb)) - (b + bb);
+}
+__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
+  s = a + b;
+  const float bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+template <typename CT>
+__device__ __forceinline__ CT bond_stretch(const float4 &xi, const float4 &xj, CT b0, CT &dx,
+                                           CT &dy, CT &dz, CT &r) {
+  dx = (CT)xi.x - (CT)xj.x;
+  dy = (CT)xi.y - (CT)xj.y;
+  dz = (CT)xi.z - (CT)xj.z;
+  r = sqrt((double)(dx * dx + dy * dy + dz * dz));
+  return r - b0;
+}
+template <>
+__device__ __forceinline__ float bond_stretch<float>(const float4 &xi, const float4 &xj, float b0,
+                                                     float &dx, float &dy, float &dz, float &r) {
+  float lx, ly, lz;
+  two_diff(xi.x, xj.x, dx, lx);
+  two_diff(xi.y, xj.y, dy, ly);
+  two_diff(xi.z, xj.z, dz, lz);
+  const float px = dx * dx, py = dy * dy, pz = dz * dz;
+  float small = fmaf(dx, dx, -px) + fmaf(dy, dy, -py) + fmaf(dz, dz, -pz) +
+                2.0f * (dx * lx + dy * ly + dz * lz);
+  float s, e1, s2, e2;
+  two_sum(px, py, s, e1);
+  two_sum(s, pz, s2, e2);
+  const float s_lo = small + e1 + e2;
+  const float rh = sqrtf(s2);
+  const float rl = (fmaf(-rh, rh, s2) + s_lo) / (2.0f * rh);
+  r = rh;
+  return (rh - b0) + rl;
+}
+
 struct BondedArgs {
   const float4 *xyzq;
   const BondT *bonds, *ureyb;
@@ -105,9 +148,8 @@ __global__ __launch_bounds__(256) void k_bonded(BondedArgs a, unsigned long long
     const bool ub = idx >= a.nb;
     const BondT b = ub ? a.ureyb[idx - a.nb] : a.bonds[idx];
     const float4 xi = x[b.i], xj = x[b.j];
-    const CT dx = (CT)xi.x - (CT)xj.x, dy = (CT)xi.y - (CT)xj.y, dz = (CT)xi.z - (CT)xj.z;
-    const CT r = sqrt_ct<CT>(dx * dx + dy * dy + dz * dz);
-    const CT db = r - (CT)b.b0;
+    CT dx, dy, dz, r;
+    const CT db = bond_stretch<CT>(xi, xj, (CT)b.b0, dx, dy, dz, r);
     CT fij = db * (CT)b.kb;
     if (CALC_E) acc[ub ? 2 : 0] += (double)(fij * db);
     fij *= (CT)2 / r;

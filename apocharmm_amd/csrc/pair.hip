@@ -139,7 +139,10 @@ __global__ __launch_bounds__(64) void k_pair(
       const uint32_t m = tv ? tile_mask[(size_t)tile * kCell + l] : 0xffffffffu;
       const CT qj = charge_of<CT>(pj.w) * qfac;
       CT fjx = 0, fjy = 0, fjz = 0;
-      CT el = 0, ev = 0, v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
+      // energies accumulate per pair in FP64 (large cancelling sums); the
+      // virial per tile in CT
+      double el = 0, ev = 0;
+      CT v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
 #pragma unroll
       for (int t = 0; t < 32; t++) {
         const int4 pi = posq[ibase + t];
@@ -174,9 +177,9 @@ __global__ __launch_bounds__(64) void k_pair(
           fiz[t] += gz;
           if (CALC_E) {
             const CT C6 = outer ? (CT)0 : P.dv6, C12 = outer ? (CT)0 : P.dv12;
-            el += u_el;
-            ev += c.y * ((CT)1 / (CT)12) * (A12 * t12 * t12 + C12) -
-                  c.x * ((CT)1 / (CT)6) * (A6 * t6 * t6 + C6);
+            el += (double)u_el;
+            ev += (double)(c.y * ((CT)1 / (CT)12) * (A12 * t12 * t12 + C12) -
+                           c.x * ((CT)1 / (CT)6) * (A6 * t6 * t6 + C6));
           }
           if (CALC_V) {
             v0 -= gx * dx;
@@ -194,8 +197,8 @@ __global__ __launch_bounds__(64) void k_pair(
         atomicAdd(&fsort[2 * (size_t)ss + jl], fx_round(fjz));
       }
       if (CALC_E) {
-        acc[0] += (double)el;
-        acc[1] += (double)ev;
+        acc[0] += el;
+        acc[1] += ev;
       }
       if (CALC_V) {
         acc[2] += (double)v0;

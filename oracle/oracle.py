@@ -244,6 +244,12 @@ def prepare(spec, table_dtype=np.float32):
     eps14 = spec.get("eps14", spec["eps"])
     rh14 = spec.get("rmin_half14", spec["rmin_half"])
     d = dict(spec)
+    # bonded coefficients are float in the reference (bondcoef float2,
+    # anglecoef float2, dihecoef float4, src/CudaBondedForce.cu) and in the C
+    # ABI: evaluate the same rounded values
+    for key in ("bond_par", "ub_par", "angle_par", "dihe_par", "imp_par"):
+        if key in d and d[key] is not None:
+            d[key] = np.asarray(d[key], np.float64).astype(np.float32).astype(np.float64)
     d.update(
         vdwtab=lj_table(spec["eps"], spec["rmin_half"], table_dtype),
         vdwtab14=lj_table(eps14, rh14, table_dtype),
