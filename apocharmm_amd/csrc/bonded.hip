@@ -63,47 +63,18 @@ template <typename CT> __device__ __forceinline__ CT asin_ct(CT x);
 template <> __device__ __forceinline__ float asin_ct<float>(float x) { return asinf(x); }
 template <> __device__ __forceinline__ double asin_ct<double>(double x) { return asin(x); }
 
-// r - b0 for a stiff bond.  In FP32 the rounding of r (~3e-8 A) alone would
-// put 2 kb * 3e-8 ~ 3e-5 kcal/mol/A into the force, so the distance is formed
-// with error-free transformations (TwoSum / FMA products): dx = hi + lo exact,
-// r^2 as a float-float sum, r - b0 from the float-float sqrt.
-__device__ __forceinline__ void two_diff(float a, float b, float &s, float &e) {
-  s = a - b;
-  const float bb = s - a;
-  e = (a - (s - bb)) - (b + bb);
-}
-__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
-  s = a + b;
-  const float bb = s - a;
-  e = (a - (s - bb)) + (b - bb);
-}
-template <typename CT>
-__device__ __forceinline__ CT bond_stretch(const float4 &xi, const float4 &xj, CT b0, CT &dx,
-                                           CT &dy, CT &dz, CT &r) {
-  dx = (CT)xi.x - (CT)xj.x;
-  dy = (CT)xi.y - (CT)xj.y;
-  dz = (CT)xi.z - (CT)xj.z;
-  r = sqrt((double)(dx * dx + dy * dy + dz * dz));
+// Stiff terms (bonds k ~ 450, angles k ~ 55) turn the FP32 rounding of r or
+// theta (~3e-8) into ~3e-5 kcal/mol/A of force, larger than the whole
+// direct-space error budget; their geometric core (r - b0, theta - theta0)
+// is therefore evaluated in FP64 in both modes (a few ops per term).  The
+// force components stay in CT.
+__device__ __forceinline__ double bond_stretch(const float4 &xi, const float4 &xj, double b0,
+                                               double &dx, double &dy, double &dz, double &r) {
+  dx = (double)xi.x - (double)xj.x;
+  dy = (double)xi.y - (double)xj.y;
+  dz = (double)xi.z - (double)xj.z;
+  r = sqrt(dx * dx + dy * dy + dz * dz);
   return r - b0;
-}
-template <>
-__device__ __forceinline__ float bond_stretch<float>(const float4 &xi, const float4 &xj, float b0,
-                                                     float &dx, float &dy, float &dz, float &r) {
-  float lx, ly, lz;
-  two_diff(xi.x, xj.x, dx, lx);
-  two_diff(xi.y, xj.y, dy, ly);
-  two_diff(xi.z, xj.z, dz, lz);
-  const float px = dx * dx, py = dy * dy, pz = dz * dz;
-  float small = fmaf(dx, dx, -px) + fmaf(dy, dy, -py) + fmaf(dz, dz, -pz) +
-                2.0f * (dx * lx + dy * ly + dz * lz);
-  float s, e1, s2, e2;
-  two_sum(px, py, s, e1);
-  two_sum(s, pz, s2, e2);
-  const float s_lo = small + e1 + e2;
-  const float rh = sqrtf(s2);
-  const float rl = (fmaf(-rh, rh, s2) + s_lo) / (2.0f * rh);
-  r = rh;
-  return (rh - b0) + rl;
 }
 
 struct BondedArgs {
@@ -148,30 +119,35 @@ __global__ __launch_bounds__(256) void k_bonded(BondedArgs a, unsigned long long
     const bool ub = idx >= a.nb;
     const BondT b = ub ? a.ureyb[idx - a.nb] : a.bonds[idx];
     const float4 xi = x[b.i], xj = x[b.j];
-    CT dx, dy, dz, r;
-    const CT db = bond_stretch<CT>(xi, xj, (CT)b.b0, dx, dy, dz, r);
-    CT fij = db * (CT)b.kb;
-    if (CALC_E) acc[ub ? 2 : 0] += (double)(fij * db);
-    fij *= (CT)2 / r;
+    double ddx, ddy, ddz, r;
+    const double db = bond_stretch(xi, xj, (double)b.b0, ddx, ddy, ddz, r);
+    if (CALC_E) acc[ub ? 2 : 0] += (double)b.kb * db * db;
+    const CT fij = (CT)(2.0 * (double)b.kb * db / r);
+    const CT dx = (CT)ddx, dy = (CT)ddy, dz = (CT)ddz;
     add_grad<CT>(f, a.stride, b.i, fij * dx, fij * dy, fij * dz);
     add_grad<CT>(f, a.stride, b.j, -fij * dx, -fij * dy, -fij * dz);
     if (CALC_V) vir_add<CT>(acc + 5, dx, dy, dz, fij * dx, fij * dy, fij * dz);
   } else if ((idx -= a.nb + a.nub) < a.na) {
     const AngleT g = a.angles[idx];
     const float4 xi = x[g.i], xj = x[g.j], xk = x[g.k];
-    const CT dxij = (CT)xi.x - (CT)xj.x, dyij = (CT)xi.y - (CT)xj.y, dzij = (CT)xi.z - (CT)xj.z;
-    const CT dxkj = (CT)xk.x - (CT)xj.x, dykj = (CT)xk.y - (CT)xj.y, dzkj = (CT)xk.z - (CT)xj.z;
-    const CT rij = sqrt_ct<CT>(dxij * dxij + dyij * dyij + dzij * dzij);
-    const CT rkj = sqrt_ct<CT>(dxkj * dxkj + dykj * dykj + dzkj * dzkj);
-    const CT rij_inv = (CT)1 / rij, rkj_inv = (CT)1 / rkj;
+    // FP64 geometric core (see bond_stretch)
+    const double ddxij = (double)xi.x - (double)xj.x, ddyij = (double)xi.y - (double)xj.y,
+                 ddzij = (double)xi.z - (double)xj.z;
+    const double ddxkj = (double)xk.x - (double)xj.x, ddykj = (double)xk.y - (double)xj.y,
+                 ddzkj = (double)xk.z - (double)xj.z;
+    const double drij = sqrt(ddxij * ddxij + ddyij * ddyij + ddzij * ddzij);
+    const double drkj = sqrt(ddxkj * ddxkj + ddykj * ddykj + ddzkj * ddzkj);
+    double dcst = (ddxij * ddxkj + ddyij * ddykj + ddzij * ddzkj) / (drij * drkj);
+    dcst = fmin(fmax(dcst, -0.999), 0.999);
+    const double dda = acos(dcst) - (double)g.t0;
+    if (CALC_E) acc[1] += (double)g.kt * dda * dda;
+    const CT df = (CT)(-2.0 * (double)g.kt * dda / sqrt(1.0 - dcst * dcst));
+    const CT cst = (CT)dcst;
+    const CT dxij = (CT)ddxij, dyij = (CT)ddyij, dzij = (CT)ddzij;
+    const CT dxkj = (CT)ddxkj, dykj = (CT)ddykj, dzkj = (CT)ddzkj;
+    const CT rij_inv = (CT)(1.0 / drij), rkj_inv = (CT)(1.0 / drkj);
     const CT uxi = dxij * rij_inv, uyi = dyij * rij_inv, uzi = dzij * rij_inv;
     const CT uxk = dxkj * rkj_inv, uyk = dykj * rkj_inv, uzk = dzkj * rkj_inv;
-    CT cst = uxi * uxk + uyi * uyk + uzi * uzk;
-    cst = fmin(fmax(cst, (CT)-0.999), (CT)0.999);
-    const CT da = acos_ct<CT>(cst) - (CT)g.t0;
-    CT df = (CT)g.kt * da;
-    if (CALC_E) acc[1] += (double)(df * da);
-    df = -(CT)2 * df / sqrt_ct<CT>((CT)1 - cst * cst);
     // gradient on i and k: -dE/dtheta / sin * d cos / dx
     const CT gix = df * rij_inv * (uxk - cst * uxi), giy = df * rij_inv * (uyk - cst * uyi),
              giz = df * rij_inv * (uzk - cst * uzi);

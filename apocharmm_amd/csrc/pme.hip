@@ -180,7 +180,11 @@ struct GridGeom {
   double inv0, inv1, inv2;
 };
 
-// Spread: 16 lanes per atom, each lane one (y, z) row of 4 x points.
+// Spread: 16 lanes per atom, each lane one (y, z) row of 4 x points.  The
+// charge-assignment weights q*theta_x*theta_y*theta_z are formed in FP64 in
+// both precision modes: they enter the int64 grid exactly, and FP32 weights
+// alone would act as ~1e-7 random charge noise on every atom, which at 98k
+// atoms becomes ~2.6e-6 of the total force (FP32 FFT/gather stay below 3e-7).
 template <typename CT>
 __global__ __launch_bounds__(256) void k_spread(const float4 *__restrict__ xyzq, int n, GridGeom g,
                                                 unsigned long long *__restrict__ grid) {
@@ -193,23 +197,21 @@ __global__ __launch_bounds__(256) void k_spread(const float4 *__restrict__ xyzq,
   frac_coord(p.x, g.inv0, g.K0, ix, wx);
   frac_coord(p.y, g.inv1, g.K1, iy, wy);
   frac_coord(p.z, g.inv2, g.K2, iz, wz);
-  CT tx[4], ty[4], tz[4], d[4];
-  bspline4<CT>((CT)wx, tx, d);
-  bspline4<CT>((CT)wy, ty, d);
-  bspline4<CT>((CT)wz, tz, d);
+  double tx[4], ty[4], tz[4], d[4];
+  bspline4<double>(wx, tx, d);
+  bspline4<double>(wy, ty, d);
+  bspline4<double>(wz, tz, d);
   const int a = row & 3, b = row >> 2;
   int y = iy + a, z = iz + b;
   if (y >= g.K1) y -= g.K1;
   if (z >= g.K2) z -= g.K2;
-  const CT qyz = (CT)p.w * ty[a] * tz[b];
+  const double qyz = (double)p.w * ty[a] * tz[b] * kGridScale;
   const size_t base = (size_t)g.K0 * ((size_t)y + (size_t)g.K1 * z);
 #pragma unroll
   for (int c = 0; c < 4; c++) {
     int x = ix + c;
     if (x >= g.K0) x -= g.K0;
-    const CT v = qyz * tx[c];
-    atomicAdd(&grid[base + x],
-              (unsigned long long)(long long)__builtin_rint((double)v * kGridScale));
+    atomicAdd(&grid[base + x], (unsigned long long)(long long)__builtin_rint(qyz * tx[c]));
   }
 }
 
