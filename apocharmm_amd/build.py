@@ -50,6 +50,16 @@ def build(verbose=False, jobs=None):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s" % r.stderr[-4000:])
     os.replace(tmp, OUT)
+    # the C++ ForceManager mirror must compile against the C ABI header
+    probe = os.path.join(OBJ, "fm_header_check.cpp")
+    with open(probe, "w") as f:
+        f.write('#include "apocharmm_amd/ForceManager.hpp"\n'
+                'int fm_header_check() { apocharmm_amd::Topology t; '
+                'apocharmm_amd::ForceManager fm(t); fm.setKappa(0.34); return fm.getForceStride(); }\n')
+    r = subprocess.run([HIPCC] + FLAGS + ["-x", "hip", "-fsyntax-only", probe], capture_output=True,
+                       text=True)
+    if r.returncode != 0:
+        raise RuntimeError("ForceManager.hpp does not compile:\n%s" % r.stderr[-4000:])
     if verbose:
         print("built", OUT)
     return OUT
