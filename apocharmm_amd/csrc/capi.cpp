@@ -6,7 +6,7 @@
 #include <cstring>
 #include <string>
 
-#include "../../include/apocharmm_amd.h"
+#include "apocharmm_amd.h"
 #include "system.hpp"
 
 namespace apo {

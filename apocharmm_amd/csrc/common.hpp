@@ -116,6 +116,7 @@ void launch_force_to_double(const int64_t *f, int natom, int stride,
 
 void nblist_build(NbSystem &sys, const float4 *d_xyzq, hipStream_t s);
 void nblist_sort_coords(NbSystem &sys, const float4 *d_xyzq, hipStream_t s);
+void nblist_cellbb(NbSystem &sys, hipStream_t s);
 void nblist_stats(NbSystem &sys, int64_t *out6, hipStream_t s);
 
 void direct_launch(NbSystem &sys, bool energy, bool virial, hipStream_t s);

@@ -420,9 +420,17 @@ void nblist_build(NbSystem &S, const float4 *x, hipStream_t s) {
   S.list_valid = true;
 }
 
+void nblist_cellbb(NbSystem &S, hipStream_t s) {
+  const float3 unit = make_float3((float)(S.box[0] / 4294967296.0), (float)(S.box[1] / 4294967296.0),
+                                  (float)(S.box[2] / 4294967296.0));
+  k_cellbb<<<nblk(S.ncell_cap, 2), 64, 0, s>>>(S.posq.get(), S.loc2glo.get(), S.meta.get(), unit,
+                                                S.cell_bb.get());
+  APO_LAUNCH_CHECK();
+}
+
 void nblist_sort_coords(NbSystem &S, const float4 *x, hipStream_t s) {
   const double3 inv = make_double3(1.0 / S.box[0], 1.0 / S.box[1], 1.0 / S.box[2]);
-  float qscale = S.precision == 1 ? 1.0f : (float)std::sqrt(kCcelec);
+  const float qscale = 1.0f;  // raw charge; ccelec is applied per tile in the pair kernel
   k_sort_coords<<<nblk(S.ssort, 256), 256, 0, s>>>(x, S.vdwtype.get(), S.loc2glo.get(),
                                                     S.meta.get(), inv, qscale, S.posq.get(),
                                                     S.type_s.get(), S.fsort.get(), S.ssort);

@@ -97,23 +97,23 @@ template <typename CT> __device__ __forceinline__ CT transpose_reduce(CT (&v)[32
   return v[0] + __shfl_xor(v[0], 1);
 }
 
-constexpr int kMaxTabLds = 528;  // 32 vdW types
+constexpr int kMaxTypesLds = 32;  // square vdW table in LDS up to 32 types
 
-template <typename CT, bool CALC_E, bool CALC_V>
+// TAB_LDS: vdW table square (ntypes x ntypes) in LDS, row offset carried with
+// the i-atom; otherwise the triangular table is read from global memory.
+template <typename CT, bool CALC_E, bool CALC_V, bool TAB_LDS>
 __global__ __launch_bounds__(64) void k_pair(
     const int4 *__restrict__ posq, const int *__restrict__ types, const int4 *__restrict__ items,
     const int *__restrict__ tile_j, const uint32_t *__restrict__ tile_mask,
-    const int *__restrict__ meta, const typename Vec2<CT>::T *__restrict__ vdwtab, int ntypes,
-    const PairConsts<CT> P, CT qfac, unsigned long long *__restrict__ fsort, int ss,
-    double *__restrict__ part) {
+    const int *__restrict__ meta, const typename Vec2<CT>::T *__restrict__ vdwtab,
+    const typename Vec2<CT>::T *__restrict__ vdwsq, int ntypes, const PairConsts<CT> P, CT qfac,
+    unsigned long long *__restrict__ fsort, int ss, double *__restrict__ part) {
   using T2 = typename Vec2<CT>::T;
-  __shared__ T2 sh_tab[kMaxTabLds];
-  const int ntab = ntypes * (ntypes + 1) / 2;
-  const bool tab_lds = ntab <= kMaxTabLds;
-  if (tab_lds)
-    for (int k = threadIdx.x; k < ntab; k += 64) sh_tab[k] = vdwtab[k];
-  __syncthreads();
-  const T2 *tab = tab_lds ? sh_tab : vdwtab;
+  __shared__ T2 sh_tab[TAB_LDS ? kMaxTypesLds * kMaxTypesLds : 1];
+  __shared__ int4 sh_i[kCell];
+  __shared__ int sh_row[kCell];
+  if (TAB_LDS)
+    for (int k = threadIdx.x; k < ntypes * ntypes; k += 64) sh_tab[k] = vdwsq[k];
 
   const int lane = threadIdx.x;
   const int half = lane >> 5, l = lane & 31;
@@ -125,11 +125,20 @@ __global__ __launch_bounds__(64) void k_pair(
   for (int item = blockIdx.x; item < nitem; item += gridDim.x) {
     const int4 it = items[item];
     const int ibase = it.x * kCell;
+    __syncthreads();  // previous item's readers are done with sh_i
+    if (lane < kCell) {
+      sh_i[lane] = posq[ibase + lane];
+      sh_row[lane] = TAB_LDS ? types[ibase + lane] * ntypes : types[ibase + lane];
+    }
+    __syncthreads();
     CT fix[32], fiy[32], fiz[32];
 #pragma unroll
     for (int t = 0; t < 32; t++) fix[t] = fiy[t] = fiz[t] = (CT)0;
 
     for (int tp = it.y; tp < it.z; tp += 2) {
+      // keep the 32 i-atom LDS reads inside the tile loop (not hoisted into
+      // 160 registers)
+      asm volatile("" ::: "memory");
       const int tile = tp + half;
       const bool tv = tile < it.z;
       const int jc = tv ? tile_j[tile] : it.x;
@@ -145,20 +154,25 @@ __global__ __launch_bounds__(64) void k_pair(
       CT v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
 #pragma unroll
       for (int t = 0; t < 32; t++) {
-        const int4 pi = posq[ibase + t];
-        const int ti = types[ibase + t];
+        const int4 pi = sh_i[t];
         const CT dx = (CT)(int)(pi.x - pj.x) * P.sx;
         const CT dy = (CT)(int)(pi.y - pj.y) * P.sy;
         const CT dz = (CT)(int)(pi.z - pj.z) * P.sz;
         const CT r2 = dx * dx + dy * dy + dz * dz;
-        if (!((m >> t) & 1u) && r2 < P.roff2) {
+        if (!(m & (1u << t)) && r2 < P.roff2) {
           const CT rinv = rsqrt_ct(r2);
           const CT r = r2 * rinv;
           const CT qq = charge_of<CT>(pi.w) * qj;
           CT u_el;
           const CT f_el = ewald_pair(r2, r, rinv, qq, P, u_el);
-          const int aa = ti > tj ? ti : tj, bb = ti > tj ? tj : ti;
-          const T2 c = tab[aa * (aa + 1) / 2 + bb];
+          T2 c;
+          if (TAB_LDS) {
+            c = sh_tab[sh_row[t] + tj];
+          } else {
+            const int ti = sh_row[t];
+            const int aa = ti > tj ? ti : tj, bb = ti > tj ? tj : ti;
+            c = vdwtab[aa * (aa + 1) / 2 + bb];
+          }
           const CT rinv2 = rinv * rinv;
           const CT rinv3 = rinv * rinv2;
           const CT rinv6 = rinv3 * rinv3;
@@ -236,30 +250,39 @@ __global__ __launch_bounds__(64) void k_pair(
 
 template <typename CT>
 static void launch_pair_t(NbSystem &S, const typename Vec2<CT>::T *tab,
-                          const PairConsts<CT> &P, CT qfac, bool E, bool V, hipStream_t s) {
+                          const typename Vec2<CT>::T *sq, const PairConsts<CT> &P, CT qfac, bool E,
+                          bool V, hipStream_t s) {
   auto *f = reinterpret_cast<unsigned long long *>(S.fsort.get());
   double *part = S.parts.get() + S.part.pair_off;
   const int G = S.pair_blocks;
+  const bool lds = S.ntypes <= kMaxTypesLds;
 #define APO_PAIR_ARGS                                                                   \
   S.posq.get(), S.type_s.get(), S.items.get(), S.tile_j.get(), S.tile_mask.get(),     \
-      S.meta.get(), tab, S.ntypes, P, qfac, f, S.ssort, part
-  if (E && V)
-    k_pair<CT, true, true><<<G, 64, 0, s>>>(APO_PAIR_ARGS);
-  else if (E)
-    k_pair<CT, true, false><<<G, 64, 0, s>>>(APO_PAIR_ARGS);
-  else if (V)
-    k_pair<CT, false, true><<<G, 64, 0, s>>>(APO_PAIR_ARGS);
-  else
-    k_pair<CT, false, false><<<G, 64, 0, s>>>(APO_PAIR_ARGS);
+      S.meta.get(), tab, sq, S.ntypes, P, qfac, f, S.ssort, part
+#define APO_PAIR_GO(EE, VV)                                                             \
+  if (lds)                                                                              \
+    k_pair<CT, EE, VV, true><<<G, 64, 0, s>>>(APO_PAIR_ARGS);                          \
+  else                                                                                  \
+    k_pair<CT, EE, VV, false><<<G, 64, 0, s>>>(APO_PAIR_ARGS);
+  if (E && V) {
+    APO_PAIR_GO(true, true)
+  } else if (E) {
+    APO_PAIR_GO(true, false)
+  } else if (V) {
+    APO_PAIR_GO(false, true)
+  } else {
+    APO_PAIR_GO(false, false)
+  }
+#undef APO_PAIR_GO
 #undef APO_PAIR_ARGS
   APO_LAUNCH_CHECK();
 }
 
 void direct_launch(NbSystem &S, bool E, bool V, hipStream_t s) {
   if (S.precision == 1)
-    launch_pair_t<double>(S, S.vdwtab_d.get(), S.pc_d, kCcelec, E, V, s);
+    launch_pair_t<double>(S, S.vdwtab_d.get(), S.vdwsq_d.get(), S.pc_d, kCcelec, E, V, s);
   else
-    launch_pair_t<float>(S, S.vdwtab.get(), S.pc_f, 1.0f, E, V, s);
+    launch_pair_t<float>(S, S.vdwtab.get(), S.vdwsq.get(), S.pc_f, (float)kCcelec, E, V, s);
 }
 
 // Sorted -> global order (addSortedForce, src/CudaPMEDirectForce.cu:1375-1386).

@@ -180,44 +180,184 @@ struct GridGeom {
   double inv0, inv1, inv2;
 };
 
-// Spread: 16 lanes per atom, each lane one (y, z) row of 4 x points.  The
-// charge-assignment weights q*theta_x*theta_y*theta_z are formed in FP64 in
-// both precision modes: they enter the int64 grid exactly, and FP32 weights
-// alone would act as ~1e-7 random charge noise on every atom, which at 98k
-// atoms becomes ~2.6e-6 of the total force (FP32 FFT/gather stay below 3e-7).
+// Brick spread (no global atomics).  One block owns a brick of the grid
+// (kBx x kBy x kBz points) and accumulates, in LDS int64 2^-40 fixed point,
+// every stencil contribution that lands in it, from the atoms of the cells
+// whose (per-step) bounding box reaches the brick; the brick is then stored
+// once as CT.  The integer sums are exact, so the grid is bitwise identical to
+// any order of accumulation, and no zero fill or HBM atomic is needed.
+constexpr int kBx = 8, kBy = 8, kBz = 16;
+
+struct BrickGeom {
+  int K0, K1, K2;
+  int nbx, nby, nbz;
+  float L0, L1, L2;
+};
+
+// Cyclic-interval overlap on a circle of length K.
+__device__ __forceinline__ bool cyc_overlap(float a, float la, float b, float lb, float K) {
+  float d1 = b - a;
+  d1 -= K * floorf(d1 / K);
+  float d2 = a - b;
+  d2 -= K * floorf(d2 / K);
+  return d1 < la || d2 < lb;
+}
+
+// fr = K frac(u + 1/2) from the 32-bit fixed-point box coordinate u.
+__device__ __forceinline__ void frac_fixed(int ufix, int K, int &ix, double &w) {
+  double u = (double)(uint32_t)ufix * (1.0 / 4294967296.0) + 0.5;
+  u -= floor(u);
+  double fr = (double)K * u;
+  if (!(fr >= 0.0 && fr < (double)K)) fr = 0.0;
+  ix = (int)fr;
+  w = fr - (double)ix;
+}
+
 template <typename CT>
-__global__ __launch_bounds__(256) void k_spread(const float4 *__restrict__ xyzq, int n, GridGeom g,
-                                                unsigned long long *__restrict__ grid) {
+__global__ __launch_bounds__(256) void k_spread_brick(const int4 *__restrict__ posq,
+                                                      const float4 *__restrict__ bb,
+                                                      const int *__restrict__ meta, BrickGeom g,
+                                                      double qunscale, CT *__restrict__ grid) {
+  __shared__ unsigned long long brick[kBx * kBy * kBz];
+  __shared__ int cand[256];
+  __shared__ int ncand;
+  const int tid = threadIdx.x;
+  const int bxi = blockIdx.x % g.nbx, byi = (blockIdx.x / g.nbx) % g.nby,
+            bzi = blockIdx.x / (g.nbx * g.nby);
+  const int x0 = bxi * kBx, y0 = byi * kBy, z0 = bzi * kBz;
+  for (int k = tid; k < kBx * kBy * kBz; k += 256) brick[k] = 0ull;
+  const int ncell = meta[0];
+  // atoms whose stencil base lies in [p0 - 3, p0 + B) reach the brick
+  const float bx0 = (float)(x0 - 3), by0 = (float)(y0 - 3), bz0 = (float)(z0 - 3);
+  const float K0 = (float)g.K0, K1 = (float)g.K1, K2 = (float)g.K2;
+  for (int c0 = 0; c0 < ncell; c0 += 256) {
+    if (tid == 0) ncand = 0;
+    __syncthreads();
+    const int c = c0 + tid;
+    if (c < ncell) {
+      const float4 cc = bb[2 * c], hh = bb[2 * c + 1];
+      // cell extent in grid units (x/L + 1/2 origin), +2 for the floor and
+      // single-precision slack
+      float ax = K0 * ((cc.x - hh.x) / g.L0 + 0.5f), lax = 2.f * hh.x * K0 / g.L0 + 2.f;
+      float ay = K1 * ((cc.y - hh.y) / g.L1 + 0.5f), lay = 2.f * hh.y * K1 / g.L1 + 2.f;
+      float az = K2 * ((cc.z - hh.z) / g.L2 + 0.5f), laz = 2.f * hh.z * K2 / g.L2 + 2.f;
+      if (cyc_overlap(ax - 1.f, lax, bx0, (float)(kBx + 3), K0) &&
+          cyc_overlap(ay - 1.f, lay, by0, (float)(kBy + 3), K1) &&
+          cyc_overlap(az - 1.f, laz, bz0, (float)(kBz + 3), K2)) {
+        int slot = atomicAdd(&ncand, 1);
+        cand[slot] = c;
+      }
+    }
+    __syncthreads();
+    const int nc = ncand;
+    for (int w = tid; w < nc * kCell; w += 256) {
+      const int loc = cand[w >> 5] * kCell + (w & 31);
+      const int4 p = posq[loc];
+      const double q = (double)__int_as_float(p.w) * qunscale;
+      if (q == 0.0) continue;
+      int ix, iy, iz;
+      double wx, wy, wz;
+      frac_fixed(p.x, g.K0, ix, wx);
+      frac_fixed(p.y, g.K1, iy, wy);
+      frac_fixed(p.z, g.K2, iz, wz);
+      // offsets of the stencil base relative to the brick origin, in [-3, K-4]
+      int dx = ix - x0, dy = iy - y0, dz = iz - z0;
+      dx -= g.K0 * (int)floorf((float)(dx + 3) / K0);
+      dy -= g.K1 * (int)floorf((float)(dy + 3) / K1);
+      dz -= g.K2 * (int)floorf((float)(dz + 3) / K2);
+      if (dx >= kBx || dy >= kBy || dz >= kBz) continue;
+      double tx[4], ty[4], tz[4], d[4];
+      bspline4<double>(wx, tx, d);
+      bspline4<double>(wy, ty, d);
+      bspline4<double>(wz, tz, d);
+      const double qs = q * kGridScale;
+#pragma unroll
+      for (int cz = 0; cz < 4; cz++) {
+        const int pz = dz + cz;
+        if (pz < 0 || pz >= kBz) continue;
+#pragma unroll
+        for (int cy = 0; cy < 4; cy++) {
+          const int py = dy + cy;
+          if (py < 0 || py >= kBy) continue;
+          const double qyz = qs * ty[cy] * tz[cz];
+#pragma unroll
+          for (int cx = 0; cx < 4; cx++) {
+            const int px = dx + cx;
+            if (px < 0 || px >= kBx) continue;
+            atomicAdd(&brick[(pz * kBy + py) * kBx + px],
+                      (unsigned long long)(long long)__builtin_rint(qyz * tx[cx]));
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // store the owned points (edge bricks may overhang the grid)
+  for (int k = tid; k < kBx * kBy * kBz; k += 256) {
+    const int px = k % kBx, py = (k / kBx) % kBy, pz = k / (kBx * kBy);
+    const int gx = x0 + px, gy = y0 + py, gz = z0 + pz;
+    if (gx < g.K0 && gy < g.K1 && gz < g.K2)
+      grid[((size_t)gz * g.K1 + gy) * g.K0 + gx] =
+          (CT)((double)(long long)brick[k] * (1.0 / kGridScale));
+  }
+}
+
+// Gather in cell order: 16 lanes per atom; the gradient is added to the
+// sorted gradient buffer (one writer per atom, no atomics).
+template <typename CT>
+__global__ __launch_bounds__(256) void k_gather_sorted(const int4 *__restrict__ posq,
+                                                       const int *__restrict__ meta, GridGeom g,
+                                                       double qunscale, const CT *__restrict__ phi,
+                                                       long long *__restrict__ fsort, int ss) {
   const int gt = blockIdx.x * 256 + threadIdx.x;
-  const int atom = gt >> 4, row = gt & 15;
-  if (atom >= n) return;
-  const float4 p = xyzq[atom];
+  const int loc = gt >> 4, row = gt & 15;
+  const bool valid = loc < meta[0] * kCell;
+  const int4 p = valid ? posq[loc] : make_int4(0, 0, 0, 0);
+  const double q = (double)__int_as_float(p.w) * qunscale;
   int ix, iy, iz;
   double wx, wy, wz;
-  frac_coord(p.x, g.inv0, g.K0, ix, wx);
-  frac_coord(p.y, g.inv1, g.K1, iy, wy);
-  frac_coord(p.z, g.inv2, g.K2, iz, wz);
-  double tx[4], ty[4], tz[4], d[4];
-  bspline4<double>(wx, tx, d);
-  bspline4<double>(wy, ty, d);
-  bspline4<double>(wz, tz, d);
+  frac_fixed(p.x, g.K0, ix, wx);
+  frac_fixed(p.y, g.K1, iy, wy);
+  frac_fixed(p.z, g.K2, iz, wz);
+  CT tx[4], ty[4], tz[4], dx[4], dy[4], dz[4];
+  bspline4<CT>((CT)wx, tx, dx);
+  bspline4<CT>((CT)wy, ty, dy);
+  bspline4<CT>((CT)wz, tz, dz);
   const int a = row & 3, b = row >> 2;
   int y = iy + a, z = iz + b;
   if (y >= g.K1) y -= g.K1;
   if (z >= g.K2) z -= g.K2;
-  const double qyz = (double)p.w * ty[a] * tz[b] * kGridScale;
-  const size_t base = (size_t)g.K0 * ((size_t)y + (size_t)g.K1 * z);
+  CT f1 = 0, f2 = 0, f3 = 0;
+  if (valid && q != 0.0) {
+    const size_t base = (size_t)g.K0 * ((size_t)y + (size_t)g.K1 * z);
 #pragma unroll
-  for (int c = 0; c < 4; c++) {
-    int x = ix + c;
-    if (x >= g.K0) x -= g.K0;
-    atomicAdd(&grid[base + x], (unsigned long long)(long long)__builtin_rint(qyz * tx[c]));
+    for (int c = 0; c < 4; c++) {
+      int x = ix + c;
+      if (x >= g.K0) x -= g.K0;
+      const CT v = phi[base + x];
+      f1 += dx[c] * ty[a] * tz[b] * v;
+      f2 += tx[c] * dy[a] * tz[b] * v;
+      f3 += tx[c] * ty[a] * dz[b] * v;
+    }
+  }
+#pragma unroll
+  for (int off = 1; off < 16; off <<= 1) {
+    f1 += __shfl_xor(f1, off);
+    f2 += __shfl_xor(f2, off);
+    f3 += __shfl_xor(f3, off);
+  }
+  if (valid && q != 0.0 && row == 0) {
+    const double qc = q * kCcelec;
+    fsort[loc] += (long long)__builtin_rint(qc * g.inv0 * g.K0 * (double)f1 * kForceScale);
+    fsort[ss + loc] += (long long)__builtin_rint(qc * g.inv1 * g.K1 * (double)f2 * kForceScale);
+    fsort[2 * (size_t)ss + loc] +=
+        (long long)__builtin_rint(qc * g.inv2 * g.K2 * (double)f3 * kForceScale);
   }
 }
 
 // x forward: int64 rows -> complex half rows, two real rows per complex FFT.
 template <typename CT>
-__global__ __launch_bounds__(256) void k_fftx_fwd(const long long *__restrict__ grid, int nrows,
+__global__ __launch_bounds__(256) void k_fftx_fwd(const CT *__restrict__ grid, int nrows,
                                                   FftPlan p, int rb,
                                                   const typename Cx<CT>::T *__restrict__ twg,
                                                   typename Cx<CT>::T *__restrict__ out) {
@@ -233,8 +373,8 @@ __global__ __launch_bounds__(256) void k_fftx_fwd(const long long *__restrict__ 
     const int r = e / n, x = e - r * n;
     const int ra = 2 * (r0 + r), rbw = ra + 1;
     C2 v;
-    v.x = ra < nrows ? (CT)((double)grid[(size_t)ra * n + x] * (1.0 / kGridScale)) : (CT)0;
-    v.y = rbw < nrows ? (CT)((double)grid[(size_t)rbw * n + x] * (1.0 / kGridScale)) : (CT)0;
+    v.x = ra < nrows ? grid[(size_t)ra * n + x] : (CT)0;
+    v.y = rbw < nrows ? grid[(size_t)rbw * n + x] : (CT)0;
     a[e] = v;
   }
   __syncthreads();
@@ -426,60 +566,6 @@ __global__ __launch_bounds__(256) void k_fftz_conv(typename Cx<CT>::T *__restric
   }
 }
 
-// Gather: 16 lanes per atom; gradient q ccelec (K/L) sum dtheta theta theta phi.
-template <typename CT>
-__global__ __launch_bounds__(256) void k_gather(const float4 *__restrict__ xyzq, int n, GridGeom g,
-                                                const CT *__restrict__ phi,
-                                                unsigned long long *__restrict__ f, int stride) {
-  const int gt = blockIdx.x * 256 + threadIdx.x;
-  const int atom = gt >> 4, row = gt & 15;
-  const bool valid = atom < n;
-  float4 p = valid ? xyzq[atom] : make_float4(0, 0, 0, 0);
-  int ix, iy, iz;
-  double wx, wy, wz;
-  frac_coord(p.x, g.inv0, g.K0, ix, wx);
-  frac_coord(p.y, g.inv1, g.K1, iy, wy);
-  frac_coord(p.z, g.inv2, g.K2, iz, wz);
-  CT tx[4], ty[4], tz[4], dx[4], dy[4], dz[4];
-  bspline4<CT>((CT)wx, tx, dx);
-  bspline4<CT>((CT)wy, ty, dy);
-  bspline4<CT>((CT)wz, tz, dz);
-  const int a = row & 3, b = row >> 2;
-  int y = iy + a, z = iz + b;
-  if (y >= g.K1) y -= g.K1;
-  if (z >= g.K2) z -= g.K2;
-  CT f1 = 0, f2 = 0, f3 = 0;
-  if (valid) {
-    const size_t base = (size_t)g.K0 * ((size_t)y + (size_t)g.K1 * z);
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      int x = ix + c;
-      if (x >= g.K0) x -= g.K0;
-      const CT v = phi[base + x];
-      f1 += dx[c] * ty[a] * tz[b] * v;
-      f2 += tx[c] * dy[a] * tz[b] * v;
-      f3 += tx[c] * ty[a] * dz[b] * v;
-    }
-  }
-#pragma unroll
-  for (int off = 1; off < 16; off <<= 1) {
-    f1 += __shfl_xor(f1, off);
-    f2 += __shfl_xor(f2, off);
-    f3 += __shfl_xor(f3, off);
-  }
-  if (valid && row == 0) {
-    const double q = (double)p.w * kCcelec;
-    const double gx = q * g.inv0 * g.K0 * (double)f1;
-    const double gy = q * g.inv1 * g.K1 * (double)f2;
-    const double gz = q * g.inv2 * g.K2 * (double)f3;
-    atomicAdd(&f[atom], (unsigned long long)(long long)__builtin_rint(gx * kForceScale));
-    atomicAdd(&f[(size_t)stride + atom],
-              (unsigned long long)(long long)__builtin_rint(gy * kForceScale));
-    atomicAdd(&f[2 * (size_t)stride + atom],
-              (unsigned long long)(long long)__builtin_rint(gz * kForceScale));
-  }
-}
-
 template <typename CT>
 static void recip_t(NbSystem &S, const float4 *x, int64_t *force, int stride, bool EV,
                     typename Cx<CT>::T *cgrid, CT *phi, const typename Cx<CT>::T *twx,
@@ -487,13 +573,15 @@ static void recip_t(NbSystem &S, const float4 *x, int64_t *force, int stride, bo
                     hipStream_t s) {
   using C2 = typename Cx<CT>::T;
   const int K0 = S.K[0], K1 = S.K[1], K2 = S.K[2], H = K0 / 2 + 1;
-  const size_t ng = (size_t)K0 * K1 * K2;
   GridGeom g{K0, K1, K2, 1.0 / S.box[0], 1.0 / S.box[1], 1.0 / S.box[2]};
+  const double qunscale = 1.0;  // sorted coordinates carry the raw charge
   hipEvent_t t0 = nullptr;
   S.tic(1, s, t0);
-  launch_fill_i64(S.qgrid.get(), ng, s);
-  k_spread<CT><<<(S.natom * 16 + 255) / 256, 256, 0, s>>>(
-      x, S.natom, g, reinterpret_cast<unsigned long long *>(S.qgrid.get()));
+  BrickGeom bg{K0, K1, K2, (K0 + kBx - 1) / kBx, (K1 + kBy - 1) / kBy, (K2 + kBz - 1) / kBz,
+               (float)S.box[0], (float)S.box[1], (float)S.box[2]};
+  CT *rgrid = reinterpret_cast<CT *>(S.qgrid.get());
+  k_spread_brick<CT><<<bg.nbx * bg.nby * bg.nbz, 256, 0, s>>>(S.posq.get(), S.cell_bb.get(),
+                                                              S.meta.get(), bg, qunscale, rgrid);
   APO_LAUNCH_CHECK();
   S.toc(1, s, t0);
   S.tic(2, s, t0);
@@ -503,8 +591,8 @@ static void recip_t(NbSystem &S, const float4 *x, int64_t *force, int stride, bo
   int rb = (int)std::max<size_t>(1, std::min<size_t>(16, (lds_cap / sizeof(C2) - K0) / (2 * K0)));
   size_t shx = (size_t)(K0 + 2 * rb * K0) * sizeof(C2);
   const int npair = (nrows + 1) / 2;
-  k_fftx_fwd<CT><<<(npair + rb - 1) / rb, 256, shx, s>>>(
-      reinterpret_cast<const long long *>(S.qgrid.get()), nrows, plan_of(S.fx), rb, twx, cgrid);
+  k_fftx_fwd<CT><<<(npair + rb - 1) / rb, 256, shx, s>>>(rgrid, nrows, plan_of(S.fx), rb, twx,
+                                                        cgrid);
   APO_LAUNCH_CHECK();
   // y forward
   const int W = 16;
@@ -538,8 +626,9 @@ static void recip_t(NbSystem &S, const float4 *x, int64_t *force, int stride, bo
   APO_LAUNCH_CHECK();
   S.toc(2, s, t0);
   S.tic(3, s, t0);
-  k_gather<CT><<<(S.natom * 16 + 255) / 256, 256, 0, s>>>(
-      x, S.natom, g, phi, reinterpret_cast<unsigned long long *>(force), stride);
+  k_gather_sorted<CT><<<(S.ssort * 16 + 255) / 256, 256, 0, s>>>(
+      S.posq.get(), S.meta.get(), g, qunscale, phi, reinterpret_cast<long long *>(S.fsort.get()),
+      S.ssort);
   APO_LAUNCH_CHECK();
   S.toc(3, s, t0);
 }

@@ -237,7 +237,7 @@ void NbSystem::setup_columns() {
 // ---------------------------------------------------------------------------
 // creation / calc_force
 // ---------------------------------------------------------------------------
-#include "../../include/apocharmm_amd.h"
+#include "apocharmm_amd.h"
 
 namespace apo {
 
@@ -295,6 +295,18 @@ NbSystem *create_system(const apo_nb_params &p, const apo_nb_topology &t) {
       tab14[k] = make_float2(6.0f * t14[2 * k], 12.0f * t14[2 * k + 1]);
       tabd[k] = make_double2(6.0 * (double)t.vdwtab[2 * k], 12.0 * (double)t.vdwtab[2 * k + 1]);
       tab14d[k] = make_double2(6.0 * (double)t14[2 * k], 12.0 * (double)t14[2 * k + 1]);
+    }
+    if (t.ntypes <= 32) {
+      std::vector<float2> sq((size_t)t.ntypes * t.ntypes);
+      std::vector<double2> sqd((size_t)t.ntypes * t.ntypes);
+      for (int a = 0; a < t.ntypes; a++)
+        for (int b = 0; b < t.ntypes; b++) {
+          int hi = a > b ? a : b, lo = a > b ? b : a;
+          sq[(size_t)a * t.ntypes + b] = tab[hi * (hi + 1) / 2 + lo];
+          sqd[(size_t)a * t.ntypes + b] = tabd[hi * (hi + 1) / 2 + lo];
+        }
+      S->vdwsq.upload(sq, 0);
+      S->vdwsq_d.upload(sqd, 0);
     }
     S->vdwtab.upload(tab, 0);
     S->vdwtab14.upload(tab14, 0);
@@ -436,7 +448,9 @@ void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int s
   hipEvent_t t_all = nullptr, t0 = nullptr;
   S.tic(5, s, t_all);
   launch_fill_i64(force, 3 * (size_t)stride, s);
-  if (terms & APO_TERM_DIRECT) {
+  const bool dir = terms & APO_TERM_DIRECT, rec = terms & APO_TERM_RECIP;
+  if (dir || rec) {
+    // both the pair kernel and PME work on the cell-sorted coordinates
     if ((flags & APO_RESET_LIST) || !S.list_valid) {
       S.tic(4, s, t0);
       nblist_build(S, x4, s);
@@ -444,10 +458,16 @@ void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int s
     }
     S.tic(7, s, t0);
     nblist_sort_coords(S, x4, s);
+    if (rec) nblist_cellbb(S, s);
     S.toc(7, s, t0);
+  }
+  if (dir) {
     S.tic(0, s, t0);
     direct_launch(S, E, V, s);
     S.toc(0, s, t0);
+  }
+  if (rec) recip_launch(S, x4, force, stride, E, V, s);
+  if (dir || rec) {
     S.tic(7, s, t0);
     direct_finalize(S, force, stride, s);
     S.toc(7, s, t0);
@@ -456,7 +476,6 @@ void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int s
   if (terms & APO_TERM_PAIRS14) pairs14_launch(S, x4, force, stride, E, V, s);
   if (terms & APO_TERM_BONDED) bonded_launch(S, x4, force, stride, E, V, s);
   S.toc(6, s, t0);
-  if (terms & APO_TERM_RECIP) recip_launch(S, x4, force, stride, E, V, s);
   if ((terms & APO_TERM_SELF) && E) self_launch(S, x4, s);
   if (E || V) reduce_energy_launch(S, terms, E, V, energy, virial, s);
   S.toc(5, s, t_all);

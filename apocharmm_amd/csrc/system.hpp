@@ -38,6 +38,8 @@ struct NbSystem {
   DevBuf<float2> vdwtab14;
   DevBuf<double2> vdwtab_d;   // FP64 instantiation
   DevBuf<double2> vdwtab14_d;
+  DevBuf<float2> vdwsq;       // square (ntypes x ntypes) copy for <= 32 types
+  DevBuf<double2> vdwsq_d;
   DevBuf<int> excl_ptr, excl_idx;
   int nin14 = 0, nex14 = 0;
   DevBuf<int2> in14, ex14;

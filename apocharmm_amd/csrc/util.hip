@@ -3,7 +3,7 @@
 #include <algorithm>
 #include <set>
 
-#include "../../include/apocharmm_amd.h"
+#include "apocharmm_amd.h"
 #include "system.hpp"
 
 namespace apo {
