@@ -62,6 +62,15 @@ class NbTopology(ctypes.Structure):
                 ("nimdihe", ctypes.c_int32), ("imdihe", _i32p), ("imdihe_par", _f32p)]
 
 
+class LtParams(ctypes.Structure):
+    _fields_ = [("timestep", ctypes.c_double), ("temperature", ctypes.c_double),
+                ("friction", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("nbfreq", ctypes.c_int32), ("comfreq", ctypes.c_int32),
+                ("nwater", ctypes.c_int32), ("use_graph", ctypes.c_int32),
+                ("waters", _i32p), ("water_geom", ctypes.c_double * 4),
+                ("mass", ctypes.POINTER(ctypes.c_double)), ("charge", _f32p)]
+
+
 _LIB = None
 
 
@@ -86,9 +95,17 @@ def lib():
         L.apo_nb_force_to_double.argtypes = [vp, i32, i32, vp, vp]
         L.apo_nb_list_stats.argtypes = [vp, i64p, vp]
         L.apo_topology_exclusions.argtypes = [i32, i32, vp, _i32p, vp, vp, vp, vp]
+        L.apo_lt_create.argtypes = [vp, ctypes.POINTER(LtParams), vp, vp, ctypes.POINTER(vp)]
+        L.apo_lt_destroy.argtypes = [vp]
+        L.apo_lt_initialize.argtypes = [vp, vp]
+        L.apo_lt_propagate.argtypes = [vp, i32, vp]
+        L.apo_lt_get_state.argtypes = [vp, vp, vp, vp, vp, vp, i64p]
+        L.apo_lt_xyzq.argtypes = [vp, ctypes.POINTER(vp)]
         for name in ("apo_nb_create", "apo_nb_destroy", "apo_nb_set_box", "apo_nb_reset_list",
                      "apo_nb_calc_force", "apo_nb_force_stride", "apo_nb_num_atoms",
-                     "apo_nb_force_to_double", "apo_nb_list_stats", "apo_topology_exclusions"):
+                     "apo_nb_force_to_double", "apo_nb_list_stats", "apo_topology_exclusions",
+                     "apo_lt_create", "apo_lt_destroy", "apo_lt_initialize", "apo_lt_propagate",
+                     "apo_lt_get_state", "apo_lt_xyzq"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
     return _LIB

@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libapocharmm_amd.so")
 OBJ = os.path.join(HERE, "_build")
 
-SOURCES = ["system.hip", "util.hip", "nblist.hip", "pair.hip", "bonded.hip", "pme.hip", "capi.cpp"]
+SOURCES = ["system.hip", "util.hip", "nblist.hip", "pair.hip", "bonded.hip", "pme.hip", "md.hip", "capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

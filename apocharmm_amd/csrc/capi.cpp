@@ -16,8 +16,23 @@ void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int s
                 double *energy, double *virial, hipStream_t s);
 }  // namespace apo
 
+namespace apo {
+struct LtSystem;
+LtSystem *create_lt(NbSystem *nb, const apo_lt_params &p, const double *xyz, const double *vel);
+void destroy_lt(LtSystem *L);
+void lt_initialize(LtSystem &L, hipStream_t caller);
+void lt_propagate(LtSystem &L, int nsteps, hipStream_t caller);
+void lt_get_state(LtSystem &L, double *xyz, double *vel, double *dprev, double *grad,
+                  double *thermo, int64_t *step);
+const float *lt_xyzq(LtSystem &L);
+}  // namespace apo
+
 struct apo_nb_system {
   apo::NbSystem *sys;
+};
+
+struct apo_lt_integrator {
+  apo::LtSystem *lt;
 };
 
 namespace {
@@ -53,6 +68,10 @@ apo::NbSystem &get(apo_nb_system *h) {
 const apo::NbSystem &get(const apo_nb_system *h) {
   if (!h || !h->sys) throw apo::InvalidArgument("null system handle");
   return *h->sys;
+}
+apo::LtSystem &get(apo_lt_integrator *h) {
+  if (!h || !h->lt) throw apo::InvalidArgument("null integrator handle");
+  return *h->lt;
 }
 }  // namespace
 
@@ -142,21 +161,21 @@ APO_API apo_status apo_nb_profile(apo_nb_system *sys, int32_t enable) {
     apo::NbSystem &S = get(sys);
     S.profile_flush();
     S.prof_on = enable != 0;
-    for (int k = 0; k < 8; k++) {
+    for (int k = 0; k < APO_PROFILE_SLOTS; k++) {
       S.prof_ms[k] = 0;
       S.prof_cnt[k] = 0;
     }
   });
 }
 
-APO_API apo_status apo_nb_profile_read(apo_nb_system *sys, double *ms8, int64_t *count8) {
+APO_API apo_status apo_nb_profile_read(apo_nb_system *sys, double *ms, int64_t *count) {
   return guard([&] {
-    if (!ms8 || !count8) throw apo::InvalidArgument("null output");
+    if (!ms || !count) throw apo::InvalidArgument("null output");
     apo::NbSystem &S = get(sys);
     S.profile_flush();
-    for (int k = 0; k < 8; k++) {
-      ms8[k] = S.prof_ms[k];
-      count8[k] = S.prof_cnt[k];
+    for (int k = 0; k < APO_PROFILE_SLOTS; k++) {
+      ms[k] = S.prof_ms[k];
+      count[k] = S.prof_cnt[k];
     }
   });
 }
@@ -178,6 +197,45 @@ APO_API apo_status apo_topology_exclusions(int32_t natom, int32_t nbond, const i
     if (excl_idx) std::memcpy(excl_idx, x.data(), x.size() * sizeof(int32_t));
     if (in14) std::memcpy(in14, i14.data(), i14.size() * sizeof(int32_t));
     if (ex14) std::memcpy(ex14, e14.data(), e14.size() * sizeof(int32_t));
+  });
+}
+
+APO_API apo_status apo_lt_create(apo_nb_system *sys, const apo_lt_params *params,
+                                 const double *xyz, const double *vel, apo_lt_integrator **out) {
+  return guard([&] {
+    if (!params || !out) throw apo::InvalidArgument("null argument");
+    *out = nullptr;
+    apo::LtSystem *L = apo::create_lt(&get(sys), *params, xyz, vel);
+    *out = new apo_lt_integrator{L};
+  });
+}
+
+APO_API apo_status apo_lt_destroy(apo_lt_integrator *lt) {
+  return guard([&] {
+    if (!lt) return;
+    apo::destroy_lt(lt->lt);
+    delete lt;
+  });
+}
+
+APO_API apo_status apo_lt_initialize(apo_lt_integrator *lt, void *stream) {
+  return guard([&] { apo::lt_initialize(get(lt), static_cast<hipStream_t>(stream)); });
+}
+
+APO_API apo_status apo_lt_propagate(apo_lt_integrator *lt, int32_t nsteps, void *stream) {
+  return guard([&] { apo::lt_propagate(get(lt), nsteps, static_cast<hipStream_t>(stream)); });
+}
+
+APO_API apo_status apo_lt_get_state(apo_lt_integrator *lt, double *xyz, double *vel,
+                                    double *dprev, double *grad, double *thermo6,
+                                    int64_t *step) {
+  return guard([&] { apo::lt_get_state(get(lt), xyz, vel, dprev, grad, thermo6, step); });
+}
+
+APO_API apo_status apo_lt_xyzq(apo_lt_integrator *lt, const float **d_xyzq) {
+  return guard([&] {
+    if (!d_xyzq) throw apo::InvalidArgument("null output");
+    *d_xyzq = apo::lt_xyzq(get(lt));
   });
 }
 

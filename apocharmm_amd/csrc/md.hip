@@ -1,0 +1,801 @@
+// Langevin thermostat leapfrog integrator with SETTLE water constraints on
+// top of the force path (CudaLangevinThermostatIntegrator,
+// src/CudaLangevinThermostatIntegrator.cu:206-258, 576-800, and
+// CudaHolonomicConstraint, src/CudaHolonomicConstraint.cu:164-303).
+//
+// State lives in HBM in FP64 SoA (x, previous half-step displacement, on-step
+// velocity), plus the float4 xyzq the force path reads and the int64 2^-40
+// gradient it writes.  One step is:
+//   [rebuild steps: image centering + list rebuild, eager]
+//   calc_force(xyzq) -> int64 gradient
+//   k_lt_step: one thread per water (3 atoms, SETTLE in registers) or per
+//              free atom: leapfrog Langevin update with Philox noise, SETTLE
+//              against the start-of-step coordinates, velocity, kinetic
+//              energy partials, the float xyzq for the next force call
+//   k_lt_finalize: fixed-order kinetic energy sum, temperatures, step + 1
+// so the integrator costs one pass over the state instead of the reference's
+// five kernels and two device copies.  Steps without a rebuild or centre of
+// mass removal replay a hipGraph of that sequence.
+#include <cmath>
+#include <vector>
+
+#include "apocharmm_amd.h"
+#include "system.hpp"
+
+namespace apo {
+
+void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int stride,
+                double *energy, double *virial, hipStream_t s);
+
+namespace {
+
+constexpr double kTimfac = 0.0488882129;   // AKMA time unit in ps (src/CudaIntegrator.cu:22)
+constexpr double kBoltz = 1.987191e-03;    // kcal/mol/K (include/Constants.h:17)
+constexpr unsigned kPhiloxM0 = 0xD2511F53u, kPhiloxM1 = 0xCD9E8D57u;
+constexpr unsigned kPhiloxW0 = 0x9E3779B9u, kPhiloxW1 = 0xBB67AE85u;
+constexpr unsigned kLangevinTag = 0x4C414E47u;
+constexpr int kStepBlock = 64;             // one wave per block in the step kernels
+
+// ---- Philox4x32-10 (Salmon et al., SC'11) and Box-Muller normals ----------
+// Counter (atom, step_lo, step_hi, tag), key (seed_lo, seed_hi): stateless, so
+// a graph replay or a restart at any step draws the same numbers.
+__device__ inline void philox4x32_10(unsigned c[4], unsigned k0, unsigned k1) {
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    if (r > 0) {
+      k0 += kPhiloxW0;
+      k1 += kPhiloxW1;
+    }
+    const unsigned h0 = __umulhi(kPhiloxM0, c[0]), l0 = kPhiloxM0 * c[0];
+    const unsigned h1 = __umulhi(kPhiloxM1, c[2]), l1 = kPhiloxM1 * c[2];
+    const unsigned n0 = h1 ^ c[1] ^ k0, n2 = h0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = l1;
+    c[2] = n2;
+    c[3] = l0;
+  }
+}
+
+__device__ inline double3 langevin_normals(unsigned long long seed, unsigned long long step,
+                                           int atom) {
+  unsigned c[4] = {(unsigned)atom, (unsigned)step, (unsigned)(step >> 32), kLangevinTag};
+  philox4x32_10(c, (unsigned)seed, (unsigned)(seed >> 32));
+  const double s = 2.3283064365386963e-10;  // 2^-32
+  const double a0 = ((double)c[0] + 0.5) * s, a1 = ((double)c[1] + 0.5) * s;
+  const double a2 = ((double)c[2] + 0.5) * s, a3 = ((double)c[3] + 0.5) * s;
+  const double two_pi = 6.283185307179586;
+  const double r0 = sqrt(-2.0 * log(a0)), r1 = sqrt(-2.0 * log(a2));
+  return make_double3(r0 * cos(two_pi * a1), r0 * sin(two_pi * a1), r1 * cos(two_pi * a3));
+}
+
+// ---- small double3 algebra -----------------------------------------------------
+__device__ inline double3 operator-(double3 a, double3 b) {
+  return make_double3(a.x - b.x, a.y - b.y, a.z - b.z);
+}
+__device__ inline double dot3(double3 a, double3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline double3 cross3(double3 a, double3 b) {
+  return make_double3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ inline double3 unit3(double3 a) {
+  const double s = 1.0 / sqrt(dot3(a, a));
+  return make_double3(a.x * s, a.y * s, a.z * s);
+}
+__device__ inline double3 frame_to_lab(double3 o, double3 ex, double3 ey, double3 ez, double u,
+                                       double v, double w) {
+  return make_double3(o.x + ex.x * u + ey.x * v + ez.x * w, o.y + ex.y * u + ey.y * v + ez.y * w,
+                      o.z + ex.z * u + ey.z * v + ez.z * w);
+}
+
+struct SettleGeom {
+  double ra, rb, rc;  // centre of mass to O, to the H-H midpoint, half H-H
+  double wO, wH;      // mass fractions
+};
+
+// SETTLE (Miyamoto & Kollman 1992): the analytic solution of the three
+// distance constraints with displacements along the reference bonds.  The
+// constraint displacements lie in the reference plane, so in the frame with
+// z along the reference normal every atom keeps its z; the canonical triangle
+// is tilted (phi about x, psi about y) to those heights, then turned about z
+// by theta with alpha sin(theta) + beta cos(theta) = gamma, the condition that
+// the mass-weighted moment of the displacements about the reference vanishes.
+// Same equations and root as src/CudaHolonomicConstraint.cu:164-303.
+__device__ inline void settle(const SettleGeom &G, double3 A0, double3 B0, double3 C0, double3 &A1,
+                              double3 &B1, double3 &C1) {
+  const double3 com = make_double3(G.wO * A1.x + G.wH * (B1.x + C1.x),
+                                   G.wO * A1.y + G.wH * (B1.y + C1.y),
+                                   G.wO * A1.z + G.wH * (B1.z + C1.z));
+  const double3 a1 = A1 - com, b1 = B1 - com, c1 = C1 - com;
+  const double3 b0 = B0 - A0, c0 = C0 - A0;
+  double3 ez = cross3(b0, c0);
+  double3 ex = cross3(a1, ez);
+  double3 ey = cross3(ez, ex);
+  ex = unit3(ex);
+  ey = unit3(ey);
+  ez = unit3(ez);
+  const double Xb0 = dot3(ex, b0), Yb0 = dot3(ey, b0);
+  const double Xc0 = dot3(ex, c0), Yc0 = dot3(ey, c0);
+  const double Za1 = dot3(ez, a1);
+  const double Xb1 = dot3(ex, b1), Yb1 = dot3(ey, b1), Zb1 = dot3(ez, b1);
+  const double Xc1 = dot3(ex, c1), Yc1 = dot3(ey, c1), Zc1 = dot3(ez, c1);
+  const double sphi = Za1 / G.ra;
+  const double cphi = sqrt(1.0 - sphi * sphi);
+  const double spsi = (Zb1 - Zc1) / (2.0 * G.rc * cphi);
+  const double cpsi = sqrt(1.0 - spsi * spsi);
+  const double Ya2 = G.ra * cphi;
+  const double Xb2 = -G.rc * cpsi;
+  const double Yb2 = -G.rb * cphi - G.rc * spsi * sphi;
+  const double Yc2 = -G.rb * cphi + G.rc * spsi * sphi;
+  const double al = Xb2 * (Xb0 - Xc0) + Yb0 * Yb2 + Yc0 * Yc2;
+  const double be = Xb2 * (Yc0 - Yb0) + Xb0 * Yb2 + Xc0 * Yc2;
+  const double ga = Xb0 * Yb1 - Xb1 * Yb0 + Xc0 * Yc1 - Xc1 * Yc0;
+  const double s2 = al * al + be * be;
+  const double sth = (al * ga - be * sqrt(s2 - ga * ga)) / s2;
+  const double cth = sqrt(1.0 - sth * sth);
+  A1 = frame_to_lab(com, ex, ey, ez, -Ya2 * sth, Ya2 * cth, Za1);
+  B1 = frame_to_lab(com, ex, ey, ez, Xb2 * cth - Yb2 * sth, Xb2 * sth + Yb2 * cth, Zb1);
+  C1 = frame_to_lab(com, ex, ey, ez, -Xb2 * cth - Yc2 * sth, -Xb2 * sth + Yc2 * cth, Zc1);
+}
+
+struct LtState {
+  double *x, *dp, *v;        // SoA, stride n
+  const double *invmass;
+  const float *charge;
+  float4 *xyzq;
+  const int64_t *grad;       // SoA, stride gstride, 2^-40
+  int gstride;
+  int n;
+  const int *waters;         // 3 * nwater
+  int nwater;
+  const int *singles;        // atoms in no water
+  int nsingle;
+};
+
+struct LtCoef {
+  double alpha;  // (1 - gamma/2) / (1 + gamma/2)
+  double den;    // 1 / (1 + gamma/2)
+  double dt, dt2, gamma, kbt;
+  double fv;     // 0.5 sqrt(1 + gamma/2) / dt
+  double idt;
+  unsigned long long seed;
+  SettleGeom G;
+};
+
+__device__ inline double3 ld3(const double *a, int n, int i) {
+  return make_double3(a[i], a[n + i], a[2 * n + i]);
+}
+__device__ inline void st3(double *a, int n, int i, double3 v) {
+  a[i] = v.x;
+  a[n + i] = v.y;
+  a[2 * n + i] = v.z;
+}
+__device__ inline double3 ldgrad(const LtState &S, int i) {
+  const double s = 9.094947017729282e-13;  // 2^-40
+  return make_double3((double)S.grad[i] * s, (double)S.grad[S.gstride + i] * s,
+                      (double)S.grad[2 * (size_t)S.gstride + i] * s);
+}
+__device__ inline void store_xyzq(const LtState &S, int i, double3 x) {
+  S.xyzq[i] = make_float4((float)x.x, (float)x.y, (float)x.z, S.charge[i]);
+}
+
+// Unconstrained leapfrog Langevin position for one atom
+// (src/CudaLangevinThermostatIntegrator.cu:576-614).
+__device__ inline double3 lt_position(const LtState &S, const LtCoef &C, unsigned long long step,
+                                      int i, double3 x, double3 dp) {
+  const double im = S.invmass[i];
+  const double3 rn = langevin_normals(C.seed, step, i);
+  const double rdf = sqrt(2.0 * C.gamma * C.kbt / im) / C.dt;
+  const double fact = C.dt2 * im * C.den;
+  const double3 g = ldgrad(S, i);
+  const double gx = g.x + rdf * rn.x, gy = g.y + rdf * rn.y, gz = g.z + rdf * rn.z;
+  return make_double3(x.x + (C.alpha * dp.x - fact * gx), x.y + (C.alpha * dp.y - fact * gy),
+                      x.z + (C.alpha * dp.z - fact * gz));
+}
+
+// Velocity, kinetic energies, state store for one atom
+// (src/CudaLangevinThermostatIntegrator.cu:616-682, 123-136).
+__device__ inline void lt_finish(const LtState &S, const LtCoef &C, int i, double3 xr, double3 xn,
+                                 double3 dp, double &ke0, double &ke1) {
+  const double im = S.invmass[i];
+  const double3 dn = xn - xr;
+  const double3 v = make_double3(C.fv * (dn.x + dp.x), C.fv * (dn.y + dp.y), C.fv * (dn.z + dp.z));
+  const double3 u1 = make_double3(dp.x * C.idt, dp.y * C.idt, dp.z * C.idt);
+  const double3 u2 = make_double3(dn.x * C.idt, dn.y * C.idt, dn.z * C.idt);
+  const double kold = 0.5 * dot3(u1, u1) / im;
+  const double kon = 0.5 * dot3(v, v) / im;
+  const double knew = 0.5 * dot3(u2, u2) / im;
+  ke0 += (1.0 / 3.0) * (kold + kon + knew);
+  ke1 += kon;
+  st3(S.x, S.n, i, xn);
+  st3(S.dp, S.n, i, dn);
+  st3(S.v, S.n, i, v);
+  store_xyzq(S, i, xn);
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// One thread per water (first nwater threads) or free atom (the rest).
+__global__ __launch_bounds__(kStepBlock) void k_lt_step(LtState S, LtCoef C,
+                                                        const unsigned long long *__restrict__ stepc,
+                                                        double2 *__restrict__ part) {
+  const int u = blockIdx.x * kStepBlock + threadIdx.x;
+  const unsigned long long step = stepc[0];
+  double ke0 = 0.0, ke1 = 0.0;
+  if (u < S.nwater) {
+    const int ia = S.waters[3 * u], ib = S.waters[3 * u + 1], ic = S.waters[3 * u + 2];
+    const double3 ra = ld3(S.x, S.n, ia), rb = ld3(S.x, S.n, ib), rc = ld3(S.x, S.n, ic);
+    const double3 pa = ld3(S.dp, S.n, ia), pb = ld3(S.dp, S.n, ib), pc = ld3(S.dp, S.n, ic);
+    double3 na = lt_position(S, C, step, ia, ra, pa);
+    double3 nb = lt_position(S, C, step, ib, rb, pb);
+    double3 nc = lt_position(S, C, step, ic, rc, pc);
+    settle(C.G, ra, rb, rc, na, nb, nc);
+    lt_finish(S, C, ia, ra, na, pa, ke0, ke1);
+    lt_finish(S, C, ib, rb, nb, pb, ke0, ke1);
+    lt_finish(S, C, ic, rc, nc, pc, ke0, ke1);
+  } else if (u < S.nwater + S.nsingle) {
+    const int i = S.singles[u - S.nwater];
+    const double3 r = ld3(S.x, S.n, i), p = ld3(S.dp, S.n, i);
+    const double3 xn = lt_position(S, C, step, i, r, p);
+    lt_finish(S, C, i, r, xn, p, ke0, ke1);
+  }
+  ke0 = wave_sum(ke0);
+  ke1 = wave_sum(ke1);
+  if (threadIdx.x == 0) part[blockIdx.x] = make_double2(ke0, ke1);
+}
+
+// Fixed-order kinetic energy sum, temperatures and their running means
+// (src/CudaLangevinThermostatIntegrator.cu:684-701), step counter + 1.
+// thermo: [0] ke (three-estimate mean) [1] ke (on-step) [2] T [3] T on-step
+//         [4] <T> [5] <T on-step>; counters: [0] step [1] averaging window.
+__global__ __launch_bounds__(256) void k_lt_finalize(const double2 *__restrict__ part, int npart,
+                                                     double ndof, double *__restrict__ thermo,
+                                                     unsigned long long *__restrict__ counters) {
+  __shared__ double s0[256], s1[256];
+  double a = 0.0, b = 0.0;
+  for (int k = threadIdx.x; k < npart; k += 256) {
+    a += part[k].x;
+    b += part[k].y;
+  }
+  s0[threadIdx.x] = a;
+  s1[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s0[threadIdx.x] += s0[threadIdx.x + w];
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double t0 = s0[0] / (0.5 * ndof * kBoltz), t1 = s1[0] / (0.5 * ndof * kBoltz);
+    const double cnt = (double)(counters[1] + 1);
+    thermo[0] = s0[0];
+    thermo[1] = s1[0];
+    thermo[2] = t0;
+    thermo[3] = t1;
+    thermo[4] += (t0 - thermo[4]) / cnt;
+    thermo[5] += (t1 - thermo[5]) / cnt;
+    counters[0] += 1;
+    counters[1] += 1;
+  }
+}
+
+// ---- initialisation (src/CudaLangevinThermostatIntegrator.cu:206-258) ------
+// Constrain x against itself, fold the correction into v, refresh xyzq.
+__global__ __launch_bounds__(kStepBlock) void k_lt_init_constrain(LtState S, LtCoef C) {
+  const int u = blockIdx.x * kStepBlock + threadIdx.x;
+  if (u < S.nwater) {
+    const int ia = S.waters[3 * u], ib = S.waters[3 * u + 1], ic = S.waters[3 * u + 2];
+    const double3 ra = ld3(S.x, S.n, ia), rb = ld3(S.x, S.n, ib), rc = ld3(S.x, S.n, ic);
+    double3 na = ra, nb = rb, nc = rc;
+    settle(C.G, ra, rb, rc, na, nb, nc);
+    const int idx[3] = {ia, ib, ic};
+    const double3 o[3] = {ra, rb, rc}, nn[3] = {na, nb, nc};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      double3 v = ld3(S.v, S.n, idx[k]);
+      v.x += (nn[k].x - o[k].x) / C.dt;
+      v.y += (nn[k].y - o[k].y) / C.dt;
+      v.z += (nn[k].z - o[k].z) / C.dt;
+      st3(S.v, S.n, idx[k], v);
+      st3(S.x, S.n, idx[k], nn[k]);
+      store_xyzq(S, idx[k], nn[k]);
+    }
+  } else if (u < S.nwater + S.nsingle) {
+    const int i = S.singles[u - S.nwater];
+    store_xyzq(S, i, ld3(S.x, S.n, i));
+  }
+}
+
+// Leapfrog back-step from the gradient at x, constrained against x; stores
+// the previous half-step displacement (InitializationKernel and the two
+// BackStepInitialization kernels).
+__global__ __launch_bounds__(kStepBlock) void k_lt_init_backstep(LtState S, LtCoef C) {
+  const int u = blockIdx.x * kStepBlock + threadIdx.x;
+  const double hg = 0.5 * C.gamma;
+  const double a1 = C.dt * (1.0 - hg) / sqrt(1.0 + hg);
+  auto back = [&](int i, double3 x) {
+    const double fact = 0.5 * C.dt2 * S.invmass[i];
+    const double3 v = ld3(S.v, S.n, i), g = ldgrad(S, i);
+    const double3 d = make_double3(a1 * v.x + fact * g.x, a1 * v.y + fact * g.y, a1 * v.z + fact * g.z);
+    return make_double3(x.x - d.x, x.y - d.y, x.z - d.z);
+  };
+  if (u < S.nwater) {
+    const int ia = S.waters[3 * u], ib = S.waters[3 * u + 1], ic = S.waters[3 * u + 2];
+    const double3 ra = ld3(S.x, S.n, ia), rb = ld3(S.x, S.n, ib), rc = ld3(S.x, S.n, ic);
+    double3 pa = back(ia, ra), pb = back(ib, rb), pc = back(ic, rc);
+    settle(C.G, ra, rb, rc, pa, pb, pc);
+    st3(S.dp, S.n, ia, ra - pa);
+    st3(S.dp, S.n, ib, rb - pb);
+    st3(S.dp, S.n, ic, rc - pc);
+  } else if (u < S.nwater + S.nsingle) {
+    const int i = S.singles[u - S.nwater];
+    const double3 r = ld3(S.x, S.n, i);
+    st3(S.dp, S.n, i, r - back(i, r));
+  }
+}
+
+// ---- centre-of-mass motion removal (src/CudaLangevinThermostatIntegrator.cu:832-878)
+__global__ __launch_bounds__(kStepBlock) void k_com_partial(const double *__restrict__ dp,
+                                                            const double *__restrict__ invmass,
+                                                            int n, double4 *__restrict__ part) {
+  const int i = blockIdx.x * kStepBlock + threadIdx.x;
+  double m = 0, a = 0, b = 0, c = 0;
+  if (i < n) {
+    m = 1.0 / invmass[i];
+    a = dp[i] * m;
+    b = dp[n + i] * m;
+    c = dp[2 * n + i] * m;
+  }
+  a = wave_sum(a);
+  b = wave_sum(b);
+  c = wave_sum(c);
+  m = wave_sum(m);
+  if (threadIdx.x == 0) part[blockIdx.x] = make_double4(a, b, c, m);
+}
+
+__global__ __launch_bounds__(256) void k_com_reduce(double4 *__restrict__ part, int npart) {
+  __shared__ double4 s[256];
+  double4 t = make_double4(0, 0, 0, 0);
+  for (int k = threadIdx.x; k < npart; k += 256) {
+    t.x += part[k].x;
+    t.y += part[k].y;
+    t.z += part[k].z;
+    t.w += part[k].w;
+  }
+  s[threadIdx.x] = t;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s[threadIdx.x].x += s[threadIdx.x + w].x;
+      s[threadIdx.x].y += s[threadIdx.x + w].y;
+      s[threadIdx.x].z += s[threadIdx.x + w].z;
+      s[threadIdx.x].w += s[threadIdx.x + w].w;
+    }
+    __syncthreads();
+  }
+  // the result goes one past the partials, read by k_com_apply
+  if (threadIdx.x == 0)
+    part[npart] = make_double4(s[0].x / s[0].w, s[0].y / s[0].w, s[0].z / s[0].w, 0.0);
+}
+
+__global__ void k_com_apply(double *__restrict__ dp, int n, const double4 *__restrict__ com) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 c = com[0];
+  dp[i] -= c.x;
+  dp[n + i] -= c.y;
+  dp[2 * n + i] -= c.z;
+}
+
+// ---- image centering (src/CharmmContext.cu:276-372, P1) ---------------------
+// One wave per group (whole molecule): unweighted centroid by a fixed-order
+// wave sum, then every atom of a group whose centroid lies outside
+// [-L/2, L/2] moves by one box edge on that axis.
+__global__ __launch_bounds__(256) void k_center(const int *__restrict__ gptr,
+                                                const int *__restrict__ gatom, int ngroup,
+                                                double3 box, double *__restrict__ x, int n,
+                                                const float *__restrict__ charge,
+                                                float4 *__restrict__ xyzq) {
+  const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (g >= ngroup) return;  // whole wave leaves together
+  const int b = gptr[g], e = gptr[g + 1];
+  double cx = 0, cy = 0, cz = 0;
+  for (int k = b + lane; k < e; k += 64) {
+    const int i = gatom[k];
+    cx += x[i];
+    cy += x[n + i];
+    cz += x[2 * n + i];
+  }
+  cx = wave_sum(cx);
+  cy = wave_sum(cy);
+  cz = wave_sum(cz);
+  const double inv = 1.0 / (double)(e - b);
+  cx *= inv;
+  cy *= inv;
+  cz *= inv;
+  auto shift = [](double c, double L) { return c < -0.5 * L ? L : (c > 0.5 * L ? -L : 0.0); };
+  const double sx = shift(cx, box.x), sy = shift(cy, box.y), sz = shift(cz, box.z);
+  if (sx == 0.0 && sy == 0.0 && sz == 0.0) return;
+  for (int k = b + lane; k < e; k += 64) {
+    const int i = gatom[k];
+    const double px = x[i] + sx, py = x[n + i] + sy, pz = x[2 * n + i] + sz;
+    x[i] = px;
+    x[n + i] = py;
+    x[2 * n + i] = pz;
+    xyzq[i] = make_float4((float)px, (float)py, (float)pz, charge[i]);
+  }
+}
+
+int nblk(long n, int b) { return (int)((n + b - 1) / b); }
+
+}  // namespace
+
+// --------------------------------------------------------------------------
+struct LtSystem {
+  NbSystem *nb = nullptr;
+  int n = 0;
+  LtCoef C{};
+  int nbfreq = 20, comfreq = 1000;
+  bool use_graph = true;
+  double ndof = 0;
+  int nwater = 0, nsingle = 0, nunit_blocks = 0;
+  DevBuf<double> x, dp, v, invmass;
+  DevBuf<float> charge;
+  DevBuf<float4> xyzq;
+  DevBuf<int64_t> grad;
+  int gstride = 0;
+  DevBuf<int> waters, singles, gptr, gatom;
+  int ngroup = 0;
+  DevBuf<double2> ke_part;
+  DevBuf<double4> com_part;
+  int com_blocks = 0;
+  DevBuf<double> thermo;                 // 6 values, see k_lt_finalize
+  DevBuf<unsigned long long> counters;   // [0] step [1] averaging window
+  unsigned long long step = 0;           // host mirror of counters[0]
+  long steps_since_rebuild = 0;
+  bool initialized = false;
+  hipStream_t st = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  unsigned long long graph_gen = ~0ull;
+
+  ~LtSystem() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (ev_in) (void)hipEventDestroy(ev_in);
+    if (ev_out) (void)hipEventDestroy(ev_out);
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  LtState state() {
+    LtState s;
+    s.x = x.get();
+    s.dp = dp.get();
+    s.v = v.get();
+    s.invmass = invmass.get();
+    s.charge = charge.get();
+    s.xyzq = xyzq.get();
+    s.grad = grad.get();
+    s.gstride = gstride;
+    s.n = n;
+    s.waters = waters.get();
+    s.nwater = nwater;
+    s.singles = singles.get();
+    s.nsingle = nsingle;
+    return s;
+  }
+
+  void force(hipStream_t s, bool reset) {
+    calc_force(*nb, reinterpret_cast<const float *>(xyzq.get()), reset ? APO_RESET_LIST : 0,
+               grad.get(), gstride, nullptr, nullptr, s);
+  }
+
+  void center(hipStream_t s) {
+    hipEvent_t t0 = nullptr;
+    nb->tic(9, s, t0);
+    const double3 box = make_double3(nb->box[0], nb->box[1], nb->box[2]);
+    k_center<<<nblk(ngroup, 4), 256, 0, s>>>(gptr.get(), gatom.get(), ngroup, box, x.get(), n,
+                                             charge.get(), xyzq.get());
+    APO_LAUNCH_CHECK();
+    nb->toc(9, s, t0);
+  }
+
+  void remove_com(hipStream_t s) {
+    hipEvent_t t0 = nullptr;
+    nb->tic(9, s, t0);
+    k_com_partial<<<com_blocks, kStepBlock, 0, s>>>(dp.get(), invmass.get(), n, com_part.get());
+    k_com_reduce<<<1, 256, 0, s>>>(com_part.get(), com_blocks);
+    k_com_apply<<<nblk(n, 256), 256, 0, s>>>(dp.get(), n, com_part.get() + com_blocks);
+    APO_LAUNCH_CHECK();
+    nb->toc(9, s, t0);
+  }
+
+  void update(hipStream_t s) {
+    hipEvent_t t0 = nullptr;
+    nb->tic(8, s, t0);
+    k_lt_step<<<nunit_blocks, kStepBlock, 0, s>>>(state(), C, counters.get(), ke_part.get());
+    k_lt_finalize<<<1, 256, 0, s>>>(ke_part.get(), nunit_blocks, ndof, thermo.get(),
+                                    counters.get());
+    APO_LAUNCH_CHECK();
+    nb->toc(8, s, t0);
+  }
+
+  // force + integrator update + kinetic energy; the body of a graph replay
+  void step_body(hipStream_t s) {
+    force(s, false);
+    update(s);
+  }
+
+  void initialize(hipStream_t s) {
+    k_lt_init_constrain<<<nunit_blocks, kStepBlock, 0, s>>>(state(), C);
+    APO_LAUNCH_CHECK();
+    force(s, true);
+    k_lt_init_backstep<<<nunit_blocks, kStepBlock, 0, s>>>(state(), C);
+    APO_LAUNCH_CHECK();
+    initialized = true;
+  }
+
+  bool graph_usable() const { return use_graph && !nb->prof_on && nb->list_valid; }
+
+  void replay(hipStream_t s) {
+    if (!gexec || graph_gen != nb->generation) {
+      if (gexec) {
+        APO_HIP(hipGraphExecDestroy(gexec));
+        gexec = nullptr;
+      }
+      hipGraph_t g = nullptr;
+      APO_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      try {
+        step_body(s);
+      } catch (...) {
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      APO_HIP(hipStreamEndCapture(s, &g));
+      hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      APO_HIP(e);
+      graph_gen = nb->generation;
+    }
+    APO_HIP(hipGraphLaunch(gexec, s));
+  }
+
+  // CudaIntegrator::propagate (src/CudaIntegrator.cu:140-230) with the
+  // thermostat's propagateOneStep (src/CudaLangevinThermostatIntegrator.cu:703-800).
+  void propagate(int nsteps, hipStream_t s) {
+    if (!initialized) initialize(s);
+    // propagate() opens with resetNeighborList(), whose image centering the
+    // first step then repeats
+    center(s);
+    steps_since_rebuild = 0;
+    for (int k = 0; k < nsteps; k++) {
+      const bool rebuild = steps_since_rebuild % nbfreq == 0;
+      const bool com = comfreq > 0 && step % (unsigned long long)comfreq == 0;
+      if (rebuild) {
+        center(s);
+        force(s, true);  // list rebuild with this step's force
+        if (com) remove_com(s);
+        update(s);
+      } else if (com || !graph_usable()) {
+        if (com) remove_com(s);
+        step_body(s);
+      } else {
+        replay(s);
+      }
+      steps_since_rebuild++;
+      step++;
+    }
+  }
+};
+
+// Molecules (connected components over bonds and waters) as centering groups.
+static void molecule_groups(int n, const std::vector<int> &pairs, std::vector<int> &gptr,
+                            std::vector<int> &gatom) {
+  std::vector<int> parent(n);
+  for (int i = 0; i < n; i++) parent[i] = i;
+  auto find = [&](int a) {
+    while (parent[a] != a) {
+      parent[a] = parent[parent[a]];
+      a = parent[a];
+    }
+    return a;
+  };
+  for (size_t k = 0; k + 1 < pairs.size(); k += 2) {
+    int a = find(pairs[k]), b = find(pairs[k + 1]);
+    if (a != b) parent[a > b ? a : b] = a > b ? b : a;
+  }
+  // groups ordered by their lowest atom, atoms ascending inside a group
+  std::vector<int> gid(n, -1), count;
+  int ng = 0;
+  for (int i = 0; i < n; i++) {
+    int r = find(i);
+    if (gid[r] < 0) {
+      gid[r] = ng++;
+      count.push_back(0);
+    }
+    count[gid[r]]++;
+  }
+  gptr.assign(ng + 1, 0);
+  for (int g = 0; g < ng; g++) gptr[g + 1] = gptr[g] + count[g];
+  std::vector<int> fill(gptr.begin(), gptr.end() - 1);
+  gatom.assign(n, 0);
+  for (int i = 0; i < n; i++) gatom[fill[gid[find(i)]]++] = i;
+}
+
+LtSystem *create_lt(NbSystem *nb, const apo_lt_params &p, const double *xyz, const double *vel) {
+  if (!nb) throw InvalidArgument("force system is null");
+  if (!xyz) throw InvalidArgument("coordinates are required");
+  if (!p.mass || !p.charge) throw InvalidArgument("mass and charge arrays are required");
+  if (!(p.timestep > 0)) throw InvalidArgument("timestep must be positive (ps)");
+  if (p.friction < 0 || p.temperature < 0) throw InvalidArgument("negative friction or temperature");
+  if (p.nwater < 0 || (p.nwater > 0 && !p.waters)) throw InvalidArgument("bad water list");
+  const int n = nb->natom;
+  auto *L = new LtSystem();
+  try {
+    L->nb = nb;
+    L->n = n;
+    L->nbfreq = p.nbfreq > 0 ? p.nbfreq : 20;
+    L->comfreq = p.comfreq == 0 ? 1000 : p.comfreq;
+    L->use_graph = p.use_graph != 0;
+    double geom[4] = {15.99940, 1.00800, 0.9572, 1.5139};
+    if (p.water_geom[0] > 0 || p.water_geom[1] > 0 || p.water_geom[2] > 0 || p.water_geom[3] > 0) {
+      for (int k = 0; k < 4; k++) {
+        if (!(p.water_geom[k] > 0)) throw InvalidArgument("water_geom entries must be positive");
+        geom[k] = p.water_geom[k];
+      }
+      if (!(geom[3] < 2.0 * geom[2])) throw InvalidArgument("water_geom: rHH must be < 2 rOH");
+    }
+    const double dt = p.timestep / kTimfac;
+    LtCoef &C = L->C;
+    C.gamma = p.timestep * p.friction;  // m_TimeStep * m_Timfac * friction
+    C.dt = dt;
+    C.dt2 = dt * dt;
+    C.kbt = kBoltz * p.temperature;
+    C.den = 1.0 / (1.0 + 0.5 * C.gamma);
+    C.alpha = (1.0 - 0.5 * C.gamma) * C.den;
+    C.fv = 0.5 * std::sqrt(1.0 + 0.5 * C.gamma) / dt;
+    C.idt = 1.0 / dt;
+    C.seed = p.seed;
+    const double M = geom[0] + 2.0 * geom[1];
+    const double rc = 0.5 * geom[3];
+    const double h = std::sqrt(geom[2] * geom[2] - rc * rc);
+    C.G.ra = 2.0 * geom[1] / M * h;
+    C.G.rb = h - C.G.ra;
+    C.G.rc = rc;
+    C.G.wO = geom[0] / M;
+    C.G.wH = geom[1] / M;
+
+    std::vector<double> invm(n);
+    for (int i = 0; i < n; i++) {
+      if (!(p.mass[i] > 0)) throw InvalidArgument("masses must be positive");
+      invm[i] = 1.0 / p.mass[i];
+    }
+    std::vector<char> in_water(n, 0);
+    std::vector<int> wl(p.waters, p.waters + 3 * (size_t)p.nwater);
+    for (int k = 0; k < 3 * p.nwater; k++) {
+      const int a = wl[k];
+      if (a < 0 || a >= n) throw InvalidArgument("water atom index out of range");
+      if (in_water[a]) throw InvalidArgument("atom listed in two waters");
+      in_water[a] = 1;
+    }
+    std::vector<int> singles;
+    for (int i = 0; i < n; i++)
+      if (!in_water[i]) singles.push_back(i);
+    L->nwater = p.nwater;
+    L->nsingle = (int)singles.size();
+    L->nunit_blocks = nblk(L->nwater + L->nsingle, kStepBlock);
+    L->com_blocks = nblk(n, kStepBlock);
+    // P1 with SETTLE: 3N - 3 - 3 per rigid water (CharmmContext::useHolonomicConstraints)
+    L->ndof = 3.0 * n - 3.0 - 3.0 * p.nwater;
+    if (L->ndof <= 0) throw InvalidArgument("no degrees of freedom left");
+
+    // centering groups: molecules over bonds and waters
+    std::vector<int> pairs = nb->h_bonds;
+    for (int k = 0; k < p.nwater; k++) {
+      pairs.push_back(wl[3 * k]);
+      pairs.push_back(wl[3 * k + 1]);
+      pairs.push_back(wl[3 * k]);
+      pairs.push_back(wl[3 * k + 2]);
+    }
+    std::vector<int> gptr, gatom;
+    molecule_groups(n, pairs, gptr, gatom);
+    L->ngroup = (int)gptr.size() - 1;
+
+    APO_HIP(hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking));
+    APO_HIP(hipEventCreateWithFlags(&L->ev_in, hipEventDisableTiming));
+    APO_HIP(hipEventCreateWithFlags(&L->ev_out, hipEventDisableTiming));
+    hipStream_t s = L->st;
+    // host AoS -> device SoA
+    std::vector<double> soa(3 * (size_t)n);
+    auto to_soa = [&](const double *a) {
+      for (int i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) soa[(size_t)d * n + i] = a ? a[3 * (size_t)i + d] : 0.0;
+    };
+    to_soa(xyz);
+    L->x.upload(soa, s);
+    to_soa(vel);
+    L->v.upload(soa, s);
+    L->dp.reserve(3 * (size_t)n);
+    launch_fill_f64(L->dp.get(), 3 * (size_t)n, s);
+    L->invmass.upload(invm, s);
+    L->charge.upload(p.charge, n, s);
+    std::vector<float4> q4(n);
+    for (int i = 0; i < n; i++)
+      q4[i] = make_float4((float)xyz[3 * (size_t)i], (float)xyz[3 * (size_t)i + 1],
+                          (float)xyz[3 * (size_t)i + 2], p.charge[i]);
+    L->xyzq.upload(q4, s);
+    L->gstride = ((n + 31) * 8 + 255) / 256 * 256 / 8;
+    L->grad.reserve(3 * (size_t)L->gstride);
+    launch_fill_i64(L->grad.get(), 3 * (size_t)L->gstride, s);
+    L->waters.upload(wl, s);
+    L->singles.upload(singles, s);
+    L->gptr.upload(gptr, s);
+    L->gatom.upload(gatom, s);
+    L->ke_part.reserve(L->nunit_blocks);
+    L->com_part.reserve(L->com_blocks + 1);
+    L->thermo.reserve(6);
+    launch_fill_f64(L->thermo.get(), 6, s);
+    std::vector<unsigned long long> c0 = {0ull, 0ull};
+    L->counters.upload(c0, s);
+    APO_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    delete L;
+    throw;
+  }
+  return L;
+}
+
+void destroy_lt(LtSystem *L) { delete L; }
+
+// Work runs on the integrator's own stream, ordered after and before the
+// caller's stream by events, so the non-rebuild steps can be captured.
+template <typename F> static void on_stream(LtSystem &L, hipStream_t caller, F &&f) {
+  APO_HIP(hipEventRecord(L.ev_in, caller));
+  APO_HIP(hipStreamWaitEvent(L.st, L.ev_in, 0));
+  f(L.st);
+  APO_HIP(hipEventRecord(L.ev_out, L.st));
+  APO_HIP(hipStreamWaitEvent(caller, L.ev_out, 0));
+}
+
+void lt_initialize(LtSystem &L, hipStream_t caller) {
+  on_stream(L, caller, [&](hipStream_t s) { L.initialize(s); });
+}
+
+void lt_propagate(LtSystem &L, int nsteps, hipStream_t caller) {
+  if (nsteps < 0) throw InvalidArgument("nsteps must be >= 0");
+  on_stream(L, caller, [&](hipStream_t s) { L.propagate(nsteps, s); });
+}
+
+void lt_get_state(LtSystem &L, double *xyz, double *vel, double *dprev, double *grad,
+                  double *thermo, int64_t *step) {
+  APO_HIP(hipStreamSynchronize(L.st));
+  const int n = L.n;
+  std::vector<double> soa(3 * (size_t)n);
+  auto from_soa = [&](const double *dev, double *out) {
+    APO_HIP(hipMemcpy(soa.data(), dev, soa.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++)
+      for (int d = 0; d < 3; d++) out[3 * (size_t)i + d] = soa[(size_t)d * n + i];
+  };
+  if (xyz) from_soa(L.x.get(), xyz);
+  if (vel) from_soa(L.v.get(), vel);
+  if (dprev) from_soa(L.dp.get(), dprev);
+  if (grad) {
+    std::vector<int64_t> g(3 * (size_t)L.gstride);
+    APO_HIP(hipMemcpy(g.data(), L.grad.get(), g.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++)
+      for (int d = 0; d < 3; d++)
+        grad[3 * (size_t)i + d] = (double)g[(size_t)d * L.gstride + i] * 9.094947017729282e-13;
+  }
+  if (thermo)
+    APO_HIP(hipMemcpy(thermo, L.thermo.get(), 6 * sizeof(double), hipMemcpyDeviceToHost));
+  if (step) *step = (int64_t)L.step;
+}
+
+const float *lt_xyzq(LtSystem &L) { return reinterpret_cast<const float *>(L.xyzq.get()); }
+
+}  // namespace apo

@@ -405,11 +405,13 @@ void nblist_build(NbSystem &S, const float4 *x, hipStream_t s) {
   S.host_nitem = hmeta[3];
   if ((size_t)S.host_ntile > S.tile_cap) {
     S.tile_cap = (size_t)(S.host_ntile * 1.25) + 64;
+    S.generation++;
     S.tile_j.reserve(S.tile_cap);
     S.tile_mask.reserve(S.tile_cap * kCell);
   }
   if ((size_t)S.host_nitem > S.item_cap) {
     S.item_cap = (size_t)(S.host_nitem * 1.25) + 64;
+    S.generation++;
     S.items.reserve(S.item_cap);
   }
   k_compact<<<S.ncell_cap, 64, 0, s>>>(meta, S.icell_start.get(), S.icell_actstart.get(),

@@ -112,6 +112,7 @@ __global__ __launch_bounds__(64) void k_pair(
   __shared__ T2 sh_tab[TAB_LDS ? kMaxTypesLds * kMaxTypesLds : 1];
   __shared__ int4 sh_i[kCell];
   __shared__ int sh_row[kCell];
+  __shared__ int sh_tj[kItemTiles];
   if (TAB_LDS)
     for (int k = threadIdx.x; k < ntypes * ntypes; k += 64) sh_tab[k] = vdwsq[k];
 
@@ -130,10 +131,21 @@ __global__ __launch_bounds__(64) void k_pair(
       sh_i[lane] = posq[ibase + lane];
       sh_row[lane] = TAB_LDS ? types[ibase + lane] * ntypes : types[ibase + lane];
     }
+    if (lane < kItemTiles && it.y + lane < it.z) sh_tj[lane] = tile_j[it.y + lane];
     __syncthreads();
     CT fix[32], fiy[32], fiz[32];
 #pragma unroll
     for (int t = 0; t < 32; t++) fix[t] = fiy[t] = fiz[t] = (CT)0;
+
+    // j-data of the next tile pair is loaded while the current one computes
+    auto jcell_of = [&](int tp) {
+      const int tile = tp + half;
+      return tile < it.z ? sh_tj[tile - it.y] : it.x;
+    };
+    int jc_n = jcell_of(it.y);
+    int4 pj_n = posq[jc_n * kCell + l];
+    int tj_n = types[jc_n * kCell + l];
+    uint32_t m_n = it.y + half < it.z ? tile_mask[(size_t)(it.y + half) * kCell + l] : 0xffffffffu;
 
     for (int tp = it.y; tp < it.z; tp += 2) {
       // keep the 32 i-atom LDS reads inside the tile loop (not hoisted into
@@ -141,11 +153,22 @@ __global__ __launch_bounds__(64) void k_pair(
       asm volatile("" ::: "memory");
       const int tile = tp + half;
       const bool tv = tile < it.z;
-      const int jc = tv ? tile_j[tile] : it.x;
+      const int jc = jc_n;
       const int jl = jc * kCell + l;
-      const int4 pj = posq[jl];
-      const int tj = types[jl];
-      const uint32_t m = tv ? tile_mask[(size_t)tile * kCell + l] : 0xffffffffu;
+      const int4 pj = pj_n;
+      const int tj = tj_n;
+      const uint32_t m = m_n;
+      if (tp + 2 < it.z) {
+        jc_n = jcell_of(tp + 2);
+        pj_n = posq[jc_n * kCell + l];
+        tj_n = types[jc_n * kCell + l];
+        m_n = tile + 2 < it.z ? tile_mask[(size_t)(tile + 2) * kCell + l] : 0xffffffffu;
+      }
+      // i-atoms with no pair inside the list cutoff in either tile are
+      // skipped with a scalar branch
+      uint32_t act = ~m;
+      for (int off = 32; off > 0; off >>= 1) act |= __shfl_xor(act, off);
+      act = __builtin_amdgcn_readfirstlane(act);
       const CT qj = charge_of<CT>(pj.w) * qfac;
       CT fjx = 0, fjy = 0, fjz = 0;
       // energies accumulate per pair in FP64 (large cancelling sums); the
@@ -154,6 +177,7 @@ __global__ __launch_bounds__(64) void k_pair(
       CT v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
 #pragma unroll
       for (int t = 0; t < 32; t++) {
+        if (!(act & (1u << t))) continue;
         const int4 pi = sh_i[t];
         const CT dx = (CT)(int)(pi.x - pj.x) * P.sx;
         const CT dy = (CT)(int)(pi.y - pj.y) * P.sy;

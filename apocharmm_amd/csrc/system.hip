@@ -318,6 +318,7 @@ NbSystem *create_system(const apo_nb_params &p, const apo_nb_topology &t) {
       if (t.bonds[b] < 0 || t.bonds[b] >= n) throw InvalidArgument("bond index out of range");
     std::vector<int32_t> eptr, eidx, in14, ex14;
     build_exclusions(n, t.nbond, t.bonds, eptr, eidx, in14, ex14);
+    S->h_bonds.assign(t.bonds, t.bonds + 2 * (size_t)t.nbond);
     S->excl_ptr.upload(eptr.data(), eptr.size(), 0);
     S->excl_idx.upload(eidx.data(), eidx.size(), 0);
     S->nin14 = (int)in14.size() / 2;
@@ -431,6 +432,7 @@ void set_box(NbSystem &S, const double *b) {
   for (int d = 0; d < 3; d++) S.box[d] = b[d];
   S.make_pair_consts();
   S.list_valid = false;
+  S.generation++;
 }
 
 void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int stride,

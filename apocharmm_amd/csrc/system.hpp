@@ -3,6 +3,7 @@
 // CudaPMEReciprocalForce, CudaBondedForce, CudaEnergyVirial).
 #pragma once
 
+#include "apocharmm_amd.h"
 #include "common.hpp"
 
 namespace apo {
@@ -52,6 +53,10 @@ struct NbSystem {
   int ncx = 1, ncy = 1, ncol = 1;
   int ncell_cap = 0;  // ceil(natom/32) + ncol + 1
   bool list_valid = false;
+  // bumped whenever a buffer or a launch constant of the per-step path
+  // changes (list buffers regrown, new box); captured step graphs compare it
+  unsigned long long generation = 0;
+  std::vector<int> h_bonds;  // 2 * nbond, host copy for molecule grouping
   DevBuf<int> meta;   // [0] ncell [1] ncand [2] ntile [3] nitem [4] maxcol
   DevBuf<int> col_count, col_start, col_ncell, col_cellstart;
   DevBuf<int> atom_col, atom_slot;
@@ -98,8 +103,8 @@ struct NbSystem {
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   std::vector<TimerRec> prof_recs;
-  double prof_ms[8] = {0};
-  long long prof_cnt[8] = {0};
+  double prof_ms[APO_PROFILE_SLOTS] = {0};
+  long long prof_cnt[APO_PROFILE_SLOTS] = {0};
   hipEvent_t prof_event();
   void tic(int slot, hipStream_t s, hipEvent_t &a);
   void toc(int slot, hipStream_t s, hipEvent_t a);

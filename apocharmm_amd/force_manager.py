@@ -136,9 +136,15 @@ class ForceManager:
                                                 self._stream_ptr()))
 
     def calcForce(self, xyzq, reset=False, calcEnergy=False, calcVirial=False, terms=0):
-        """xyzq: device float32 tensor (natom, 4).  Overwrites self.force with
-        the int64 2^-40 gradient; energies/virial when requested."""
-        assert xyzq.dtype == _torch().float32 and xyzq.is_cuda and xyzq.is_contiguous()
+        """xyzq: device float32 tensor (natom, 4), or the integer address of
+        such a device array (e.g. LangevinThermostatIntegrator.getXYZQ()).
+        Overwrites self.force with the int64 2^-40 gradient; energies/virial
+        when requested."""
+        if isinstance(xyzq, int):
+            ptr = xyzq
+        else:
+            assert xyzq.dtype == _torch().float32 and xyzq.is_cuda and xyzq.is_contiguous()
+            ptr = xyzq.data_ptr()
         flags = terms
         if reset:
             flags |= _lib.APO_RESET_LIST
@@ -147,7 +153,7 @@ class ForceManager:
         if calcVirial:
             flags |= _lib.APO_CALC_VIRIAL
         _lib.check(_lib.lib().apo_nb_calc_force(
-            self._handle, ctypes.c_void_p(xyzq.data_ptr()), flags,
+            self._handle, ctypes.c_void_p(ptr), flags,
             ctypes.c_void_p(self.force.data_ptr()), self.stride,
             ctypes.c_void_p(self.energy.data_ptr()), ctypes.c_void_p(self.virial.data_ptr()),
             self._stream_ptr()))

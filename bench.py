@@ -1,19 +1,22 @@
-"""Benchmark: ns/day of the CHARMM force path on the 98,304-atom TIP3P PME box.
+"""Benchmark: ns/day of NVT Langevin dynamics on the 98,304-atom TIP3P PME box.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2]
-                    [--precision mixed|fp64] [--no-cpu-baseline]
+                    [--precision mixed|fp64] [--step md|force] [--no-cpu-baseline]
 
-One "step" is one force evaluation of the whole system through the C ABI
-(bonded + direct-space pair kernel + 1-4/exclusion corrections + PME spread,
-FFT, convolution, gather), with the neighbor list rebuilt every 20 steps (the
-reference's default nonbonded-list update frequency, src/CudaIntegrator.cu:29),
-inputs resident in HBM.  ns/day = steps * dt[ps] / 1000 / (wall_s / 86400)
-with dt = 2 fs (unittest-benchmarks.cpp:286-287).  The integrator (Langevin +
-SETTLE) is not part of this step.
+One "step" (default --step md) is one leapfrog Langevin step at dt = 2 fs,
+friction 1/ps, 300 K with SETTLE rigid waters: the full force path (bonded +
+direct-space pair kernel + 1-4/exclusion corrections + PME spread, FFT,
+convolution, gather) and the integrator update, through the C ABI
+(apo_lt_propagate), with the neighbor list rebuilt and images centred every
+20 steps (the reference's default, src/CudaIntegrator.cu:29) and the other
+steps replayed as a hipGraph.  All state is resident in HBM.
+ns/day = steps * dt[ps] / 1000 / (wall_s / 86400) (unittest-benchmarks.cpp:286-287).
+--step force times force evaluations alone.
 
 Prints one JSON line with the metric, a roofline entry for the dominant kernel
-(the pair kernel, FP32 VALU work counted as 61 flop per pair inside r_off) and
-the CPU baseline (the FP64 oracle on this host's cores, projected to ns/day).
+(the pair kernel, FP32 VALU work counted as 61 flop per pair inside r_off,
+timed with HIP events in a second, profiled pass of the same steps) and the
+CPU baseline (the FP64 oracle force + integrator step on this host's cores).
 """
 import argparse
 import json
@@ -25,6 +28,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 DT_PS = 0.002
+FRICTION = 1.0        # 1/ps (unittest-langevinThermostat.cpp:49)
+TEMP = 300.0
+SEED = 2003
 REBUILD_EVERY = 20
 FLOP_PER_PAIR = 61
 PEAK_FP32_TFLOPS = 157.3
@@ -44,26 +50,34 @@ def count_pairs(spec, rc):
     return int(pairs - n_excl)
 
 
-def cpu_baseline(spec, threads):
-    import numpy as np  # noqa: F401
+def cpu_baseline(spec, threads, md):
+    import numpy as np
     from apocharmm_amd import systems
     from oracle import oracle as O
     s = O.prepare(spec)
     xyzq = systems.xyzq_of(spec)
+    n = len(xyzq)
+    ol = O.Langevin(1.0 / np.asarray(spec["masses"], np.float64),
+                    np.asarray(spec["waters"], np.int32), DT_PS, TEMP, FRICTION, SEED)
+    x = np.ascontiguousarray(xyzq[:, :3], np.float64)
+    v = np.zeros((n, 3))
+    dprev = np.zeros((n, 3))
     O.evaluate(s, xyzq, nthreads=threads)  # warm
     t0 = time.perf_counter()
     reps = 0
     while True:
-        O.evaluate(s, xyzq, nthreads=threads)
+        r = O.evaluate(s, xyzq, nthreads=threads)
+        if md:
+            ol.step(r["grad"], x, dprev, v, reps)
         reps += 1
         dt = time.perf_counter() - t0
         if dt > 10.0 or reps >= 8:
             break
     t_eval = dt / reps
+    what = "force evaluations + Langevin/SETTLE steps" if md else "force evaluations"
     return {"value": DT_PS * 86.4 / t_eval, "unit": "ns/day", "cores": threads, "kind": "port",
-            "sample": "%d full FP64 force evaluations of the same system (oracle, cell list + "
-                      "PME, %d OpenMP threads), %.3f s each; ns/day = dt*86.4/t_eval"
-                      % (reps, threads, t_eval)}
+            "sample": "%d FP64 %s of the same system (oracle, cell list + PME, %d OpenMP "
+                      "threads), %.3f s each; ns/day = dt*86.4/t_step" % (reps, what, threads, t_eval)}
 
 
 def main():
@@ -73,6 +87,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c2")
     ap.add_argument("--precision", default="mixed")
+    ap.add_argument("--step", choices=("md", "force"), default="md")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -80,39 +95,54 @@ def main():
               "single MI355X; --gpus %d is not supported" % a.gpus, file=sys.stderr)
         sys.exit(2)
 
-    import torch
-    from apocharmm_amd import systems
-    from apocharmm_amd.force_manager import ForceManager
     import ctypes
-    from apocharmm_amd import _lib
+
+    import numpy as np
+    import torch
+    from apocharmm_amd import _lib, systems
+    from apocharmm_amd.force_manager import ForceManager
+    from apocharmm_amd.integrator import LangevinThermostatIntegrator, maxwell_velocities
 
     spec = {"c1": systems.config1, "c2": systems.config2, "c3": systems.config3}[a.config]()
     n = len(spec["charges"])
     torch.cuda.set_device(0)
     fm = ForceManager(spec, a.precision).initialize()
-    xyzq = torch.tensor(systems.xyzq_of(spec), device="cuda")
-    stream = torch.cuda.current_stream()
+    md = a.step == "md"
+    if md:
+        vseed = {"c1": 1002, "c2": 2002, "c3": 3005}[a.config]
+        vel = maxwell_velocities(spec["masses"], TEMP, vseed)
+        lt = LangevinThermostatIntegrator(DT_PS, friction=FRICTION, temperature=TEMP, seed=SEED)
+        lt.setNonbondedListUpdateFrequency(REBUILD_EVERY)
+        lt.setCharmmContext(fm, np.asarray(spec["xyz"], np.float64), vel)
+        lt.initialize()
 
-    def step(i):
-        fm.calcForce(xyzq, reset=(i % REBUILD_EVERY == 0))
+        def run(k):
+            lt.propagate(k)
+    else:
+        xyzq = torch.tensor(systems.xyzq_of(spec), device="cuda")
 
-    for i in range(a.warmup):
-        step(i)
-    torch.cuda.synchronize()
-    _lib.check(_lib.lib().apo_nb_profile(fm._handle, 1))
+        def run(k):
+            for i in range(k):
+                fm.calcForce(xyzq, reset=(i % REBUILD_EVERY == 0))
+
+    run(a.warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for i in range(a.steps):
-        step(i)
+    run(a.steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    ms = (ctypes.c_double * 8)()
-    cnt = (ctypes.c_int64 * 8)()
+
+    # second pass of the same steps with HIP-event kernel timers (eager launches)
+    _lib.check(_lib.lib().apo_nb_profile(fm._handle, 1))
+    run(a.steps)
+    torch.cuda.synchronize()
+    ms = (ctypes.c_double * 10)()
+    cnt = (ctypes.c_int64 * 10)()
     _lib.check(_lib.lib().apo_nb_profile_read(fm._handle, ms, cnt))
     _lib.check(_lib.lib().apo_nb_profile(fm._handle, 0))
     kern = {name: (ms[k] / max(cnt[k], 1), int(cnt[k])) for k, name in enumerate(
         ("pair", "pme_spread", "pme_fft_conv", "pme_gather", "list_rebuild", "calc_force",
-         "bonded_14", "sort_finalize"))}
+         "bonded_14", "sort_finalize", "integrator", "center_com"))}
     ms_step = wall / a.steps * 1e3
     value = a.steps * DT_PS / 1000.0 / (wall / 86400.0)
 
@@ -120,8 +150,15 @@ def main():
     pair_ms = kern["pair"][0]
     achieved = pairs * FLOP_PER_PAIR / (pair_ms * 1e-3) / 1e12
     stats = fm.listStats()
+    thermo = None
+    if md:
+        th = lt._state(())["thermo"]
+        thermo = {"T_K": round(float(th[2]), 2), "T_avg_K": round(float(th[4]), 2),
+                  "steps_total": int(lt.getCurrentStep())}
+    what = "NVT Langevin + SETTLE step" if md else "force evaluation only (no integrator)"
     out = {
-        "metric": "ns/day @98k-atom PME water (force path, 2 fs step)",
+        "metric": "ns/day @98k-atom PME water (%s, 2 fs)" % ("NVT Langevin MD" if md else
+                                                              "force path"),
         "value": round(value, 3),
         "unit": "ns/day",
         "n_gpus": 1,
@@ -132,15 +169,19 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32+int64" if a.precision == "mixed" else "fp64+int64",
-        "data": "synthetic (seeded TIP3P lattice, random orientations)",
+        "data": "synthetic (seeded TIP3P lattice, random orientations, Maxwell velocities)",
         "config": {"workload": "%s_tip3p_%d_atoms_pme%d" % (a.config, n, int(spec["fft_grid"][0])),
-                   "natom": n, "box_A": float(spec["box"][0]), "pme_grid": [int(v) for v in spec["fft_grid"]],
+                   "natom": n, "box_A": float(spec["box"][0]),
+                   "pme_grid": [int(v) for v in spec["fft_grid"]],
                    "cutoffs_A": [float(spec["ctonnb"]), float(spec["ctofnb"]), float(spec["cutnb"])],
                    "kappa": float(spec["kappa"]), "dt_fs": 2.0, "list_rebuild_every": REBUILD_EVERY,
-                   "step": "force evaluation only (no integrator)", "precision": a.precision,
+                   "step": what, "friction_per_ps": FRICTION if md else None,
+                   "temperature_K": TEMP if md else None, "precision": a.precision,
                    "parallelism": "single GPU"},
         "kernels_ms_avg": {k: round(v[0], 5) for k, v in kern.items()},
+        "kernel_launches": {k: v[1] for k, v in kern.items()},
         "list": stats,
+        "thermo": thermo,
         "roofline": {"bound": "mfma", "engine": "fp32 VALU (no dense contraction on this path)",
                      "kernel": "k_pair (direct-space pair kernel)", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
@@ -149,9 +190,11 @@ def main():
     }
     if not a.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 16)
-        out["cpu_baseline"] = cpu_baseline(spec, threads)
+        out["cpu_baseline"] = cpu_baseline(spec, threads, md)
     else:
         out["cpu_baseline"] = None
+    if md:
+        lt.close()
     fm.close()
     print(json.dumps(out), flush=True)
 

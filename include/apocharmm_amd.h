@@ -218,11 +218,14 @@ APO_API apo_status apo_nb_list_stats(apo_nb_system *sys, int64_t *out6,
  * returns, per slot, the summed milliseconds and the number of launches:
  * slot 0 pair kernel, 1 PME spread, 2 PME FFT + convolution (5 passes),
  * 3 PME gather, 4 neighbor-list rebuild, 5 whole calc_force, 6 bonded +
- * 1-4/exclusions, 7 sort/finalize of the direct-space coordinates/forces. */
-enum { APO_PROFILE_SLOTS = 8 };
+ * 1-4/exclusions, 7 sort/finalize of the direct-space coordinates/forces,
+ * 8 integrator update (apo_lt_propagate: leapfrog + SETTLE + kinetic energy),
+ * 9 image centering + centre-of-mass removal.  While recording, integrator
+ * steps run eagerly (no graph replay). */
+enum { APO_PROFILE_SLOTS = 10 };
 APO_API apo_status apo_nb_profile(apo_nb_system *sys, int32_t enable);
-APO_API apo_status apo_nb_profile_read(apo_nb_system *sys, double *ms8,
-                                       int64_t *count8);
+APO_API apo_status apo_nb_profile_read(apo_nb_system *sys, double *ms,
+                                       int64_t *count);
 
 /* Host-side topology helper (no device work): 1-2/1-3/1-4 exclusion CSR and
  * the in14/ex14 pair lists derived from a bond list, exactly as
@@ -236,6 +239,73 @@ APO_API apo_status apo_topology_exclusions(int32_t natom, int32_t nbond,
                                            int32_t *excl_ptr,
                                            int32_t *excl_idx, int32_t *in14,
                                            int32_t *ex14);
+
+/* ---- Langevin thermostat integrator with SETTLE --------------------------
+ * CudaLangevinThermostatIntegrator (include/CudaLangevinThermostatIntegrator.h,
+ * src/CudaLangevinThermostatIntegrator.cu) with CudaHolonomicConstraint's
+ * SETTLE (src/CudaHolonomicConstraint.cu:164-303), driving an apo_nb_system
+ * as CharmmContext drives ForceManager.  FP64 coordinates, velocities and
+ * half-step displacements stay on the device; each step calls the force path
+ * on the float xyzq it keeps. */
+typedef struct apo_lt_integrator apo_lt_integrator;
+
+typedef struct apo_lt_params {
+  double timestep;    /* ps: CudaIntegrator(timeStep), src/CudaIntegrator.cu:38-40 */
+  double temperature; /* K: setReferenceTemperature */
+  double friction;    /* 1/ps: setThermostatFriction; 0 gives NVE leapfrog */
+  uint64_t seed;      /* setThermostatRngSeed; the noise of (step, atom) is a
+                         pure function of (seed, step, atom) */
+  int32_t nbfreq;     /* setNonbondedListUpdateFrequency; <= 0 means 20 */
+  int32_t comfreq;    /* setRemoveCenterOfMassFrequency; 0 means 1000, < 0 never */
+  int32_t nwater;     /* rigid waters constrained by SETTLE */
+  int32_t use_graph;  /* nonzero: replay steps without a list rebuild as a
+                         hipGraph */
+  const int32_t *waters; /* 3*nwater atom indices (O, H1, H2)
+                            (CharmmPSF::getWaterMolecules) */
+  double water_geom[4];  /* mO, mH, rOH, rHH of the rigid water; all zero
+                            selects 15.9994, 1.008, 0.9572, 1.5139
+                            (src/CudaHolonomicConstraint.cu:18-33) */
+  const double *mass;    /* natom, amu */
+  const float *charge;   /* natom, e; the q of the xyzq handed to calc_force */
+} apo_lt_params;
+
+/* Binds an integrator to `sys` (setCharmmContext) with host coordinates
+ * xyz[3*natom] (x0 y0 z0 x1 ...; A) and velocities vel[3*natom] (A per AKMA
+ * time unit, CharmmContext::assignVelocitiesAtTemperature; NULL = zero).  The
+ * force system must outlive the integrator. */
+APO_API apo_status apo_lt_create(apo_nb_system *sys, const apo_lt_params *params,
+                                 const double *xyz, const double *vel,
+                                 apo_lt_integrator **out);
+APO_API apo_status apo_lt_destroy(apo_lt_integrator *lt);
+
+/* CudaLangevinThermostatIntegrator::initialize
+ * (src/CudaLangevinThermostatIntegrator.cu:206-258): constrain, force,
+ * leapfrog back-step.  apo_lt_propagate calls it when it has not run. */
+APO_API apo_status apo_lt_initialize(apo_lt_integrator *lt, void *stream);
+
+/* CudaIntegrator::propagate(nsteps) (src/CudaIntegrator.cu:140-230): image
+ * centering and list rebuild every nbfreq steps, centre-of-mass removal every
+ * comfreq steps, then per step force, Langevin leapfrog update, SETTLE,
+ * velocities, kinetic energy and the float xyzq.  Enqueued after the work
+ * already on `stream`; the host waits only at list rebuilds. */
+APO_API apo_status apo_lt_propagate(apo_lt_integrator *lt, int32_t nsteps,
+                                    void *stream);
+
+/* Copies the state to the host (synchronises): any output may be NULL.
+ *  xyz, vel, dprev, grad: double[3*natom] AoS; vel is the on-step velocity of
+ *    the last step, dprev the last half-step displacement (getCoordsDelta),
+ *    grad the gradient the last step used;
+ *  thermo: double[6] = kinetic energy (mean of the old half-step, on-step and
+ *    new half-step estimates), on-step kinetic energy, the two temperatures,
+ *    and their running means (getKineticEnergy, getAverageTemperature);
+ *  step: steps propagated since creation. */
+APO_API apo_status apo_lt_get_state(apo_lt_integrator *lt, double *xyz, double *vel,
+                                    double *dprev, double *grad, double *thermo6,
+                                    int64_t *step);
+
+/* Device float xyzq[4*natom] the integrator keeps for the force path
+ * (CharmmContext::getXYZQ); valid until apo_lt_destroy. */
+APO_API apo_status apo_lt_xyzq(apo_lt_integrator *lt, const float **d_xyzq);
 
 #ifdef __cplusplus
 }

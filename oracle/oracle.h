@@ -15,6 +15,8 @@
 #ifndef APO_ORACLE_H
 #define APO_ORACLE_H
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -74,6 +76,29 @@ int orc_fft3d(const int *K, double *data, int sign, int nthreads);
 /* Reciprocal building blocks exposed for closed-form checks. */
 void orc_bspline(int order, double w, double *theta, double *dtheta);
 void orc_bsp_mod(int order, int nfft, double *bsp_mod);
+
+
+/* ---- Langevin thermostat leapfrog + SETTLE (md.c) ---------------------------
+ * AoS [n][3] doubles; dt in AKMA time units, gamma = dt[ps] * friction[1/ps],
+ * kbt in kcal/mol; geom = (mO, mH, rOH, rHH) of the rigid water; waters are
+ * (O, H1, H2) index triplets. */
+void orc_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
+void orc_langevin_normals(uint64_t seed, uint64_t step, int atom, double *n3);
+void orc_settle_geometry(const double *geom, double *abc);
+void orc_settle(int nw, const int *w, const double *geom, const double *xref, double *x);
+int orc_shake_water(int nw, const int *w, const double *geom, const double *xref, double *x,
+                    double tol, int maxit);
+void orc_lt_init_constrain(int n, int nw, const int *w, const double *geom, double dt, double *x,
+                           double *v, double *scratch);
+void orc_lt_init_backstep(int n, const double *invmass, int nw, const int *w, const double *geom,
+                          double dt, double gamma, const double *grad, const double *x,
+                          const double *v, double *dprev, double *scratch);
+void orc_lt_remove_com(int n, const double *invmass, double *dprev);
+void orc_lt_step(int n, const double *invmass, int nw, const int *w, const double *geom,
+                 double dt, double gamma, double kbt, uint64_t seed, uint64_t step,
+                 const double *grad, double *x, double *dprev, double *v, double *ke,
+                 double *scratch);
+void orc_center(int ng, const int *gptr, const int *gatom, const double *box, double *x);
 
 #ifdef __cplusplus
 }

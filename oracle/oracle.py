@@ -41,6 +41,19 @@ def lib():
         L.orc_fft3d.argtypes = [_ip, _dp, i, i]
         L.orc_bspline.argtypes = [i, d, _dp, _dp]
         L.orc_bsp_mod.argtypes = [i, i, _dp]
+        u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+        u64 = ctypes.c_uint64
+        L.orc_philox4x32_10.argtypes = [u32p, u32p, u32p]
+        L.orc_langevin_normals.argtypes = [u64, u64, i, _dp]
+        L.orc_settle_geometry.argtypes = [_dp, _dp]
+        L.orc_settle.argtypes = [i, _ip, _dp, _dp, _dp]
+        L.orc_shake_water.argtypes = [i, _ip, _dp, _dp, _dp, d, i]
+        L.orc_lt_init_constrain.argtypes = [i, i, _ip, _dp, d, _dp, _dp, _dp]
+        L.orc_lt_init_backstep.argtypes = [i, _dp, i, _ip, _dp, d, d, _dp, _dp, _dp, _dp, _dp]
+        L.orc_lt_remove_com.argtypes = [i, _dp, _dp]
+        L.orc_lt_step.argtypes = [i, _dp, i, _ip, _dp, d, d, d, u64, u64, _dp, _dp, _dp, _dp,
+                                  _dp, _dp]
+        L.orc_center.argtypes = [i, _ip, _ip, _dp, _dp]
         _LIB = L
     return _LIB
 
@@ -258,3 +271,100 @@ def prepare(spec, table_dtype=np.float32):
         order=int(spec.get("order", 4)), e14fac=float(spec.get("e14fac", 1.0)),
     )
     return d
+
+
+# --------------------------------------------------------------------------- #
+# Langevin thermostat leapfrog + SETTLE (oracle/md.c)
+# --------------------------------------------------------------------------- #
+TIMFAC = 0.0488882129        # AKMA time unit in ps (src/CudaIntegrator.cu:22)
+KBOLTZ = 1.987191e-03        # kcal/mol/K (include/Constants.h:17)
+TIP3P_GEOM = (15.99940, 1.00800, 0.9572, 1.5139)   # mO, mH, rOH, rHH
+
+
+def philox4x32_10(ctr, key):
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32_10(np.ascontiguousarray(ctr, np.uint32),
+                            np.ascontiguousarray(key, np.uint32), out)
+    return out
+
+
+def langevin_normals(seed, step, atom):
+    out = np.zeros(3)
+    lib().orc_langevin_normals(seed, step, atom, out)
+    return out
+
+
+def settle_geometry(geom=TIP3P_GEOM):
+    out = np.zeros(3)
+    lib().orc_settle_geometry(np.asarray(geom, np.float64), out)
+    return out
+
+
+def settle(waters, xref, x, geom=TIP3P_GEOM):
+    """SETTLE x (n,3) against xref in place; waters (nw,3) int32."""
+    w = np.ascontiguousarray(waters, np.int32).reshape(-1)
+    lib().orc_settle(len(w) // 3, w, np.asarray(geom, np.float64),
+                     np.ascontiguousarray(xref, np.float64).reshape(-1), x.reshape(-1))
+    return x
+
+
+def shake_water(waters, xref, x, geom=TIP3P_GEOM, tol=1e-15, maxit=10000):
+    w = np.ascontiguousarray(waters, np.int32).reshape(-1)
+    it = lib().orc_shake_water(len(w) // 3, w, np.asarray(geom, np.float64),
+                               np.ascontiguousarray(xref, np.float64).reshape(-1),
+                               x.reshape(-1), tol, maxit)
+    if it < 0:
+        raise RuntimeError("SHAKE did not converge")
+    return x
+
+
+class Langevin:
+    """CPU restatement of CudaLangevinThermostatIntegrator on numpy state.
+
+    x, v, dprev are (n, 3) float64 arrays updated in place; grad is dU/dx
+    (n, 3).  dt_ps, friction_ps (1/ps), temperature (K) as in the reference
+    setters; internally dt is in AKMA units and gamma = dt_ps * friction.
+    """
+
+    def __init__(self, invmass, waters, dt_ps, temperature, friction_ps, seed, geom=TIP3P_GEOM):
+        self.invmass = np.ascontiguousarray(invmass, np.float64)
+        self.n = len(self.invmass)
+        self.w = np.ascontiguousarray(waters, np.int32).reshape(-1)
+        self.nw = len(self.w) // 3
+        self.geom = np.asarray(geom, np.float64)
+        self.dt = dt_ps / TIMFAC
+        self.gamma = dt_ps * friction_ps
+        self.kbt = KBOLTZ * temperature
+        self.seed = int(seed)
+        self.scratch = np.zeros(3 * self.n)
+
+    def init_constrain(self, x, v):
+        lib().orc_lt_init_constrain(self.n, self.nw, self.w, self.geom, self.dt, x.reshape(-1),
+                                    v.reshape(-1), self.scratch)
+
+    def init_backstep(self, grad, x, v):
+        dprev = np.zeros((self.n, 3))
+        lib().orc_lt_init_backstep(self.n, self.invmass, self.nw, self.w, self.geom, self.dt,
+                                   self.gamma, np.ascontiguousarray(grad, np.float64).reshape(-1),
+                                   np.ascontiguousarray(x).reshape(-1),
+                                   np.ascontiguousarray(v).reshape(-1), dprev.reshape(-1),
+                                   self.scratch)
+        return dprev
+
+    def remove_com(self, dprev):
+        lib().orc_lt_remove_com(self.n, self.invmass, dprev.reshape(-1))
+
+    def step(self, grad, x, dprev, v, step):
+        ke = np.zeros(2)
+        lib().orc_lt_step(self.n, self.invmass, self.nw, self.w, self.geom, self.dt, self.gamma,
+                          self.kbt, self.seed, int(step),
+                          np.ascontiguousarray(grad, np.float64).reshape(-1), x.reshape(-1),
+                          dprev.reshape(-1), v.reshape(-1), ke, self.scratch)
+        return ke
+
+
+def center(groups_ptr, groups_atom, box, x):
+    lib().orc_center(len(groups_ptr) - 1, np.ascontiguousarray(groups_ptr, np.int32),
+                     np.ascontiguousarray(groups_atom, np.int32), np.asarray(box, np.float64),
+                     x.reshape(-1))
+    return x

@@ -1,0 +1,165 @@
+"""Pins of the integrator oracle (oracle/md.c): known-answer vectors for the
+RNG, SETTLE against an independent iterative SHAKE, and closed forms of the
+leapfrog Langevin scheme (src/CudaLangevinThermostatIntegrator.cu:138-167,
+576-682)."""
+import numpy as np
+import pytest
+
+from apocharmm_amd import systems
+
+O = pytest.importorskip("oracle.oracle")
+
+
+def test_philox_known_answers():
+    # Random123 kat_vectors for philox4x32 with 10 rounds
+    kat = [
+        ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+        ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+        ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+         (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+    ]
+    for ctr, key, want in kat:
+        assert [int(v) for v in O.philox4x32_10(ctr, key)] == list(want)
+
+
+def test_langevin_normals_distribution():
+    z = np.array([O.langevin_normals(2003, s, a) for s in range(40) for a in range(1000)])
+    assert z.shape == (40000, 3)
+    assert abs(z.mean()) < 0.02
+    assert abs(z.var() - 1.0) < 0.02
+    # kurtosis of a normal is 3
+    assert abs((z ** 4).mean() / z.var() ** 2 - 3.0) < 0.1
+    # counter independence: neighbouring atoms/steps/seeds are uncorrelated
+    a = np.array([O.langevin_normals(7, 5, i)[0] for i in range(5000)])
+    b = np.array([O.langevin_normals(7, 6, i)[0] for i in range(5000)])
+    c = np.array([O.langevin_normals(8, 5, i)[0] for i in range(5000)])
+    assert abs(np.corrcoef(a, b)[0, 1]) < 0.05 and abs(np.corrcoef(a, c)[0, 1]) < 0.05
+
+
+@pytest.fixture(scope="module")
+def water1000():
+    sp = systems.config1()
+    return np.asarray(sp["waters"], np.int32), np.asarray(sp["xyz"], np.float64), sp
+
+
+def test_settle_equals_iterative_shake(water1000):
+    w, x0, _ = water1000
+    rng = np.random.default_rng(5)
+    x1 = x0 + rng.normal(0, 0.02, x0.shape)
+    xs = O.settle(w, x0, x1.copy())
+    xk = O.shake_water(w, x0, x1.copy(), tol=1e-14)
+    assert np.abs(xs - xk).max() < 1e-11
+
+
+def test_settle_geometry_and_momentum(water1000):
+    w, x0, _ = water1000
+    rng = np.random.default_rng(6)
+    x1 = x0 + rng.normal(0, 0.03, x0.shape)
+    xs = O.settle(w, x0, x1.copy())
+    d = lambda x, i, j: np.linalg.norm(x[w[:, i]] - x[w[:, j]], axis=1)
+    geom = O.TIP3P_GEOM
+    assert np.abs(d(xs, 0, 1) - geom[2]).max() < 1e-10
+    assert np.abs(d(xs, 0, 2) - geom[2]).max() < 1e-10
+    assert np.abs(d(xs, 1, 2) - geom[3]).max() < 1e-10
+    m = np.array([geom[0], geom[1], geom[1]])
+    com = lambda x: (x[w] * m[None, :, None]).sum(1) / m.sum()
+    assert np.abs(com(xs) - com(x1)).max() < 1e-12
+    # the canonical triangle: ra + rb is the O to H-H midpoint height
+    ra, rb, rc = O.settle_geometry()
+    assert abs((ra + rb) ** 2 + rc ** 2 - geom[2] ** 2) < 1e-12
+    assert abs(ra * geom[0] - 2 * geom[1] * rb) < 1e-12
+
+
+def test_settle_of_rigid_input_is_identity(water1000):
+    w, x0, _ = water1000
+    xr = O.settle(w, x0, x0.copy())          # make exactly rigid at the TIP3P geometry
+    xs = O.settle(w, xr, xr.copy())
+    assert np.abs(xs - xr).max() < 1e-12
+
+
+def _free_particles(n, mass=12.0):
+    invm = np.full(n, 1.0 / mass)
+    return invm, np.zeros((0, 3), np.int32)
+
+
+def test_leapfrog_harmonic_closed_form():
+    """gamma = 0: x_{n+1} - 2 x_n + x_{n-1} = -dt^2 k x_n / m, so
+    x_n = A cos(n theta) with cos(theta) = 1 - dt^2 omega^2 / 2."""
+    invm, w = _free_particles(1, mass=4.0)
+    lt = O.Langevin(invm, w, dt_ps=0.002, temperature=0.0, friction_ps=0.0, seed=1)
+    k = 30.0
+    x = np.array([[1.0, 0.0, 0.0]])
+    v = np.zeros((1, 3))
+    dprev = lt.init_backstep(k * x, x, v)
+    xs = [x[0, 0]]
+    for s in range(400):
+        lt.step(k * x, x, dprev, v, s)
+        xs.append(x[0, 0])
+    theta = np.arccos(1.0 - lt.dt ** 2 * k * invm[0] / 2.0)
+    # x_1 = x_0 (1 - dt^2 w^2/2) fixes the phase: x_n = cos(n theta)
+    ref = np.cos(theta * np.arange(len(xs)))
+    assert np.abs(np.array(xs) - ref).max() < 1e-9
+
+
+def test_init_then_step_returns_initial_velocity():
+    """With gamma = 0 the back-step makes the first on-step velocity equal the
+    initial one whatever the force; with gamma > 0 and T = 0 it is alpha v."""
+    rng = np.random.default_rng(3)
+    n = 50
+    invm = 1.0 / rng.uniform(1, 20, n)
+    w = np.zeros((0, 3), np.int32)
+    for gamma_ps, fac in ((0.0, 1.0), (150.0, None)):
+        lt = O.Langevin(invm, w, dt_ps=0.002, temperature=0.0, friction_ps=gamma_ps, seed=1)
+        x = rng.normal(size=(n, 3))
+        v0 = rng.normal(size=(n, 3)) * 0.01
+        g = rng.normal(size=(n, 3))
+        v = v0.copy()
+        dprev = lt.init_backstep(g, x, v)
+        lt.step(g, x, dprev, v, 0)
+        alpha = (1 - lt.gamma / 2) / (1 + lt.gamma / 2)
+        want = v0 * (1.0 if fac else alpha)
+        assert np.abs(v - want).max() < 1e-12 * np.abs(v0).max() + 1e-15
+
+
+def test_langevin_equipartition_free_particles():
+    """Free particles: the half-step displacement has variance dt^2 kT/m and
+    the on-step velocity kT/m exactly, so both kinetic energies average
+    (3/2) N kT."""
+    n = 4000
+    rng = np.random.default_rng(9)
+    invm = 1.0 / rng.uniform(1, 16, n)
+    w = np.zeros((0, 3), np.int32)
+    T = 300.0
+    lt = O.Langevin(invm, w, dt_ps=0.002, temperature=T, friction_ps=100.0, seed=2003)
+    x = np.zeros((n, 3))
+    v = np.zeros((n, 3))
+    g = np.zeros((n, 3))
+    dprev = lt.init_backstep(g, x, v)
+    ke = []
+    for s in range(300):
+        ke.append(lt.step(g, x, dprev, v, s))
+    ke = np.array(ke[100:])
+    want = 1.5 * n * O.KBOLTZ * T
+    assert abs(ke[:, 0].mean() / want - 1) < 0.01
+    assert abs(ke[:, 1].mean() / want - 1) < 0.01
+
+
+def test_remove_com_and_centering():
+    rng = np.random.default_rng(4)
+    n = 30
+    invm = 1.0 / rng.uniform(1, 16, n)
+    d = rng.normal(size=(n, 3))
+    O.Langevin(invm, np.zeros((0, 3), np.int32), 0.001, 300, 0, 0).remove_com(d)
+    assert np.abs(((1 / invm)[:, None] * d).sum(0)).max() < 1e-12
+    box = np.array([10.0, 12.0, 14.0])
+    gptr = np.array([0, 3, 5, 6], np.int32)
+    gatom = np.arange(6, dtype=np.int32)
+    x = np.array([[5.5, 0, 0], [5.8, 0, 0], [5.9, 0, 0],      # centroid 5.73 > 5 -> -10
+                  [-7.5, 6.5, 0], [-7.0, 6.1, 0],             # x -7.25 -> +10, y 6.3 > 6 -> -12
+                  [1.0, -6.9, 7.1]])                          # y -6.9 -> +12, z 7.1 -> -14
+    xc = O.center(gptr, gatom, box, x.copy())
+    shift = xc - x
+    assert np.allclose(shift[:3], [-10, 0, 0])
+    assert np.allclose(shift[3:5], [10, -12, 0])
+    assert np.allclose(shift[5], [0, 12, -14])
+    assert np.array_equal(O.center(gptr, gatom, box, xc.copy()), xc)
