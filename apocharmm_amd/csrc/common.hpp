@@ -20,6 +20,15 @@ constexpr double kPi = 3.14159265358979323846;
 // Cell (tile edge) size and tiles per pair-kernel work item.
 constexpr int kCell = 32;
 constexpr int kItemTiles = 16;
+// Pair-kernel grid: blocks (one wave each) per CU.  The kernel loops over
+// work items, so this only sets how many waves each SIMD can interleave.
+#ifndef APO_PAIR_BLOCKS_PER_CU
+#define APO_PAIR_BLOCKS_PER_CU 32
+#endif
+constexpr int kPairBlocksPerCu = APO_PAIR_BLOCKS_PER_CU;
+#ifndef APO_PAIR_XCD_MAP
+#define APO_PAIR_XCD_MAP 0
+#endif
 // Largest xy column population the LDS column sort handles.
 constexpr int kMaxColumn = 2048;
 // Pair-kernel per-block partials: elec, vdw, virial xx xy xz yy yz zz.

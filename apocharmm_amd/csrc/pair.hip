@@ -123,7 +123,15 @@ __global__ __launch_bounds__(64) void k_pair(
 #pragma unroll
   for (int k = 0; k < kPairPart; k++) acc[k] = 0.0;
 
-  for (int item = blockIdx.x; item < nitem; item += gridDim.x) {
+#if APO_PAIR_XCD_MAP
+  // consecutive items (neighbouring i-cells, shared j-cells) on one XCD:
+  // blocks are dispatched round-robin over the 8 XCDs
+  const int nb8 = gridDim.x >> 3;
+  const int first = (gridDim.x & 7) == 0 ? (blockIdx.x & 7) * nb8 + (blockIdx.x >> 3) : blockIdx.x;
+#else
+  const int first = blockIdx.x;
+#endif
+  for (int item = first; item < nitem; item += gridDim.x) {
     const int4 it = items[item];
     const int ibase = it.x * kCell;
     __syncthreads();  // previous item's readers are done with sh_i

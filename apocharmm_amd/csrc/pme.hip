@@ -302,13 +302,16 @@ __global__ __launch_bounds__(256) void k_spread_brick(const int4 *__restrict__ p
   }
 }
 
-// Gather in cell order: 16 lanes per atom; the gradient is added to the
-// sorted gradient buffer (one writer per atom, no atomics).
+// Gather in cell order: 16 lanes per atom; the gradient goes straight into
+// the global-order output with int64 atomics (three per atom), so the gather
+// can run while the pair kernel and the bonded terms write the same atoms.
 template <typename CT>
 __global__ __launch_bounds__(256) void k_gather_sorted(const int4 *__restrict__ posq,
-                                                       const int *__restrict__ meta, GridGeom g,
+                                                       const int *__restrict__ meta,
+                                                       const int *__restrict__ loc2glo, GridGeom g,
                                                        double qunscale, const CT *__restrict__ phi,
-                                                       long long *__restrict__ fsort, int ss) {
+                                                       unsigned long long *__restrict__ force,
+                                                       int stride) {
   const int gt = blockIdx.x * 256 + threadIdx.x;
   const int loc = gt >> 4, row = gt & 15;
   const bool valid = loc < meta[0] * kCell;
@@ -347,11 +350,14 @@ __global__ __launch_bounds__(256) void k_gather_sorted(const int4 *__restrict__ 
     f3 += __shfl_xor(f3, off);
   }
   if (valid && q != 0.0 && row == 0) {
-    const double qc = q * kCcelec;
-    fsort[loc] += (long long)__builtin_rint(qc * g.inv0 * g.K0 * (double)f1 * kForceScale);
-    fsort[ss + loc] += (long long)__builtin_rint(qc * g.inv1 * g.K1 * (double)f2 * kForceScale);
-    fsort[2 * (size_t)ss + loc] +=
-        (long long)__builtin_rint(qc * g.inv2 * g.K2 * (double)f3 * kForceScale);
+    const int gi = loc2glo[loc];
+    if (gi >= 0) {
+      const double qc = q * kCcelec;
+      auto fx = [](double v) { return (unsigned long long)(long long)__builtin_rint(v); };
+      atomicAdd(&force[gi], fx(qc * g.inv0 * g.K0 * (double)f1 * kForceScale));
+      atomicAdd(&force[stride + gi], fx(qc * g.inv1 * g.K1 * (double)f2 * kForceScale));
+      atomicAdd(&force[2 * (size_t)stride + gi], fx(qc * g.inv2 * g.K2 * (double)f3 * kForceScale));
+    }
   }
 }
 
@@ -627,8 +633,8 @@ static void recip_t(NbSystem &S, const float4 *x, int64_t *force, int stride, bo
   S.toc(2, s, t0);
   S.tic(3, s, t0);
   k_gather_sorted<CT><<<(S.ssort * 16 + 255) / 256, 256, 0, s>>>(
-      S.posq.get(), S.meta.get(), g, qunscale, phi, reinterpret_cast<long long *>(S.fsort.get()),
-      S.ssort);
+      S.posq.get(), S.meta.get(), S.loc2glo.get(), g, qunscale, phi,
+      reinterpret_cast<unsigned long long *>(force), stride);
   APO_LAUNCH_CHECK();
   S.toc(3, s, t0);
 }

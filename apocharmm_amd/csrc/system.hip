@@ -277,6 +277,9 @@ NbSystem *create_system(const apo_nb_params &p, const apo_nb_topology &t) {
     hipDeviceProp_t prop;
     APO_HIP(hipGetDeviceProperties(&prop, S->device));
     S->ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    for (hipStream_t &st : S->side) APO_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    for (hipEvent_t *e : {&S->ev_fork, &S->ev_sorted, &S->ev_pme, &S->ev_bond})
+      APO_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
 
     const int n = t.natom;
     for (int i = 0; i < n; i++)
@@ -403,7 +406,7 @@ NbSystem *create_system(const apo_nb_params &p, const apo_nb_topology &t) {
       b->reserve(S->ncell_cap + 1);
 
     // partials layout (fixed block counts -> deterministic reductions)
-    S->pair_blocks = 4 * S->ncu;
+    S->pair_blocks = kPairBlocksPerCu * S->ncu;
     PartLayout &L = S->part;
     int off = 0;
     L.pair_off = off; L.pair_n = S->pair_blocks; off += L.pair_n * kPairPart;
@@ -447,10 +450,24 @@ void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int s
   int terms = flags & APO_TERM_ALL;
   if (!terms) terms = APO_TERM_ALL;
   const float4 *x4 = reinterpret_cast<const float4 *>(xyzq);
-  hipEvent_t t_all = nullptr, t0 = nullptr;
+  hipEvent_t t_all = nullptr, t0 = nullptr, t1 = nullptr;
   S.tic(5, s, t_all);
   launch_fill_i64(force, 3 * (size_t)stride, s);
   const bool dir = terms & APO_TERM_DIRECT, rec = terms & APO_TERM_RECIP;
+  const bool bnd = terms & (APO_TERM_PAIRS14 | APO_TERM_BONDED);
+  // Three branches that meet before the sorted pair gradient is folded into
+  // `force`:  bonded + 1-4 (side[1], int64 atomics into force) | pair kernel
+  // (caller's stream, atomics into the sorted buffer) | PME spread, FFT,
+  // gather (side[0], atomics into force).
+  if (bnd) {
+    APO_HIP(hipEventRecord(S.ev_fork, s));
+    APO_HIP(hipStreamWaitEvent(S.side[1], S.ev_fork, 0));
+    S.tic(6, S.side[1], t1);
+    if (terms & APO_TERM_PAIRS14) pairs14_launch(S, x4, force, stride, E, V, S.side[1]);
+    if (terms & APO_TERM_BONDED) bonded_launch(S, x4, force, stride, E, V, S.side[1]);
+    S.toc(6, S.side[1], t1);
+    APO_HIP(hipEventRecord(S.ev_bond, S.side[1]));
+  }
   if (dir || rec) {
     // both the pair kernel and PME work on the cell-sorted coordinates
     if ((flags & APO_RESET_LIST) || !S.list_valid) {
@@ -462,22 +479,25 @@ void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int s
     nblist_sort_coords(S, x4, s);
     if (rec) nblist_cellbb(S, s);
     S.toc(7, s, t0);
+    if (rec) {
+      APO_HIP(hipEventRecord(S.ev_sorted, s));
+      APO_HIP(hipStreamWaitEvent(S.side[0], S.ev_sorted, 0));
+      recip_launch(S, x4, force, stride, E, V, S.side[0]);
+      APO_HIP(hipEventRecord(S.ev_pme, S.side[0]));
+    }
   }
   if (dir) {
     S.tic(0, s, t0);
     direct_launch(S, E, V, s);
     S.toc(0, s, t0);
   }
-  if (rec) recip_launch(S, x4, force, stride, E, V, s);
+  if (rec) APO_HIP(hipStreamWaitEvent(s, S.ev_pme, 0));
+  if (bnd) APO_HIP(hipStreamWaitEvent(s, S.ev_bond, 0));
   if (dir || rec) {
     S.tic(7, s, t0);
     direct_finalize(S, force, stride, s);
     S.toc(7, s, t0);
   }
-  S.tic(6, s, t0);
-  if (terms & APO_TERM_PAIRS14) pairs14_launch(S, x4, force, stride, E, V, s);
-  if (terms & APO_TERM_BONDED) bonded_launch(S, x4, force, stride, E, V, s);
-  S.toc(6, s, t0);
   if ((terms & APO_TERM_SELF) && E) self_launch(S, x4, s);
   if (E || V) reduce_energy_launch(S, terms, E, V, energy, virial, s);
   S.toc(5, s, t_all);
@@ -524,6 +544,10 @@ void NbSystem::profile_flush() {
 
 NbSystem::~NbSystem() {
   for (auto e : ev_pool) (void)hipEventDestroy(e);
+  for (hipEvent_t e : {ev_fork, ev_sorted, ev_pme, ev_bond})
+    if (e) (void)hipEventDestroy(e);
+  for (hipStream_t st : side)
+    if (st) (void)hipStreamDestroy(st);
 }
 
 }  // namespace apo

@@ -97,6 +97,11 @@ struct NbSystem {
   PartLayout part;
   DevBuf<double> parts;
 
+  // calc_force runs PME and the bonded/1-4 terms on two side streams that
+  // fork from and join back into the caller's stream
+  hipStream_t side[2] = {nullptr, nullptr};
+  hipEvent_t ev_fork = nullptr, ev_sorted = nullptr, ev_pme = nullptr, ev_bond = nullptr;
+
   // HIP-event kernel timers (apo_nb_profile)
   struct TimerRec { int slot; hipEvent_t a, b; };
   bool prof_on = false;
