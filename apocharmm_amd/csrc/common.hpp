@@ -17,18 +17,30 @@ constexpr double kInvForceScale = 1.0 / 1099511627776.0;
 constexpr double kGridScale = 1099511627776.0;  // 2^40
 constexpr double kCcelec = 332.0716;            // src/CudaDirectForceKernels.cu:59
 constexpr double kPi = 3.14159265358979323846;
-// Cell (tile edge) size and tiles per pair-kernel work item.
+// Cell (tile edge) size; pair-kernel i-group size (slots of one cell) and
+// 64-lane j steps per work item.
 constexpr int kCell = 32;
-constexpr int kItemTiles = 16;
+constexpr int kGroup = 16;
+constexpr int kGroupsPerCell = kCell / kGroup;
+constexpr int kItemSteps = 16;
+// a j entry is (candidate tile index within its i-cell << 5 | slot in the
+// j-cell) in the low 16 bits and the 16-bit skip mask in the high bits
+constexpr int kMaxCellTiles = 2048;
 // Pair-kernel grid: blocks (one wave each) per CU.  The kernel loops over
 // work items, so this only sets how many waves each SIMD can interleave.
 #ifndef APO_PAIR_BLOCKS_PER_CU
 #define APO_PAIR_BLOCKS_PER_CU 32
 #endif
 constexpr int kPairBlocksPerCu = APO_PAIR_BLOCKS_PER_CU;
-#ifndef APO_PAIR_XCD_MAP
-#define APO_PAIR_XCD_MAP 0
+// FFT passes: real-row pairs per block (x) and columns per block (y, z).
+#ifndef APO_FFT_ROWS_PER_BLOCK
+#define APO_FFT_ROWS_PER_BLOCK 16
 #endif
+#ifndef APO_FFT_COLS_PER_BLOCK
+#define APO_FFT_COLS_PER_BLOCK 16
+#endif
+constexpr int kFftRowsPerBlock = APO_FFT_ROWS_PER_BLOCK;
+constexpr int kFftColsPerBlock = APO_FFT_COLS_PER_BLOCK;
 // Largest xy column population the LDS column sort handles.
 constexpr int kMaxColumn = 2048;
 // Pair-kernel per-block partials: elec, vdw, virial xx xy xz yy yz zz.

@@ -1,6 +1,7 @@
 // Neighbor list: xy-column binning, deterministic z sort per column in LDS,
 // 32-atom cells with bounding boxes, cell-pair tiles inside the list cutoff,
-// per-tile 32x32 exclusion/prune masks and pair-kernel work items.
+// per-tile 32x32 exclusion/prune masks, then per 16-slot i-group the packed
+// j slots that have a live pair, and the pair-kernel work items.
 //
 // Reference behaviour (re-designed, not translated):
 //   CudaNeighborListSort   src/CudaNeighborListSort.cu:31-716 (columns of edge
@@ -271,53 +272,82 @@ __global__ __launch_bounds__(64) void k_tile_mask(
   if (tv && l == 0) cand_active[tile] = hb != 0u;
 }
 
-// 8. active tiles per i-cell and work items of <= kItemTiles tiles.
-__global__ __launch_bounds__(64) void k_active_count(const int *__restrict__ meta,
-                                                     const int *__restrict__ icell_start,
-                                                     const int *__restrict__ cand_active,
-                                                     int *__restrict__ icell_act,
-                                                     int *__restrict__ icell_nitem) {
+// 8. per i-group (kGroup consecutive slots of a cell) the number of j slots
+// with at least one live pair in the 32x32 masks, rounded up to whole 64-lane
+// steps, and the number of work items of <= kItemSteps steps.  Cells beyond
+// the live count write zeros so one scan covers every group.
+__global__ __launch_bounds__(64) void k_grp_count(const int *__restrict__ meta,
+                                                  const int *__restrict__ icell_start,
+                                                  const uint32_t *__restrict__ cand_mask,
+                                                  int *__restrict__ grp_cnt,
+                                                  int *__restrict__ grp_nitem,
+                                                  int *__restrict__ meta_w) {
+  constexpr uint32_t kFull = (1u << kGroup) - 1u;
   const int icell = blockIdx.x;
-  if (icell >= meta[0]) return;
-  const int s = icell_start[icell], e = icell_start[icell + 1];
-  int c = 0;
-  for (int k = s + threadIdx.x; k < e; k += 64) c += cand_active[k];
-  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
-  if (threadIdx.x == 0) {
-    icell_act[icell] = c;
-    icell_nitem[icell] = (c + kItemTiles - 1) / kItemTiles;
+  const int lane = threadIdx.x, l = lane & 31, half = lane >> 5;
+  int cnt[kGroupsPerCell] = {};
+  if (icell < meta[0]) {
+    const int s = icell_start[icell], e = icell_start[icell + 1];
+    // the entry format indexes at most kMaxCellTiles tiles per cell
+    if (e - s > kMaxCellTiles && lane == 0) atomicOr(&meta_w[7], 1);
+    for (int k0 = s; k0 < e; k0 += 2) {
+      const int k = k0 + half;
+      const uint32_t m = k < e ? cand_mask[(size_t)k * kCell + l] : 0xffffffffu;
+#pragma unroll
+      for (int h = 0; h < kGroupsPerCell; h++)
+        cnt[h] += __popcll(__ballot(((m >> (kGroup * h)) & kFull) != kFull));
+    }
   }
+  if (lane == 0)
+#pragma unroll
+    for (int h = 0; h < kGroupsPerCell; h++) {
+      const int c = (cnt[h] + 63) / 64 * 64;
+      grp_cnt[icell * kGroupsPerCell + h] = c;
+      grp_nitem[icell * kGroupsPerCell + h] = (c + 64 * kItemSteps - 1) / (64 * kItemSteps);
+    }
 }
 
-// 9. compaction into the final tile arrays and items (deterministic order).
-__global__ __launch_bounds__(64) void k_compact(
+// 9. entries (deterministic order: candidate tiles in j order, slots
+// ascending), null padding (full skip mask on the cell's own first slot) and
+// work items.
+__global__ __launch_bounds__(64) void k_grp_emit(
     const int *__restrict__ meta, const int *__restrict__ icell_start,
-    const int *__restrict__ icell_actstart, const int *__restrict__ icell_itemstart,
-    const int *__restrict__ cand_j, const uint32_t *__restrict__ cand_mask,
-    const int *__restrict__ cand_active, int *__restrict__ tile_j, uint32_t *__restrict__ tile_mask,
-    int4 *__restrict__ items) {
+    const uint32_t *__restrict__ cand_mask, const int *__restrict__ grp_start,
+    const int *__restrict__ grp_itemstart, uint32_t *__restrict__ ent, int4 *__restrict__ items) {
+  constexpr uint32_t kFull = (1u << kGroup) - 1u;
   const int icell = blockIdx.x;
   if (icell >= meta[0]) return;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x, l = lane & 31, half = lane >> 5;
   const int s = icell_start[icell], e = icell_start[icell + 1];
-  const int a0 = icell_actstart[icell], a1 = icell_actstart[icell + 1];
-  int count = 0;
-  for (int k0 = s; k0 < e; k0 += 64) {
-    int k = k0 + lane;
-    bool act = k < e && cand_active[k];
-    unsigned long long m = __ballot(act);
-    if (act) {
-      int pos = a0 + count + __popcll(m & ((1ull << lane) - 1ull));
-      tile_j[pos] = cand_j[k];
-      for (int w = 0; w < kCell; w++)
-        tile_mask[(size_t)pos * kCell + w] = cand_mask[(size_t)k * kCell + w];
+  if (e - s > kMaxCellTiles) return;  // flagged in k_grp_count; the host raises
+  int run[kGroupsPerCell] = {};
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int k0 = s; k0 < e; k0 += 2) {
+    const int k = k0 + half;
+    const uint32_t m = k < e ? cand_mask[(size_t)k * kCell + l] : 0xffffffffu;
+    const uint32_t jloc = (uint32_t)((k - s) << 5 | l);
+#pragma unroll
+    for (int h = 0; h < kGroupsPerCell; h++) {
+      const uint32_t mg = (m >> (kGroup * h)) & kFull;
+      const bool live = mg != kFull;
+      const unsigned long long b = __ballot(live);
+      if (live) {
+        const int pos = grp_start[icell * kGroupsPerCell + h] + run[h] + __popcll(b & below);
+        ent[pos] = jloc | (mg << 16);
+      }
+      run[h] += __popcll(b);
     }
-    count += __popcll(m);
   }
-  const int i0 = icell_itemstart[icell], i1 = icell_itemstart[icell + 1];
-  for (int it = i0 + lane; it < i1; it += 64) {
-    int ts = a0 + (it - i0) * kItemTiles;
-    items[it] = make_int4(icell, ts, min(ts + kItemTiles, a1), 0);
+#pragma unroll
+  for (int h = 0; h < kGroupsPerCell; h++) {
+    const int g = icell * kGroupsPerCell + h;
+    const int st = grp_start[g], en = grp_start[g + 1];
+    for (int p = st + run[h] + lane; p < en; p += 64) ent[p] = kFull << 16;
+    const int i0 = grp_itemstart[g], i1 = grp_itemstart[g + 1];
+    for (int it = i0 + lane; it < i1; it += 64) {
+      const int a = st + (it - i0) * 64 * kItemSteps;
+      items[it] = make_int4(g, a, min(a + 64 * kItemSteps, en), s);
+    }
   }
 }
 
@@ -391,33 +421,36 @@ void nblist_build(NbSystem &S, const float4 *x, hipStream_t s) {
         S.cand_active.get());
     APO_LAUNCH_CHECK();
   }
-  k_active_count<<<S.ncell_cap, 64, 0, s>>>(meta, S.icell_start.get(), S.cand_active.get(),
-                                            S.icell_act.get(), S.icell_nitem.get());
+  const int ngrp = S.ncell_cap * kGroupsPerCell;
+  k_grp_count<<<S.ncell_cap, 64, 0, s>>>(meta, S.icell_start.get(), S.cand_mask.get(),
+                                         S.grp_cnt.get(), S.grp_nitem.get(), meta);
   APO_LAUNCH_CHECK();
-  launch_scan(S.icell_act.get(), S.icell_actstart.get(), meta + 0, S.ncell_cap, s);
-  launch_scan(S.icell_nitem.get(), S.icell_itemstart.get(), meta + 0, S.ncell_cap, s);
-  k_copy_total<<<1, 64, 0, s>>>(S.icell_actstart.get(), meta + 0, 0, meta + 2);
-  k_copy_total<<<1, 64, 0, s>>>(S.icell_itemstart.get(), meta + 0, 0, meta + 3);
+  launch_scan(S.grp_cnt.get(), S.grp_start.get(), nullptr, ngrp, s);
+  launch_scan(S.grp_nitem.get(), S.grp_itemstart.get(), nullptr, ngrp, s);
+  k_copy_total<<<1, 64, 0, s>>>(S.grp_start.get(), nullptr, ngrp, meta + 2);
+  k_copy_total<<<1, 64, 0, s>>>(S.grp_itemstart.get(), nullptr, ngrp, meta + 3);
   APO_LAUNCH_CHECK();
   APO_HIP(hipMemcpyAsync(hmeta, meta, sizeof(hmeta), hipMemcpyDeviceToHost, s));
   APO_HIP(hipStreamSynchronize(s));
-  S.host_ntile = hmeta[2];
+  if (hmeta[7]) {
+    S.list_valid = false;
+    throw RuntimeError("neighbor list: a cell has more than 2048 candidate tiles");
+  }
+  S.host_nent = hmeta[2];
   S.host_nitem = hmeta[3];
-  if ((size_t)S.host_ntile > S.tile_cap) {
-    S.tile_cap = (size_t)(S.host_ntile * 1.25) + 64;
+  if ((size_t)S.host_nent > S.ent_cap) {
+    S.ent_cap = (size_t)(S.host_nent * 1.25) + 64 * kItemSteps;
     S.generation++;
-    S.tile_j.reserve(S.tile_cap);
-    S.tile_mask.reserve(S.tile_cap * kCell);
+    S.ent.reserve(S.ent_cap);
   }
   if ((size_t)S.host_nitem > S.item_cap) {
     S.item_cap = (size_t)(S.host_nitem * 1.25) + 64;
     S.generation++;
     S.items.reserve(S.item_cap);
   }
-  k_compact<<<S.ncell_cap, 64, 0, s>>>(meta, S.icell_start.get(), S.icell_actstart.get(),
-                                       S.icell_itemstart.get(), S.cand_j.get(), S.cand_mask.get(),
-                                       S.cand_active.get(), S.tile_j.get(), S.tile_mask.get(),
-                                       S.items.get());
+  k_grp_emit<<<S.ncell_cap, 64, 0, s>>>(meta, S.icell_start.get(), S.cand_mask.get(),
+                                        S.grp_start.get(), S.grp_itemstart.get(), S.ent.get(),
+                                        S.items.get());
   APO_LAUNCH_CHECK();
   S.list_valid = true;
 }
@@ -439,12 +472,12 @@ void nblist_sort_coords(NbSystem &S, const float4 *x, hipStream_t s) {
   APO_LAUNCH_CHECK();
 }
 
-// Pairs inside the list cutoff covered by the tiles (unmasked bits).
-__global__ void k_count_pairs(const uint32_t *__restrict__ tile_mask, int ntile,
+// Pairs inside the list cutoff covered by the list (clear skip-mask bits).
+__global__ void k_count_pairs(const uint32_t *__restrict__ ent, int nent,
                               unsigned long long *__restrict__ out) {
   size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   unsigned c = 0;
-  if (k < (size_t)ntile * kCell) c = __popc(~tile_mask[k]);
+  if (k < (size_t)nent) c = __popc(~(ent[k] >> 16) & ((1u << kGroup) - 1u));
   for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
   if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, (unsigned long long)c);
 }
@@ -453,9 +486,9 @@ void nblist_stats(NbSystem &S, int64_t *out6, hipStream_t s) {
   DevBuf<unsigned long long> cnt;
   cnt.reserve(1);
   APO_HIP(hipMemsetAsync(cnt.get(), 0, sizeof(unsigned long long), s));
-  if (S.list_valid && S.host_ntile > 0)
-    k_count_pairs<<<nblk((long)S.host_ntile * kCell, 256), 256, 0, s>>>(S.tile_mask.get(),
-                                                                       S.host_ntile, cnt.get());
+  if (S.list_valid && S.host_nent > 0)
+    k_count_pairs<<<nblk((long)S.host_nent, 256), 256, 0, s>>>(S.ent.get(), S.host_nent,
+                                                               cnt.get());
   APO_LAUNCH_CHECK();
   unsigned long long pairs = 0;
   int hmeta[8];

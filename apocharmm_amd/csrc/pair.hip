@@ -4,16 +4,16 @@
 // and pair_vdw_force VFSW :263-286).
 //
 // gfx950 design (not the reference's warp-rotation scheme):
-//  * one wave64 = one work item (an i-cell and <= 16 of its tiles); lanes
-//    0-31 hold the j-atoms of tile A, lanes 32-63 those of tile B;
-//  * the 32 i-atoms are visited in a fully unrolled loop, each one
-//    wave-uniform (scalar loads), so the i-data costs no LDS or VALU traffic;
-//  * i-forces live in 96 registers per lane and are reduced once per item by
-//    a transposed butterfly; j-forces are flushed per tile;
+//  * one wave64 = one work item: an i-group of 16 consecutive cell-sorted
+//    slots and its packed j list (slots with a live pair at list build, 64
+//    per step), so few lanes idle on pairs far from the group;
+//  * the 16 i-atoms are wave-uniform (SGPR operands); i-forces live in 48
+//    registers per lane and are reduced once per item; j-forces are flushed
+//    per step, skipped when zero;
 //  * coordinates are 32-bit fixed point per box edge: the int difference
 //    wraps to the minimum image exactly, then converts to float;
-//  * forces accumulate in FP32 inside a tile and are rounded to int64 2^-40
-//    fixed point per flush, so results are bitwise reproducible.
+//  * forces accumulate in FP32 within a step/item and are rounded to int64
+//    2^-40 fixed point per flush, so results are bitwise reproducible.
 #include <cmath>
 
 #include "system.hpp"
@@ -58,150 +58,126 @@ template <> __device__ __forceinline__ double charge_of<double>(int bits) {
   return (double)__int_as_float(bits);
 }
 
-// Sums v[0..31] over the 64 lanes; lanes 2t and 2t+1 return the total of v[t].
-template <typename CT> __device__ __forceinline__ CT transpose_reduce(CT (&v)[32], int lane) {
+// Sums v[3k + c] (atom k < kGroup, component c) over the 64 lanes.  Each
+// exchange halves the set a lane keeps (xor 32, 16, 8, 4 select atoms 8, 4,
+// 2, 1), then xor 2 and 1 finish the sums: lane L ends with atom (L >> 2) in
+// v[0..2].
+template <typename CT> __device__ __forceinline__ void group_reduce(CT (&v)[3 * kGroup], int lane) {
+  static_assert(kGroup == 16, "group_reduce is written for 16-atom groups");
 #pragma unroll
-  for (int k = 0; k < 16; k++) {
-    const bool up = lane & 32;
-    CT send = up ? v[k] : v[k + 16];
-    CT keep = up ? v[k + 16] : v[k];
-    v[k] = keep + __shfl_xor(send, 32);
-  }
+  for (int w = 24, bit = 32; w >= 3; w >>= 1, bit >>= 1) {
 #pragma unroll
-  for (int k = 0; k < 8; k++) {
-    const bool up = lane & 16;
-    CT send = up ? v[k] : v[k + 8];
-    CT keep = up ? v[k + 8] : v[k];
-    v[k] = keep + __shfl_xor(send, 16);
-  }
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    const bool up = lane & 8;
-    CT send = up ? v[k] : v[k + 4];
-    CT keep = up ? v[k + 4] : v[k];
-    v[k] = keep + __shfl_xor(send, 8);
+    for (int k = 0; k < w; k++) {
+      const bool up = lane & bit;
+      const CT send = up ? v[k] : v[k + w];
+      const CT keep = up ? v[k + w] : v[k];
+      v[k] = keep + __shfl_xor(send, bit);
+    }
   }
 #pragma unroll
-  for (int k = 0; k < 2; k++) {
-    const bool up = lane & 4;
-    CT send = up ? v[k] : v[k + 2];
-    CT keep = up ? v[k + 2] : v[k];
-    v[k] = keep + __shfl_xor(send, 4);
-  }
-  {
-    const bool up = lane & 2;
-    CT send = up ? v[0] : v[1];
-    CT keep = up ? v[1] : v[0];
-    v[0] = keep + __shfl_xor(send, 2);
-  }
-  return v[0] + __shfl_xor(v[0], 1);
+  for (int k = 0; k < 3; k++)
+    for (int off = 2; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off);
 }
 
 constexpr int kMaxTypesLds = 32;  // square vdW table in LDS up to 32 types
 
-// TAB_LDS: vdW table square (ntypes x ntypes) in LDS, row offset carried with
-// the i-atom; otherwise the triangular table is read from global memory.
+// One wave64 = one work item: an i-group of kGroup consecutive sorted slots
+// and up to kItemSteps steps of 64 packed j entries (candidate tile of the
+// i-cell, slot in that j-cell, skip mask).  Each lane holds one j
+// atom per step; the kGroup i-atoms are wave-uniform (SGPR operands) and
+// visited in an unrolled loop, skipped with a scalar branch when no lane has
+// a live pair with them.  i-forces accumulate in 3*kGroup registers and are
+// reduced once per item; j-forces are flushed per step.
+// TAB_LDS: vdW table square (ntypes x ntypes) in LDS; otherwise the
+// triangular table is read from global memory.
 template <typename CT, bool CALC_E, bool CALC_V, bool TAB_LDS>
 __global__ __launch_bounds__(64) void k_pair(
     const int4 *__restrict__ posq, const int *__restrict__ types, const int4 *__restrict__ items,
-    const int *__restrict__ tile_j, const uint32_t *__restrict__ tile_mask,
-    const int *__restrict__ meta, const typename Vec2<CT>::T *__restrict__ vdwtab,
-    const typename Vec2<CT>::T *__restrict__ vdwsq, int ntypes, const PairConsts<CT> P, CT qfac,
-    unsigned long long *__restrict__ fsort, int ss, double *__restrict__ part) {
+    const uint32_t *__restrict__ ent, const int *__restrict__ cand_j, const int *__restrict__ meta,
+    const typename Vec2<CT>::T *__restrict__ vdwtab, const typename Vec2<CT>::T *__restrict__ vdwsq,
+    int ntypes, const PairConsts<CT> P, CT qfac, unsigned long long *__restrict__ fsort, int ss,
+    double *__restrict__ part) {
   using T2 = typename Vec2<CT>::T;
-  __shared__ T2 sh_tab[TAB_LDS ? kMaxTypesLds * kMaxTypesLds : 1];
-  __shared__ int4 sh_i[kCell];
-  __shared__ int sh_row[kCell];
-  __shared__ int sh_tj[kItemTiles];
-  if (TAB_LDS)
+  extern __shared__ char smem_pair[];
+  T2 *sh_tab = reinterpret_cast<T2 *>(smem_pair);  // ntypes^2 entries when TAB_LDS
+  if (TAB_LDS) {
     for (int k = threadIdx.x; k < ntypes * ntypes; k += 64) sh_tab[k] = vdwsq[k];
-
+    __syncthreads();
+  }
   const int lane = threadIdx.x;
-  const int half = lane >> 5, l = lane & 31;
   const int nitem = meta[3];
   double acc[kPairPart];
 #pragma unroll
   for (int k = 0; k < kPairPart; k++) acc[k] = 0.0;
 
-#if APO_PAIR_XCD_MAP
-  // consecutive items (neighbouring i-cells, shared j-cells) on one XCD:
-  // blocks are dispatched round-robin over the 8 XCDs
-  const int nb8 = gridDim.x >> 3;
-  const int first = (gridDim.x & 7) == 0 ? (blockIdx.x & 7) * nb8 + (blockIdx.x >> 3) : blockIdx.x;
-#else
-  const int first = blockIdx.x;
-#endif
-  for (int item = first; item < nitem; item += gridDim.x) {
+  for (int item = blockIdx.x; item < nitem; item += gridDim.x) {
     const int4 it = items[item];
-    const int ibase = it.x * kCell;
-    __syncthreads();  // previous item's readers are done with sh_i
-    if (lane < kCell) {
-      sh_i[lane] = posq[ibase + lane];
-      sh_row[lane] = TAB_LDS ? types[ibase + lane] * ntypes : types[ibase + lane];
-    }
-    if (lane < kItemTiles && it.y + lane < it.z) sh_tj[lane] = tile_j[it.y + lane];
-    __syncthreads();
-    CT fix[32], fiy[32], fiz[32];
+    const int ibase = it.x * kGroup;
+    // i-data to SGPRs: lane k < kGroup loads slot ibase + k
+    const int4 pil = posq[ibase + (lane & (kGroup - 1))];
+    const int til = types[ibase + (lane & (kGroup - 1))];
+    int pix[kGroup], piy[kGroup], piz[kGroup], rowi[kGroup];
+    CT qi[kGroup];
 #pragma unroll
-    for (int t = 0; t < 32; t++) fix[t] = fiy[t] = fiz[t] = (CT)0;
+    for (int k = 0; k < kGroup; k++) {
+      pix[k] = __builtin_amdgcn_readlane(pil.x, k);
+      piy[k] = __builtin_amdgcn_readlane(pil.y, k);
+      piz[k] = __builtin_amdgcn_readlane(pil.z, k);
+      qi[k] = charge_of<CT>(__builtin_amdgcn_readlane(pil.w, k));
+      const int t = __builtin_amdgcn_readlane(til, k);
+      rowi[k] = TAB_LDS ? t * ntypes : t;
+    }
+    CT fi[3 * kGroup];
+#pragma unroll
+    for (int k = 0; k < 3 * kGroup; k++) fi[k] = (CT)0;
 
-    // j-data of the next tile pair is loaded while the current one computes
-    auto jcell_of = [&](int tp) {
-      const int tile = tp + half;
-      return tile < it.z ? sh_tj[tile - it.y] : it.x;
-    };
-    int jc_n = jcell_of(it.y);
-    int4 pj_n = posq[jc_n * kCell + l];
-    int tj_n = types[jc_n * kCell + l];
-    uint32_t m_n = it.y + half < it.z ? tile_mask[(size_t)(it.y + half) * kCell + l] : 0xffffffffu;
-
-    for (int tp = it.y; tp < it.z; tp += 2) {
-      // keep the 32 i-atom LDS reads inside the tile loop (not hoisted into
-      // 160 registers)
-      asm volatile("" ::: "memory");
-      const int tile = tp + half;
-      const bool tv = tile < it.z;
-      const int jc = jc_n;
-      const int jl = jc * kCell + l;
+    // the next step's j data is loaded while the current one computes
+    const int *tiles = cand_j + it.w;
+    auto slot_of = [&](uint32_t e) { return tiles[(e & 0xffffu) >> 5] * kCell + (int)(e & 31u); };
+    uint32_t e_n = ent[it.y + lane];
+    int jl_n = slot_of(e_n);
+    int4 pj_n = posq[jl_n];
+    int tj_n = types[jl_n];
+    for (int e0 = it.y; e0 < it.z; e0 += 64) {
+      const uint32_t e = e_n;
+      const int jl = jl_n;
       const int4 pj = pj_n;
       const int tj = tj_n;
-      const uint32_t m = m_n;
-      if (tp + 2 < it.z) {
-        jc_n = jcell_of(tp + 2);
-        pj_n = posq[jc_n * kCell + l];
-        tj_n = types[jc_n * kCell + l];
-        m_n = tile + 2 < it.z ? tile_mask[(size_t)(tile + 2) * kCell + l] : 0xffffffffu;
+      if (e0 + 64 < it.z) {
+        e_n = ent[e0 + 64 + lane];
+        jl_n = slot_of(e_n);
+        pj_n = posq[jl_n];
+        tj_n = types[jl_n];
       }
-      // i-atoms with no pair inside the list cutoff in either tile are
-      // skipped with a scalar branch
-      uint32_t act = ~m;
+      const uint32_t m = e >> 16;
+      // i-atoms with no live pair in this step are skipped with a scalar branch
+      uint32_t act = ~m & ((1u << kGroup) - 1u);
       for (int off = 32; off > 0; off >>= 1) act |= __shfl_xor(act, off);
       act = __builtin_amdgcn_readfirstlane(act);
       const CT qj = charge_of<CT>(pj.w) * qfac;
       CT fjx = 0, fjy = 0, fjz = 0;
       // energies accumulate per pair in FP64 (large cancelling sums); the
-      // virial per tile in CT
+      // virial per step in CT
       double el = 0, ev = 0;
       CT v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
 #pragma unroll
-      for (int t = 0; t < 32; t++) {
-        if (!(act & (1u << t))) continue;
-        const int4 pi = sh_i[t];
-        const CT dx = (CT)(int)(pi.x - pj.x) * P.sx;
-        const CT dy = (CT)(int)(pi.y - pj.y) * P.sy;
-        const CT dz = (CT)(int)(pi.z - pj.z) * P.sz;
+      for (int k = 0; k < kGroup; k++) {
+        if (!(act & (1u << k))) continue;
+        const CT dx = (CT)(int)(pix[k] - pj.x) * P.sx;
+        const CT dy = (CT)(int)(piy[k] - pj.y) * P.sy;
+        const CT dz = (CT)(int)(piz[k] - pj.z) * P.sz;
         const CT r2 = dx * dx + dy * dy + dz * dz;
-        if (!(m & (1u << t)) && r2 < P.roff2) {
+        if (!(m & (1u << k)) && r2 < P.roff2) {
           const CT rinv = rsqrt_ct(r2);
           const CT r = r2 * rinv;
-          const CT qq = charge_of<CT>(pi.w) * qj;
+          const CT qq = qi[k] * qj;
           CT u_el;
           const CT f_el = ewald_pair(r2, r, rinv, qq, P, u_el);
           T2 c;
           if (TAB_LDS) {
-            c = sh_tab[sh_row[t] + tj];
+            c = sh_tab[rowi[k] + tj];
           } else {
-            const int ti = sh_row[t];
+            const int ti = rowi[k];
             const int aa = ti > tj ? ti : tj, bb = ti > tj ? tj : ti;
             c = vdwtab[aa * (aa + 1) / 2 + bb];
           }
@@ -218,9 +194,9 @@ __global__ __launch_bounds__(64) void k_pair(
           fjx -= gx;
           fjy -= gy;
           fjz -= gz;
-          fix[t] += gx;
-          fiy[t] += gy;
-          fiz[t] += gz;
+          fi[3 * k] += gx;
+          fi[3 * k + 1] += gy;
+          fi[3 * k + 2] += gz;
           if (CALC_E) {
             const CT C6 = outer ? (CT)0 : P.dv6, C12 = outer ? (CT)0 : P.dv12;
             el += (double)u_el;
@@ -237,7 +213,8 @@ __global__ __launch_bounds__(64) void k_pair(
           }
         }
       }
-      if (tv && m != 0xffffffffu) {
+      // j-forces that stayed zero (no pair inside r_off) cost no memory traffic
+      if (fjx != (CT)0 || fjy != (CT)0 || fjz != (CT)0) {
         atomicAdd(&fsort[jl], fx_round(fjx));
         atomicAdd(&fsort[ss + jl], fx_round(fjy));
         atomicAdd(&fsort[2 * (size_t)ss + jl], fx_round(fjz));
@@ -255,15 +232,13 @@ __global__ __launch_bounds__(64) void k_pair(
         acc[7] += (double)v5;
       }
     }
-    const CT sx = transpose_reduce<CT>(fix, lane);
-    const CT sy = transpose_reduce<CT>(fiy, lane);
-    const CT sz = transpose_reduce<CT>(fiz, lane);
-    if (!(lane & 1)) {
-      const int il = ibase + (lane >> 1);
-      if (sx != (CT)0 || sy != (CT)0 || sz != (CT)0) {
-        atomicAdd(&fsort[il], fx_round(sx));
-        atomicAdd(&fsort[ss + il], fx_round(sy));
-        atomicAdd(&fsort[2 * (size_t)ss + il], fx_round(sz));
+    group_reduce<CT>(fi, lane);
+    if ((lane & 3) == 0) {
+      const int il = ibase + (lane >> 2);
+      if (fi[0] != (CT)0 || fi[1] != (CT)0 || fi[2] != (CT)0) {
+        atomicAdd(&fsort[il], fx_round(fi[0]));
+        atomicAdd(&fsort[ss + il], fx_round(fi[1]));
+        atomicAdd(&fsort[2 * (size_t)ss + il], fx_round(fi[2]));
       }
     }
   }
@@ -288,12 +263,13 @@ static void launch_pair_t(NbSystem &S, const typename Vec2<CT>::T *tab,
   double *part = S.parts.get() + S.part.pair_off;
   const int G = S.pair_blocks;
   const bool lds = S.ntypes <= kMaxTypesLds;
+  const size_t lds_bytes = (size_t)S.ntypes * S.ntypes * sizeof(typename Vec2<CT>::T);
 #define APO_PAIR_ARGS                                                                   \
-  S.posq.get(), S.type_s.get(), S.items.get(), S.tile_j.get(), S.tile_mask.get(),     \
-      S.meta.get(), tab, sq, S.ntypes, P, qfac, f, S.ssort, part
+  S.posq.get(), S.type_s.get(), S.items.get(), S.ent.get(), S.cand_j.get(), S.meta.get(), tab, \
+      sq, S.ntypes, P, qfac, f, S.ssort, part
 #define APO_PAIR_GO(EE, VV)                                                             \
   if (lds)                                                                              \
-    k_pair<CT, EE, VV, true><<<G, 64, 0, s>>>(APO_PAIR_ARGS);                          \
+    k_pair<CT, EE, VV, true><<<G, 64, lds_bytes, s>>>(APO_PAIR_ARGS);                  \
   else                                                                                  \
     k_pair<CT, EE, VV, false><<<G, 64, 0, s>>>(APO_PAIR_ARGS);
   if (E && V) {

@@ -594,14 +594,15 @@ static void recip_t(NbSystem &S, const float4 *x, int64_t *force, int stride, bo
   const size_t lds_cap = 64 * 1024;
   // x forward
   const int nrows = K1 * K2;
-  int rb = (int)std::max<size_t>(1, std::min<size_t>(16, (lds_cap / sizeof(C2) - K0) / (2 * K0)));
+  int rb = (int)std::max<size_t>(
+      1, std::min<size_t>(kFftRowsPerBlock, (lds_cap / sizeof(C2) - K0) / (2 * K0)));
   size_t shx = (size_t)(K0 + 2 * rb * K0) * sizeof(C2);
   const int npair = (nrows + 1) / 2;
   k_fftx_fwd<CT><<<(npair + rb - 1) / rb, 256, shx, s>>>(rgrid, nrows, plan_of(S.fx), rb, twx,
                                                         cgrid);
   APO_LAUNCH_CHECK();
   // y forward
-  const int W = 16;
+  const int W = kFftColsPerBlock;
   const int wy = (int)std::max<size_t>(1, std::min<size_t>(W, (lds_cap / sizeof(C2) - K1) / (2 * K1)));
   size_t shy = (size_t)(K1 + 2 * wy * K1) * sizeof(C2);
   const int ncy = (H + wy - 1) / wy;

@@ -401,9 +401,9 @@ NbSystem *create_system(const apo_nb_params &p, const apo_nb_topology &t) {
     S->atom_slot.reserve(n);
     S->colkeys.reserve(n);
     S->zkey.reserve(n);
-    for (auto *b : {&S->icell_cnt, &S->icell_start, &S->icell_act, &S->icell_actstart,
-                    &S->icell_nitem, &S->icell_itemstart})
-      b->reserve(S->ncell_cap + 1);
+    for (auto *b : {&S->icell_cnt, &S->icell_start}) b->reserve(S->ncell_cap + 1);
+    for (auto *b : {&S->grp_cnt, &S->grp_start, &S->grp_nitem, &S->grp_itemstart})
+      b->reserve((size_t)S->ncell_cap * kGroupsPerCell + 1);
 
     // partials layout (fixed block counts -> deterministic reductions)
     S->pair_blocks = kPairBlocksPerCu * S->ncu;

@@ -57,7 +57,7 @@ struct NbSystem {
   // changes (list buffers regrown, new box); captured step graphs compare it
   unsigned long long generation = 0;
   std::vector<int> h_bonds;  // 2 * nbond, host copy for molecule grouping
-  DevBuf<int> meta;   // [0] ncell [1] ncand [2] ntile [3] nitem [4] maxcol
+  DevBuf<int> meta;   // [0] ncell [1] ncand [2] nent [3] nitem [4] maxcol
   DevBuf<int> col_count, col_start, col_ncell, col_cellstart;
   DevBuf<int> atom_col, atom_slot;
   DevBuf<unsigned long long> colkeys;
@@ -65,15 +65,18 @@ struct NbSystem {
   DevBuf<int> loc2glo, glo2loc;
   DevBuf<float4> cell_bb;  // 2 per cell: center, half extent
   DevBuf<int> icell_cnt, icell_start;
-  DevBuf<int> icell_act, icell_actstart, icell_nitem, icell_itemstart;
   DevBuf<int> cand_i, cand_j;
   DevBuf<uint32_t> cand_mask;
   DevBuf<int> cand_active;
-  DevBuf<int> tile_j;
-  DevBuf<uint32_t> tile_mask;
+  // pair list: per i-group (kGroup consecutive slots of a cell) the j slots
+  // with at least one live pair, packed as (candidate tile index << 5 | slot)
+  // | skip-mask << 16 and padded to whole 64-lane steps; items = (group,
+  // first entry, end entry, first candidate tile of the group's cell)
+  DevBuf<int> grp_cnt, grp_start, grp_nitem, grp_itemstart;
+  DevBuf<uint32_t> ent;
   DevBuf<int4> items;
-  size_t cand_cap = 0, tile_cap = 0, item_cap = 0;
-  int host_ncell = 0, host_ncand = 0, host_ntile = 0, host_nitem = 0;
+  size_t cand_cap = 0, ent_cap = 0, item_cap = 0;
+  int host_ncell = 0, host_ncand = 0, host_nent = 0, host_nitem = 0;
 
   // ---- per-step direct-space buffers -----------------------------------------
   DevBuf<int4> posq;       // fixed-point coords (x,y,z as uint32) + scaled q

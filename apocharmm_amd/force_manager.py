@@ -182,7 +182,7 @@ class ForceManager:
     def listStats(self):
         out = (ctypes.c_int64 * 6)()
         _lib.check(_lib.lib().apo_nb_list_stats(self._handle, out, self._stream_ptr()))
-        keys = ("cells", "tiles", "items", "pairs_in_list", "columns", "max_column")
+        keys = ("cells", "entries", "items", "pairs_in_list", "columns", "max_column")
         return dict(zip(keys, list(out)))
 
     def close(self):

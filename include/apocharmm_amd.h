@@ -203,9 +203,10 @@ APO_API apo_status apo_nb_force_to_double(const int64_t *d_force,
                                           double *d_out, void *stream);
 
 /* Neighbor-list statistics of the last build (synchronises the stream):
- * out[0] cells, out[1] tiles (32x32 cell pairs after pruning),
- * out[2] work items, out[3] atom pairs inside the list cutoff covered by the
- * tiles, out[4] columns, out[5] largest column population.  The pair count
+ * out[0] cells, out[1] j entries of the pair list (8-slot i-groups times the
+ * j slots with a live pair, padded to 64-lane steps), out[2] work items,
+ * out[3] atom pairs inside the list cutoff covered by the list, out[4]
+ * columns, out[5] largest column population.  The pair count
  * is the reference's debug cross-check quantity
  * (src/CudaNeighborListBuild.cu:2215-2264). */
 APO_API apo_status apo_nb_list_stats(apo_nb_system *sys, int64_t *out6,
