@@ -27,7 +27,10 @@ __device__ __forceinline__ uint32_t to_fixed(double x, double inv_len) {
   return (uint32_t)(int64_t)llrint(f * 4294967296.0);
 }
 
-// 1. column of each atom (wrapped xy) and its slot in the column.
+// 1. column of each atom (wrapped xy) and its slot in the column.  Atoms
+// that are neighbours in index order usually share a column, so the lanes of
+// a wave that hit the same column take their slots with one atomic (the
+// first such lane adds the group's size, the others take consecutive slots).
 __global__ void k_assign(const float4 *__restrict__ xyzq, int n, double3 inv_len, double3 len,
                          int ncx, int ncy, int *__restrict__ col_count, int *__restrict__ atom_col,
                          int *__restrict__ atom_slot, float *__restrict__ zkey,
@@ -47,8 +50,22 @@ __global__ void k_assign(const float4 *__restrict__ xyzq, int n, double3 inv_len
   int cx = max(0, min((int)(fx * ncx), ncx - 1)), cy = max(0, min((int)(fy * ncy), ncy - 1));
   int col = cx + ncx * cy;
   atom_col[i] = col;
-  atom_slot[i] = atomicAdd(&col_count[col], 1);
   zkey[i] = (float)(fz * len.z);
+  const int lane = threadIdx.x & 63;
+  unsigned long long todo = __ballot(true);
+  int slot = 0;
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int lcol = __shfl(col, leader, 64);
+    const unsigned long long peers = __ballot(col == lcol) & todo;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&col_count[lcol], __popcll(peers));
+    base = __shfl(base, leader, 64);
+    if (col == lcol && ((todo >> lane) & 1ull))
+      slot = base + __popcll(peers & ((1ull << lane) - 1ull));
+    todo &= ~peers;
+  }
+  atom_slot[i] = slot;
 }
 
 // 2. cells per column; largest column.
@@ -231,52 +248,61 @@ __global__ __launch_bounds__(64) void k_tiles(const float4 *__restrict__ bb,
 
 // 7. 32x32 masks: bit t of lane l set <=> pair (i-cell atom t, j-cell atom l)
 // is skipped: padding, diagonal i >= j, topological exclusion (1-2/1-3/1-4),
-// or farther than the list cutoff at build time.
-__global__ __launch_bounds__(64) void k_tile_mask(
-    const int *__restrict__ cand_i, const int *__restrict__ cand_j, int ncand,
-    const int4 *__restrict__ posq, const int *__restrict__ loc2glo, const int *__restrict__ glo2loc,
+// or farther than the list cutoff at build time.  Four waves per i-cell walk
+// its candidate tiles two at a time (lanes 0-31 and 32-63); each wave loads
+// the i-cell's atoms once and reads them as wave-uniform values.
+constexpr int kMaskWaves = 4;
+__global__ __launch_bounds__(64 * kMaskWaves) void k_tile_mask(
+    const int *__restrict__ meta, const int *__restrict__ icell_start,
+    const int *__restrict__ cand_j, const int4 *__restrict__ posq,
+    const int *__restrict__ loc2glo, const int *__restrict__ glo2loc,
     const int *__restrict__ excl_ptr, const int *__restrict__ excl_idx, float3 unit, float rl2,
-    uint32_t *__restrict__ cand_mask, int *__restrict__ cand_active) {
-  const int lane = threadIdx.x, l = lane & 31;
-  const int tile = blockIdx.x * 2 + (lane >> 5);
-  const bool tv = tile < ncand;
-  uint32_t m = 0xffffffffu;
-  if (tv) {
-    const int ic = cand_i[tile], jc = cand_j[tile];
+    uint32_t *__restrict__ cand_mask) {
+  const int ic = blockIdx.x;
+  if (ic >= meta[0]) return;
+  const int lane = threadIdx.x & 63, l = lane & 31, half = lane >> 5;
+  const int wave = threadIdx.x >> 6;
+  const int4 pil = posq[ic * kCell + l];
+  const int gil = loc2glo[ic * kCell + l];
+  uint32_t ipad = 0;  // bit t: i slot t is padding
+  {
+    const unsigned long long b = __ballot(gil < 0);
+    ipad = (uint32_t)b;
+  }
+  const int s = icell_start[ic], e = icell_start[ic + 1];
+  for (int k0 = s + 2 * wave; k0 < e; k0 += 2 * kMaskWaves) {
+    const int k = k0 + half;
+    const bool tv = k < e;
+    const int jc = tv ? cand_j[k] : ic;
     const int jl = jc * kCell + l;
     const int gj = loc2glo[jl];
-    if (gj >= 0) {
-      m = 0u;
-      const int4 pj = posq[jl];
-      for (int t = 0; t < kCell; t++) {
-        const int il = ic * kCell + t;
-        bool bad = loc2glo[il] < 0 || (ic == jc && t >= l);
-        if (!bad) {
-          const int4 pi = posq[il];
-          float dx = (float)(int)(pi.x - pj.x) * unit.x;
-          float dy = (float)(int)(pi.y - pj.y) * unit.y;
-          float dz = (float)(int)(pi.z - pj.z) * unit.z;
-          bad = dx * dx + dy * dy + dz * dz > rl2;
-        }
-        if (bad) m |= 1u << t;
-      }
-      for (int e = excl_ptr[gj]; e < excl_ptr[gj + 1]; e++) {
-        int lp = glo2loc[excl_idx[e]];
+    uint32_t m = 0xffffffffu;
+    const int4 pj = posq[jl];
+    if (tv && gj >= 0) m = ipad | (jc == ic ? ~((1u << l) - 1u) : 0u);
+#pragma unroll 8
+    for (int t = 0; t < kCell; t++) {
+      const int px = __builtin_amdgcn_readlane(pil.x, t);
+      const int py = __builtin_amdgcn_readlane(pil.y, t);
+      const int pz = __builtin_amdgcn_readlane(pil.z, t);
+      const float dx = (float)(int)(px - pj.x) * unit.x;
+      const float dy = (float)(int)(py - pj.y) * unit.y;
+      const float dz = (float)(int)(pz - pj.z) * unit.z;
+      if (dx * dx + dy * dy + dz * dz > rl2) m |= 1u << t;
+    }
+    if (tv && gj >= 0)
+      for (int x = excl_ptr[gj]; x < excl_ptr[gj + 1]; x++) {
+        const int lp = glo2loc[excl_idx[x]];
         if ((lp >> 5) == ic) m |= 1u << (lp & 31);
       }
-    }
-    cand_mask[(size_t)tile * kCell + l] = m;
+    if (tv) cand_mask[(size_t)k * kCell + l] = m;
   }
-  unsigned long long b = __ballot(tv && m != 0xffffffffu);
-  unsigned hb = (lane < 32) ? (unsigned)b : (unsigned)(b >> 32);
-  if (tv && l == 0) cand_active[tile] = hb != 0u;
 }
 
 // 8. per i-group (kGroup consecutive slots of a cell) the number of j slots
 // with at least one live pair in the 32x32 masks, rounded up to whole 64-lane
 // steps, and the number of work items of <= kItemSteps steps.  Cells beyond
 // the live count write zeros so one scan covers every group.
-__global__ __launch_bounds__(64) void k_grp_count(const int *__restrict__ meta,
+__global__ __launch_bounds__(64 * kGroupsPerCell) void k_grp_count(const int *__restrict__ meta,
                                                   const int *__restrict__ icell_start,
                                                   const uint32_t *__restrict__ cand_mask,
                                                   int *__restrict__ grp_cnt,
@@ -284,70 +310,58 @@ __global__ __launch_bounds__(64) void k_grp_count(const int *__restrict__ meta,
                                                   int *__restrict__ meta_w) {
   constexpr uint32_t kFull = (1u << kGroup) - 1u;
   const int icell = blockIdx.x;
-  const int lane = threadIdx.x, l = lane & 31, half = lane >> 5;
-  int cnt[kGroupsPerCell] = {};
+  const int lane = threadIdx.x & 63, l = lane & 31, half = lane >> 5;
+  const int h = threadIdx.x >> 6;  // one wave per group
+  int cnt = 0;
   if (icell < meta[0]) {
     const int s = icell_start[icell], e = icell_start[icell + 1];
     // the entry format indexes at most kMaxCellTiles tiles per cell
-    if (e - s > kMaxCellTiles && lane == 0) atomicOr(&meta_w[7], 1);
+    if (e - s > kMaxCellTiles && threadIdx.x == 0) atomicOr(&meta_w[7], 1);
     for (int k0 = s; k0 < e; k0 += 2) {
       const int k = k0 + half;
       const uint32_t m = k < e ? cand_mask[(size_t)k * kCell + l] : 0xffffffffu;
-#pragma unroll
-      for (int h = 0; h < kGroupsPerCell; h++)
-        cnt[h] += __popcll(__ballot(((m >> (kGroup * h)) & kFull) != kFull));
+      cnt += __popcll(__ballot(((m >> (kGroup * h)) & kFull) != kFull));
     }
   }
-  if (lane == 0)
-#pragma unroll
-    for (int h = 0; h < kGroupsPerCell; h++) {
-      const int c = (cnt[h] + 63) / 64 * 64;
-      grp_cnt[icell * kGroupsPerCell + h] = c;
-      grp_nitem[icell * kGroupsPerCell + h] = (c + 64 * kItemSteps - 1) / (64 * kItemSteps);
-    }
+  if (lane == 0) {
+    const int c = (cnt + 63) / 64 * 64;
+    grp_cnt[icell * kGroupsPerCell + h] = c;
+    grp_nitem[icell * kGroupsPerCell + h] = (c + 64 * kItemSteps - 1) / (64 * kItemSteps);
+  }
 }
 
 // 9. entries (deterministic order: candidate tiles in j order, slots
 // ascending), null padding (full skip mask on the cell's own first slot) and
 // work items.
-__global__ __launch_bounds__(64) void k_grp_emit(
+__global__ __launch_bounds__(64 * kGroupsPerCell) void k_grp_emit(
     const int *__restrict__ meta, const int *__restrict__ icell_start,
     const uint32_t *__restrict__ cand_mask, const int *__restrict__ grp_start,
     const int *__restrict__ grp_itemstart, uint32_t *__restrict__ ent, int4 *__restrict__ items) {
   constexpr uint32_t kFull = (1u << kGroup) - 1u;
   const int icell = blockIdx.x;
   if (icell >= meta[0]) return;
-  const int lane = threadIdx.x, l = lane & 31, half = lane >> 5;
+  const int lane = threadIdx.x & 63, l = lane & 31, half = lane >> 5;
+  const int h = threadIdx.x >> 6;  // one wave per group
   const int s = icell_start[icell], e = icell_start[icell + 1];
   if (e - s > kMaxCellTiles) return;  // flagged in k_grp_count; the host raises
-  int run[kGroupsPerCell] = {};
+  const int g = icell * kGroupsPerCell + h;
+  const int st = grp_start[g], en = grp_start[g + 1];
+  int run = 0;
   const unsigned long long below = (1ull << lane) - 1ull;
   for (int k0 = s; k0 < e; k0 += 2) {
     const int k = k0 + half;
     const uint32_t m = k < e ? cand_mask[(size_t)k * kCell + l] : 0xffffffffu;
-    const uint32_t jloc = (uint32_t)((k - s) << 5 | l);
-#pragma unroll
-    for (int h = 0; h < kGroupsPerCell; h++) {
-      const uint32_t mg = (m >> (kGroup * h)) & kFull;
-      const bool live = mg != kFull;
-      const unsigned long long b = __ballot(live);
-      if (live) {
-        const int pos = grp_start[icell * kGroupsPerCell + h] + run[h] + __popcll(b & below);
-        ent[pos] = jloc | (mg << 16);
-      }
-      run[h] += __popcll(b);
-    }
+    const uint32_t mg = (m >> (kGroup * h)) & kFull;
+    const bool live = mg != kFull;
+    const unsigned long long b = __ballot(live);
+    if (live) ent[st + run + __popcll(b & below)] = (uint32_t)((k - s) << 5 | l) | (mg << 16);
+    run += __popcll(b);
   }
-#pragma unroll
-  for (int h = 0; h < kGroupsPerCell; h++) {
-    const int g = icell * kGroupsPerCell + h;
-    const int st = grp_start[g], en = grp_start[g + 1];
-    for (int p = st + run[h] + lane; p < en; p += 64) ent[p] = kFull << 16;
-    const int i0 = grp_itemstart[g], i1 = grp_itemstart[g + 1];
-    for (int it = i0 + lane; it < i1; it += 64) {
-      const int a = st + (it - i0) * 64 * kItemSteps;
-      items[it] = make_int4(g, a, min(a + 64 * kItemSteps, en), s);
-    }
+  for (int p = st + run + lane; p < en; p += 64) ent[p] = kFull << 16;
+  const int i0 = grp_itemstart[g], i1 = grp_itemstart[g + 1];
+  for (int it = i0 + lane; it < i1; it += 64) {
+    const int a = st + (it - i0) * 64 * kItemSteps;
+    items[it] = make_int4(g, a, min(a + 64 * kItemSteps, en), s);
   }
 }
 
@@ -409,20 +423,16 @@ void nblist_build(NbSystem &S, const float4 *x, hipStream_t s) {
     S.cand_i.reserve(S.cand_cap);
     S.cand_j.reserve(S.cand_cap);
     S.cand_mask.reserve(S.cand_cap * kCell);
-    S.cand_active.reserve(S.cand_cap);
   }
   k_tiles<<<S.ncell_cap, 64, 0, s>>>(S.cell_bb.get(), meta, Lf, rl2, nullptr, S.icell_start.get(),
                                      S.cand_i.get(), S.cand_j.get(), 1);
   APO_LAUNCH_CHECK();
-  if (S.host_ncand > 0) {
-    k_tile_mask<<<nblk(S.host_ncand, 2), 64, 0, s>>>(
-        S.cand_i.get(), S.cand_j.get(), S.host_ncand, S.posq.get(), S.loc2glo.get(),
-        S.glo2loc.get(), S.excl_ptr.get(), S.excl_idx.get(), unit, rl2, S.cand_mask.get(),
-        S.cand_active.get());
-    APO_LAUNCH_CHECK();
-  }
+  k_tile_mask<<<S.ncell_cap, 64 * kMaskWaves, 0, s>>>(meta, S.icell_start.get(), S.cand_j.get(), S.posq.get(),
+                                         S.loc2glo.get(), S.glo2loc.get(), S.excl_ptr.get(),
+                                         S.excl_idx.get(), unit, rl2, S.cand_mask.get());
+  APO_LAUNCH_CHECK();
   const int ngrp = S.ncell_cap * kGroupsPerCell;
-  k_grp_count<<<S.ncell_cap, 64, 0, s>>>(meta, S.icell_start.get(), S.cand_mask.get(),
+  k_grp_count<<<S.ncell_cap, 64 * kGroupsPerCell, 0, s>>>(meta, S.icell_start.get(), S.cand_mask.get(),
                                          S.grp_cnt.get(), S.grp_nitem.get(), meta);
   APO_LAUNCH_CHECK();
   launch_scan(S.grp_cnt.get(), S.grp_start.get(), nullptr, ngrp, s);
@@ -448,7 +458,7 @@ void nblist_build(NbSystem &S, const float4 *x, hipStream_t s) {
     S.generation++;
     S.items.reserve(S.item_cap);
   }
-  k_grp_emit<<<S.ncell_cap, 64, 0, s>>>(meta, S.icell_start.get(), S.cand_mask.get(),
+  k_grp_emit<<<S.ncell_cap, 64 * kGroupsPerCell, 0, s>>>(meta, S.icell_start.get(), S.cand_mask.get(),
                                         S.grp_start.get(), S.grp_itemstart.get(), S.ent.get(),
                                         S.items.get());
   APO_LAUNCH_CHECK();

@@ -477,11 +477,11 @@ void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int s
     }
     S.tic(7, s, t0);
     nblist_sort_coords(S, x4, s);
-    if (rec) nblist_cellbb(S, s);
     S.toc(7, s, t0);
     if (rec) {
       APO_HIP(hipEventRecord(S.ev_sorted, s));
       APO_HIP(hipStreamWaitEvent(S.side[0], S.ev_sorted, 0));
+      nblist_cellbb(S, S.side[0]);  // only the spread reads the cell boxes
       recip_launch(S, x4, force, stride, E, V, S.side[0]);
       APO_HIP(hipEventRecord(S.ev_pme, S.side[0]));
     }

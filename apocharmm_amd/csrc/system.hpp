@@ -67,7 +67,6 @@ struct NbSystem {
   DevBuf<int> icell_cnt, icell_start;
   DevBuf<int> cand_i, cand_j;
   DevBuf<uint32_t> cand_mask;
-  DevBuf<int> cand_active;
   // pair list: per i-group (kGroup consecutive slots of a cell) the j slots
   // with at least one live pair, packed as (candidate tile index << 5 | slot)
   // | skip-mask << 16 and padded to whole 64-lane steps; items = (group,
