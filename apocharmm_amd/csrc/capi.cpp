@@ -22,6 +22,7 @@ LtSystem *create_lt(NbSystem *nb, const apo_lt_params &p, const double *xyz, con
 void destroy_lt(LtSystem *L);
 void lt_initialize(LtSystem &L, hipStream_t caller);
 void lt_propagate(LtSystem &L, int nsteps, hipStream_t caller);
+void lt_propagate_batch(LtSystem **Ls, int n, int nsteps, hipStream_t caller);
 void lt_get_state(LtSystem &L, double *xyz, double *vel, double *dprev, double *grad,
                   double *thermo, int64_t *step);
 const float *lt_xyzq(LtSystem &L);
@@ -110,7 +111,7 @@ APO_API apo_status apo_nb_reset_list(apo_nb_system *sys, const float *d_xyzq, vo
   return guard([&] {
     if (!d_xyzq) throw apo::InvalidArgument("null xyzq");
     apo::nblist_build(get(sys), reinterpret_cast<const float4 *>(d_xyzq),
-                      reinterpret_cast<hipStream_t>(stream));
+                      reinterpret_cast<hipStream_t>(stream), true);
   });
 }
 
@@ -224,6 +225,16 @@ APO_API apo_status apo_lt_initialize(apo_lt_integrator *lt, void *stream) {
 
 APO_API apo_status apo_lt_propagate(apo_lt_integrator *lt, int32_t nsteps, void *stream) {
   return guard([&] { apo::lt_propagate(get(lt), nsteps, static_cast<hipStream_t>(stream)); });
+}
+
+APO_API apo_status apo_lt_propagate_batch(apo_lt_integrator *const *lts, int32_t n,
+                                          int32_t nsteps, void *stream) {
+  return guard([&] {
+    if (n > 0 && !lts) throw apo::InvalidArgument("null integrator array");
+    std::vector<apo::LtSystem *> v((size_t)(n > 0 ? n : 0));
+    for (int i = 0; i < n; i++) v[i] = &get(lts[i]);
+    apo::lt_propagate_batch(v.data(), n, nsteps, static_cast<hipStream_t>(stream));
+  });
 }
 
 APO_API apo_status apo_lt_get_state(apo_lt_integrator *lt, double *xyz, double *vel,

@@ -135,7 +135,9 @@ void launch_scan(const int *in, int *out, const int *n_dev, int n_max,
 void launch_force_to_double(const int64_t *f, int natom, int stride,
                             double *out, hipStream_t s);
 
-void nblist_build(NbSystem &sys, const float4 *d_xyzq, hipStream_t s);
+void nblist_build(NbSystem &sys, const float4 *d_xyzq, hipStream_t s, bool sizing);
+void list_export_check(NbSystem &sys, hipStream_t s);
+void list_reserve(NbSystem &sys, int ncand, int nent, int nitem);
 void nblist_sort_coords(NbSystem &sys, const float4 *d_xyzq, hipStream_t s);
 void nblist_cellbb(NbSystem &sys, hipStream_t s);
 void nblist_stats(NbSystem &sys, int64_t *out6, hipStream_t s);

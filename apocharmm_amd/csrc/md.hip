@@ -26,6 +26,7 @@ namespace apo {
 
 void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int stride,
                 double *energy, double *virial, hipStream_t s);
+void list_export_check(NbSystem &S, hipStream_t s);
 
 namespace {
 
@@ -461,11 +462,14 @@ struct LtSystem {
   bool initialized = false;
   hipStream_t st = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  unsigned long long graph_gen = ~0ull;
+  // captured step bodies: [0] plain step, [1] step with image centering and
+  // list rebuild; each is recaptured when the force system's generation moves
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  unsigned long long graph_gen[2] = {~0ull, ~0ull};
 
   ~LtSystem() {
-    if (gexec) (void)hipGraphExecDestroy(gexec);
+    for (hipGraphExec_t g : gexec)
+      if (g) (void)hipGraphExecDestroy(g);
     if (ev_in) (void)hipEventDestroy(ev_in);
     if (ev_out) (void)hipEventDestroy(ev_out);
     if (st) (void)hipStreamDestroy(st);
@@ -524,9 +528,15 @@ struct LtSystem {
     nb->toc(8, s, t0);
   }
 
-  // force + integrator update + kinetic energy; the body of a graph replay
+  // force + integrator update + kinetic energy
   void step_body(hipStream_t s) {
     force(s, false);
+    update(s);
+  }
+  // the same after image centering and a list rebuild
+  void rebuild_body(hipStream_t s) {
+    center(s);
+    force(s, true);
     update(s);
   }
 
@@ -539,53 +549,67 @@ struct LtSystem {
     initialized = true;
   }
 
-  bool graph_usable() const { return use_graph && !nb->prof_on && nb->list_valid; }
+  bool graph_usable() const {
+    return use_graph && !nb->prof_on && nb->list_valid && nb->cand_cap > 0 && nb->ent_cap > 0 &&
+           nb->item_cap > 0;
+  }
 
-  void replay(hipStream_t s) {
-    if (!gexec || graph_gen != nb->generation) {
-      if (gexec) {
-        APO_HIP(hipGraphExecDestroy(gexec));
-        gexec = nullptr;
+  void replay(int which, hipStream_t s) {
+    hipGraphExec_t &ex = gexec[which];
+    if (!ex || graph_gen[which] != nb->generation) {
+      if (ex) {
+        APO_HIP(hipGraphExecDestroy(ex));
+        ex = nullptr;
       }
       hipGraph_t g = nullptr;
       APO_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       try {
-        step_body(s);
+        if (which) rebuild_body(s);
+        else step_body(s);
       } catch (...) {
         (void)hipStreamEndCapture(s, &g);
         if (g) (void)hipGraphDestroy(g);
         throw;
       }
       APO_HIP(hipStreamEndCapture(s, &g));
-      hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+      hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       APO_HIP(e);
-      graph_gen = nb->generation;
+      graph_gen[which] = nb->generation;
     }
-    APO_HIP(hipGraphLaunch(gexec, s));
+    APO_HIP(hipGraphLaunch(ex, s));
   }
 
   // CudaIntegrator::propagate (src/CudaIntegrator.cu:140-230) with the
   // thermostat's propagateOneStep (src/CudaLangevinThermostatIntegrator.cu:703-800).
   void propagate(int nsteps, hipStream_t s) {
+    begin(s);
+    for (int k = 0; k < nsteps; k++) one_step(s);
+  }
+
+  void begin(hipStream_t s) {
     if (!initialized) initialize(s);
     // propagate() opens with resetNeighborList(), whose image centering the
     // first step then repeats
     center(s);
     steps_since_rebuild = 0;
-    for (int k = 0; k < nsteps; k++) {
+  }
+
+  void one_step(hipStream_t s) {
+    {
       const bool rebuild = steps_since_rebuild % nbfreq == 0;
       const bool com = comfreq > 0 && step % (unsigned long long)comfreq == 0;
-      if (rebuild) {
-        center(s);
-        force(s, true);  // list rebuild with this step's force
-        if (com) remove_com(s);
-        update(s);
-      } else if (com || !graph_usable()) {
-        if (com) remove_com(s);
-        step_body(s);
+      // centre-of-mass removal only touches the half-step displacements, so
+      // it may precede the force call
+      if (com) remove_com(s);
+      if (!graph_usable()) {
+        if (rebuild) rebuild_body(s);
+        else step_body(s);
       } else {
-        replay(s);
+        // errors and capacity growth of earlier (device-side) rebuilds are
+        // handled before the next one is queued
+        if (rebuild) list_export_check(*nb, s);
+        replay(rebuild ? 1 : 0, s);
       }
       steps_since_rebuild++;
       step++;
@@ -771,9 +795,50 @@ void lt_propagate(LtSystem &L, int nsteps, hipStream_t caller) {
   on_stream(L, caller, [&](hipStream_t s) { L.propagate(nsteps, s); });
 }
 
+// Many independent systems advanced together: step k of every system is
+// queued before step k+1 of any, each system on its own stream (so their
+// kernels overlap on the device), with no host synchronisation between
+// list rebuilds.  Per system the work and its order are exactly those of
+// lt_propagate, so every system's trajectory is bitwise the same as alone.
+void lt_propagate_batch(LtSystem **Ls, int n, int nsteps, hipStream_t caller) {
+  if (n < 0 || nsteps < 0) throw InvalidArgument("negative count");
+  for (int i = 0; i < n; i++)
+    if (!Ls[i]) throw InvalidArgument("null integrator in batch");
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < i; j++)
+      if (Ls[i] == Ls[j] || Ls[i]->nb == Ls[j]->nb)
+        throw InvalidArgument("a batch must not hold one system twice");
+  for (int i = 0; i < n; i++) {
+    APO_HIP(hipEventRecord(Ls[i]->ev_in, caller));
+    APO_HIP(hipStreamWaitEvent(Ls[i]->st, Ls[i]->ev_in, 0));
+  }
+  // one stream per system: no cross-stream fork/join inside a system's step
+  std::vector<bool> mode(n);
+  for (int i = 0; i < n; i++) {
+    mode[i] = Ls[i]->nb->single_stream;
+    if (!mode[i]) {
+      Ls[i]->nb->single_stream = true;
+      Ls[i]->nb->generation++;  // captured steps have the other shape
+    }
+  }
+  for (int i = 0; i < n; i++) Ls[i]->begin(Ls[i]->st);
+  for (int k = 0; k < nsteps; k++)
+    for (int i = 0; i < n; i++) Ls[i]->one_step(Ls[i]->st);
+  for (int i = 0; i < n; i++)
+    if (!mode[i]) {
+      Ls[i]->nb->single_stream = false;
+      Ls[i]->nb->generation++;
+    }
+  for (int i = 0; i < n; i++) {
+    APO_HIP(hipEventRecord(Ls[i]->ev_out, Ls[i]->st));
+    APO_HIP(hipStreamWaitEvent(caller, Ls[i]->ev_out, 0));
+  }
+}
+
 void lt_get_state(LtSystem &L, double *xyz, double *vel, double *dprev, double *grad,
                   double *thermo, int64_t *step) {
   APO_HIP(hipStreamSynchronize(L.st));
+  list_export_check(*L.nb, L.st);  // every rebuild so far has finished
   const int n = L.n;
   std::vector<double> soa(3 * (size_t)n);
   auto from_soa = [&](const double *dev, double *out) {

@@ -223,7 +223,7 @@ __global__ __launch_bounds__(64) void k_tiles(const float4 *__restrict__ bb,
                                               int *__restrict__ icell_cnt,
                                               const int *__restrict__ icell_start,
                                               int *__restrict__ cand_i, int *__restrict__ cand_j,
-                                              int emit) {
+                                              int emit, int cand_cap) {
   const int icell = blockIdx.x;
   const int ncell = meta[0];
   if (icell >= ncell) return;
@@ -238,8 +238,10 @@ __global__ __launch_bounds__(64) void k_tiles(const float4 *__restrict__ bb,
     unsigned long long m = __ballot(hit);
     if (emit && hit) {
       int pos = base + count + __popcll(m & ((1ull << lane) - 1ull));
-      cand_i[pos] = icell;
-      cand_j[pos] = j;
+      if (pos < cand_cap) {  // overflow is flagged by k_list_guard
+        cand_i[pos] = icell;
+        cand_j[pos] = j;
+      }
     }
     count += __popcll(m);
   }
@@ -257,7 +259,7 @@ __global__ __launch_bounds__(64 * kMaskWaves) void k_tile_mask(
     const int *__restrict__ cand_j, const int4 *__restrict__ posq,
     const int *__restrict__ loc2glo, const int *__restrict__ glo2loc,
     const int *__restrict__ excl_ptr, const int *__restrict__ excl_idx, float3 unit, float rl2,
-    uint32_t *__restrict__ cand_mask) {
+    uint32_t *__restrict__ cand_mask, int cand_cap) {
   const int ic = blockIdx.x;
   if (ic >= meta[0]) return;
   const int lane = threadIdx.x & 63, l = lane & 31, half = lane >> 5;
@@ -269,7 +271,7 @@ __global__ __launch_bounds__(64 * kMaskWaves) void k_tile_mask(
     const unsigned long long b = __ballot(gil < 0);
     ipad = (uint32_t)b;
   }
-  const int s = icell_start[ic], e = icell_start[ic + 1];
+  const int s = icell_start[ic], e = min(icell_start[ic + 1], cand_cap);
   for (int k0 = s + 2 * wave; k0 < e; k0 += 2 * kMaskWaves) {
     const int k = k0 + half;
     const bool tv = k < e;
@@ -307,7 +309,7 @@ __global__ __launch_bounds__(64 * kGroupsPerCell) void k_grp_count(const int *__
                                                   const uint32_t *__restrict__ cand_mask,
                                                   int *__restrict__ grp_cnt,
                                                   int *__restrict__ grp_nitem,
-                                                  int *__restrict__ meta_w) {
+                                                  int *__restrict__ meta_w, int cand_cap) {
   constexpr uint32_t kFull = (1u << kGroup) - 1u;
   const int icell = blockIdx.x;
   const int lane = threadIdx.x & 63, l = lane & 31, half = lane >> 5;
@@ -319,7 +321,7 @@ __global__ __launch_bounds__(64 * kGroupsPerCell) void k_grp_count(const int *__
     if (e - s > kMaxCellTiles && threadIdx.x == 0) atomicOr(&meta_w[7], 1);
     for (int k0 = s; k0 < e; k0 += 2) {
       const int k = k0 + half;
-      const uint32_t m = k < e ? cand_mask[(size_t)k * kCell + l] : 0xffffffffu;
+      const uint32_t m = k < e && k < cand_cap ? cand_mask[(size_t)k * kCell + l] : 0xffffffffu;
       cnt += __popcll(__ballot(((m >> (kGroup * h)) & kFull) != kFull));
     }
   }
@@ -342,6 +344,7 @@ __global__ __launch_bounds__(64 * kGroupsPerCell) void k_grp_emit(
   if (icell >= meta[0]) return;
   const int lane = threadIdx.x & 63, l = lane & 31, half = lane >> 5;
   const int h = threadIdx.x >> 6;  // one wave per group
+  if (meta[8]) return;  // the list overflowed a capacity: k_list_guard emptied it
   const int s = icell_start[icell], e = icell_start[icell + 1];
   if (e - s > kMaxCellTiles) return;  // flagged in k_grp_count; the host raises
   const int g = icell * kGroupsPerCell + h;
@@ -365,9 +368,83 @@ __global__ __launch_bounds__(64 * kGroupsPerCell) void k_grp_emit(
   }
 }
 
+// 10. After the counts are known: any capacity overflow or input error
+// leaves the list empty (no pair kernel work, no out-of-range write) and sets
+// meta[8]; the host learns of it from the exported counts.
+__global__ void k_list_guard(int *__restrict__ meta, int cand_cap, int ent_cap, int item_cap) {
+  if (threadIdx.x != 0) return;
+  const bool over = meta[1] > cand_cap || meta[2] > ent_cap || meta[3] > item_cap;
+  if (over || meta[5] || meta[6] || meta[7]) {
+    meta[8] = over ? 1 : 2;
+    meta[3] = 0;
+  }
+}
+
+// 11. Copies the build's counts and flags to pinned host memory, with the
+// running build number in slot 15 (meta[16] survives the per-build clear).
+__global__ void k_meta_export(int *__restrict__ meta, int *__restrict__ host) {
+  if (threadIdx.x == 0) {
+    const int seq = meta[16] + 1;
+    meta[16] = seq;
+    for (int k = 0; k < 15; k++) host[k] = meta[k];
+    __threadfence_system();
+    host[15] = seq;
+    __threadfence_system();
+  }
+}
+
 static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
 
-void nblist_build(NbSystem &S, const float4 *x, hipStream_t s) {
+// Host side of the exported counts: errors of a finished build raise here;
+// capacities that the last finished build used above 85 % grow (the device
+// is drained first, captured step graphs see the new generation).
+void list_export_check(NbSystem &S, hipStream_t s) {
+  const volatile int *h = S.h_meta;
+  if (!h || h[15] == 0) return;
+  int m[16];
+  for (int k = 0; k < 16; k++) m[k] = h[k];
+  if (m[15] == S.meta_checked) return;
+  S.meta_checked = m[15];
+  if (m[6]) throw RuntimeError("non-finite coordinates in xyzq at a neighbor-list rebuild");
+  if (m[5]) throw RuntimeError("neighbor list: an xy column exceeds the LDS sort capacity");
+  if (m[7]) throw RuntimeError("neighbor list: a cell has more than 2048 candidate tiles");
+  if (m[8] == 1)
+    throw RuntimeError("neighbor list: a rebuild exceeded the list capacity; the forces of the "
+                       "steps after it are invalid (capacity was sized from earlier builds)");
+  const bool grow = m[1] > 0.85 * S.cand_cap || m[2] > 0.85 * S.ent_cap || m[3] > 0.85 * S.item_cap;
+  if (grow) {
+    APO_HIP(hipStreamSynchronize(s));
+    APO_HIP(hipDeviceSynchronize());
+    list_reserve(S, m[1], m[2], m[3]);
+  }
+}
+
+void list_reserve(NbSystem &S, int ncand, int nent, int nitem) {
+  if ((size_t)ncand > S.cand_cap * 0.85 || S.cand_cap == 0) {
+    S.cand_cap = (size_t)(ncand * 1.5) + 64;
+    S.cand_i.reserve(S.cand_cap);
+    S.cand_j.reserve(S.cand_cap);
+    S.cand_mask.reserve(S.cand_cap * kCell);
+    S.generation++;
+  }
+  if ((size_t)nent > S.ent_cap * 0.85 || S.ent_cap == 0) {
+    S.ent_cap = (size_t)(nent * 1.5) + 64 * kItemSteps;
+    S.ent.reserve(S.ent_cap);
+    S.generation++;
+  }
+  if ((size_t)nitem > S.item_cap * 0.85 || S.item_cap == 0) {
+    S.item_cap = (size_t)(nitem * 1.5) + 64;
+    S.items.reserve(S.item_cap);
+    S.generation++;
+  }
+}
+
+// Builds the list from xyzq.  sizing = true synchronises twice to size the
+// buffers exactly and raises input errors at once (the first build and
+// apo_nb_reset_list); otherwise the build is host-sync free, guarded by the
+// capacities, and its errors surface at the next check of the exported
+// counts (so a rebuild can be part of a captured step graph).
+void nblist_build(NbSystem &S, const float4 *x, hipStream_t s, bool sizing) {
   const int n = S.natom;
   const double3 inv = make_double3(1.0 / S.box[0], 1.0 / S.box[1], 1.0 / S.box[2]);
   const double3 len = make_double3(S.box[0], S.box[1], S.box[2]);
@@ -376,6 +453,12 @@ void nblist_build(NbSystem &S, const float4 *x, hipStream_t s) {
   const float3 Lf = make_float3((float)S.box[0], (float)S.box[1], (float)S.box[2]);
   const float rl2 = (float)(S.cutnb * S.cutnb);
   int *meta = S.meta.get();
+  sizing = sizing || S.cand_cap == 0 || S.ent_cap == 0 || S.item_cap == 0;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  APO_HIP(hipStreamIsCapturing(s, &cap));
+  if (sizing && cap != hipStreamCaptureStatusNone)
+    throw RuntimeError("the first neighbor-list build cannot be captured in a graph");
+  if (!sizing && cap == hipStreamCaptureStatusNone) list_export_check(S, s);
 
   launch_fill_i32(meta, 16, s);
   launch_fill_i32(S.col_count.get(), S.ncol + 1, s);
@@ -400,67 +483,62 @@ void nblist_build(NbSystem &S, const float4 *x, hipStream_t s) {
                                                 S.cell_bb.get());
   APO_LAUNCH_CHECK();
   k_tiles<<<S.ncell_cap, 64, 0, s>>>(S.cell_bb.get(), meta, Lf, rl2, S.icell_cnt.get(), nullptr,
-                                     nullptr, nullptr, 0);
+                                     nullptr, nullptr, 0, 0);
   APO_LAUNCH_CHECK();
   launch_scan(S.icell_cnt.get(), S.icell_start.get(), meta + 0, S.ncell_cap, s);
   k_copy_total<<<1, 64, 0, s>>>(S.icell_start.get(), meta + 0, 0, meta + 1);
   APO_LAUNCH_CHECK();
   int hmeta[8];
-  APO_HIP(hipMemcpyAsync(hmeta, meta, sizeof(hmeta), hipMemcpyDeviceToHost, s));
-  APO_HIP(hipStreamSynchronize(s));
-  if (hmeta[6]) {
-    S.list_valid = false;
-    throw RuntimeError("non-finite coordinates in xyzq");
+  if (sizing) {
+    APO_HIP(hipMemcpyAsync(hmeta, meta, sizeof(hmeta), hipMemcpyDeviceToHost, s));
+    APO_HIP(hipStreamSynchronize(s));
+    if (hmeta[6]) {
+      S.list_valid = false;
+      throw RuntimeError("non-finite coordinates in xyzq");
+    }
+    if (hmeta[5]) {
+      S.list_valid = false;
+      throw RuntimeError("neighbor list: an xy column exceeds the LDS sort capacity");
+    }
+    S.host_ncell = hmeta[0];
+    S.host_ncand = hmeta[1];
+    list_reserve(S, hmeta[1], 0, 0);
   }
-  if (hmeta[5]) {
-    S.list_valid = false;
-    throw RuntimeError("neighbor list: an xy column exceeds the LDS sort capacity");
-  }
-  S.host_ncell = hmeta[0];
-  S.host_ncand = hmeta[1];
-  if ((size_t)S.host_ncand > S.cand_cap) {
-    S.cand_cap = (size_t)(S.host_ncand * 1.25) + 64;
-    S.cand_i.reserve(S.cand_cap);
-    S.cand_j.reserve(S.cand_cap);
-    S.cand_mask.reserve(S.cand_cap * kCell);
-  }
+  const int ccap = (int)S.cand_cap;
   k_tiles<<<S.ncell_cap, 64, 0, s>>>(S.cell_bb.get(), meta, Lf, rl2, nullptr, S.icell_start.get(),
-                                     S.cand_i.get(), S.cand_j.get(), 1);
+                                     S.cand_i.get(), S.cand_j.get(), 1, ccap);
   APO_LAUNCH_CHECK();
-  k_tile_mask<<<S.ncell_cap, 64 * kMaskWaves, 0, s>>>(meta, S.icell_start.get(), S.cand_j.get(), S.posq.get(),
-                                         S.loc2glo.get(), S.glo2loc.get(), S.excl_ptr.get(),
-                                         S.excl_idx.get(), unit, rl2, S.cand_mask.get());
+  k_tile_mask<<<S.ncell_cap, 64 * kMaskWaves, 0, s>>>(
+      meta, S.icell_start.get(), S.cand_j.get(), S.posq.get(), S.loc2glo.get(), S.glo2loc.get(),
+      S.excl_ptr.get(), S.excl_idx.get(), unit, rl2, S.cand_mask.get(), ccap);
   APO_LAUNCH_CHECK();
   const int ngrp = S.ncell_cap * kGroupsPerCell;
-  k_grp_count<<<S.ncell_cap, 64 * kGroupsPerCell, 0, s>>>(meta, S.icell_start.get(), S.cand_mask.get(),
-                                         S.grp_cnt.get(), S.grp_nitem.get(), meta);
+  k_grp_count<<<S.ncell_cap, 64 * kGroupsPerCell, 0, s>>>(
+      meta, S.icell_start.get(), S.cand_mask.get(), S.grp_cnt.get(), S.grp_nitem.get(), meta, ccap);
   APO_LAUNCH_CHECK();
   launch_scan(S.grp_cnt.get(), S.grp_start.get(), nullptr, ngrp, s);
   launch_scan(S.grp_nitem.get(), S.grp_itemstart.get(), nullptr, ngrp, s);
   k_copy_total<<<1, 64, 0, s>>>(S.grp_start.get(), nullptr, ngrp, meta + 2);
   k_copy_total<<<1, 64, 0, s>>>(S.grp_itemstart.get(), nullptr, ngrp, meta + 3);
   APO_LAUNCH_CHECK();
-  APO_HIP(hipMemcpyAsync(hmeta, meta, sizeof(hmeta), hipMemcpyDeviceToHost, s));
-  APO_HIP(hipStreamSynchronize(s));
-  if (hmeta[7]) {
-    S.list_valid = false;
-    throw RuntimeError("neighbor list: a cell has more than 2048 candidate tiles");
+  if (sizing) {
+    APO_HIP(hipMemcpyAsync(hmeta, meta, sizeof(hmeta), hipMemcpyDeviceToHost, s));
+    APO_HIP(hipStreamSynchronize(s));
+    if (hmeta[7]) {
+      S.list_valid = false;
+      throw RuntimeError("neighbor list: a cell has more than 2048 candidate tiles");
+    }
+    S.host_nent = hmeta[2];
+    S.host_nitem = hmeta[3];
+    list_reserve(S, 0, hmeta[2], hmeta[3]);
   }
-  S.host_nent = hmeta[2];
-  S.host_nitem = hmeta[3];
-  if ((size_t)S.host_nent > S.ent_cap) {
-    S.ent_cap = (size_t)(S.host_nent * 1.25) + 64 * kItemSteps;
-    S.generation++;
-    S.ent.reserve(S.ent_cap);
-  }
-  if ((size_t)S.host_nitem > S.item_cap) {
-    S.item_cap = (size_t)(S.host_nitem * 1.25) + 64;
-    S.generation++;
-    S.items.reserve(S.item_cap);
-  }
+  k_list_guard<<<1, 64, 0, s>>>(meta, ccap, (int)S.ent_cap, (int)S.item_cap);
+  APO_LAUNCH_CHECK();
   k_grp_emit<<<S.ncell_cap, 64 * kGroupsPerCell, 0, s>>>(meta, S.icell_start.get(), S.cand_mask.get(),
                                         S.grp_start.get(), S.grp_itemstart.get(), S.ent.get(),
                                         S.items.get());
+  APO_LAUNCH_CHECK();
+  k_meta_export<<<1, 64, 0, s>>>(meta, S.h_meta);
   APO_LAUNCH_CHECK();
   S.list_valid = true;
 }
@@ -493,17 +571,19 @@ __global__ void k_count_pairs(const uint32_t *__restrict__ ent, int nent,
 }
 
 void nblist_stats(NbSystem &S, int64_t *out6, hipStream_t s) {
+  int hmeta[16];
+  APO_HIP(hipMemcpyAsync(hmeta, S.meta.get(), sizeof(hmeta), hipMemcpyDeviceToHost, s));
+  APO_HIP(hipStreamSynchronize(s));
+  list_export_check(S, s);
+  const int nent = hmeta[8] ? 0 : std::min(hmeta[2], (int)S.ent_cap);
   DevBuf<unsigned long long> cnt;
   cnt.reserve(1);
   APO_HIP(hipMemsetAsync(cnt.get(), 0, sizeof(unsigned long long), s));
-  if (S.list_valid && S.host_nent > 0)
-    k_count_pairs<<<nblk((long)S.host_nent, 256), 256, 0, s>>>(S.ent.get(), S.host_nent,
-                                                               cnt.get());
+  if (S.list_valid && nent > 0)
+    k_count_pairs<<<nblk((long)nent, 256), 256, 0, s>>>(S.ent.get(), nent, cnt.get());
   APO_LAUNCH_CHECK();
   unsigned long long pairs = 0;
-  int hmeta[8];
   APO_HIP(hipMemcpyAsync(&pairs, cnt.get(), sizeof(pairs), hipMemcpyDeviceToHost, s));
-  APO_HIP(hipMemcpyAsync(hmeta, S.meta.get(), sizeof(hmeta), hipMemcpyDeviceToHost, s));
   APO_HIP(hipStreamSynchronize(s));
   out6[0] = hmeta[0];
   out6[1] = hmeta[2];

@@ -392,7 +392,10 @@ NbSystem *create_system(const apo_nb_params &p, const apo_nb_topology &t) {
     S->glo2loc.reserve(n);
     launch_fill_i32(S->glo2loc.get(), n, 0);
     S->cell_bb.reserve(2 * (size_t)S->ncell_cap);
-    S->meta.reserve(16);
+    S->meta.reserve(32);
+    launch_fill_i32(S->meta.get(), 32, 0);
+    APO_HIP(hipHostMalloc(reinterpret_cast<void **>(&S->h_meta), 16 * sizeof(int)));
+    for (int k = 0; k < 16; k++) S->h_meta[k] = 0;
     S->col_count.reserve(S->ncol + 1);
     S->col_start.reserve(S->ncol + 1);
     S->col_ncell.reserve(S->ncol + 1);
@@ -459,31 +462,37 @@ void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int s
   // `force`:  bonded + 1-4 (side[1], int64 atomics into force) | pair kernel
   // (caller's stream, atomics into the sorted buffer) | PME spread, FFT,
   // gather (side[0], atomics into force).
+  const bool fork = !S.single_stream;
+  hipStream_t sb = fork ? S.side[1] : s, sp = fork ? S.side[0] : s;
   if (bnd) {
-    APO_HIP(hipEventRecord(S.ev_fork, s));
-    APO_HIP(hipStreamWaitEvent(S.side[1], S.ev_fork, 0));
-    S.tic(6, S.side[1], t1);
-    if (terms & APO_TERM_PAIRS14) pairs14_launch(S, x4, force, stride, E, V, S.side[1]);
-    if (terms & APO_TERM_BONDED) bonded_launch(S, x4, force, stride, E, V, S.side[1]);
-    S.toc(6, S.side[1], t1);
-    APO_HIP(hipEventRecord(S.ev_bond, S.side[1]));
+    if (fork) {
+      APO_HIP(hipEventRecord(S.ev_fork, s));
+      APO_HIP(hipStreamWaitEvent(sb, S.ev_fork, 0));
+    }
+    S.tic(6, sb, t1);
+    if (terms & APO_TERM_PAIRS14) pairs14_launch(S, x4, force, stride, E, V, sb);
+    if (terms & APO_TERM_BONDED) bonded_launch(S, x4, force, stride, E, V, sb);
+    S.toc(6, sb, t1);
+    if (fork) APO_HIP(hipEventRecord(S.ev_bond, sb));
   }
   if (dir || rec) {
     // both the pair kernel and PME work on the cell-sorted coordinates
     if ((flags & APO_RESET_LIST) || !S.list_valid) {
       S.tic(4, s, t0);
-      nblist_build(S, x4, s);
+      nblist_build(S, x4, s, false);
       S.toc(4, s, t0);
     }
     S.tic(7, s, t0);
     nblist_sort_coords(S, x4, s);
     S.toc(7, s, t0);
     if (rec) {
-      APO_HIP(hipEventRecord(S.ev_sorted, s));
-      APO_HIP(hipStreamWaitEvent(S.side[0], S.ev_sorted, 0));
-      nblist_cellbb(S, S.side[0]);  // only the spread reads the cell boxes
-      recip_launch(S, x4, force, stride, E, V, S.side[0]);
-      APO_HIP(hipEventRecord(S.ev_pme, S.side[0]));
+      if (fork) {
+        APO_HIP(hipEventRecord(S.ev_sorted, s));
+        APO_HIP(hipStreamWaitEvent(sp, S.ev_sorted, 0));
+      }
+      nblist_cellbb(S, sp);  // only the spread reads the cell boxes
+      recip_launch(S, x4, force, stride, E, V, sp);
+      if (fork) APO_HIP(hipEventRecord(S.ev_pme, sp));
     }
   }
   if (dir) {
@@ -491,8 +500,8 @@ void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int s
     direct_launch(S, E, V, s);
     S.toc(0, s, t0);
   }
-  if (rec) APO_HIP(hipStreamWaitEvent(s, S.ev_pme, 0));
-  if (bnd) APO_HIP(hipStreamWaitEvent(s, S.ev_bond, 0));
+  if (fork && rec) APO_HIP(hipStreamWaitEvent(s, S.ev_pme, 0));
+  if (fork && bnd) APO_HIP(hipStreamWaitEvent(s, S.ev_bond, 0));
   if (dir || rec) {
     S.tic(7, s, t0);
     direct_finalize(S, force, stride, s);
@@ -543,6 +552,7 @@ void NbSystem::profile_flush() {
 }
 
 NbSystem::~NbSystem() {
+  if (h_meta) (void)hipHostFree(h_meta);
   for (auto e : ev_pool) (void)hipEventDestroy(e);
   for (hipEvent_t e : {ev_fork, ev_sorted, ev_pme, ev_bond})
     if (e) (void)hipEventDestroy(e);

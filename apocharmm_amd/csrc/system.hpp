@@ -76,6 +76,10 @@ struct NbSystem {
   DevBuf<int4> items;
   size_t cand_cap = 0, ent_cap = 0, item_cap = 0;
   int host_ncell = 0, host_ncand = 0, host_nent = 0, host_nitem = 0;
+  // counts and flags of each build, exported to pinned host memory (slot 15
+  // = build number) and checked lazily on the host
+  int *h_meta = nullptr;
+  int meta_checked = 0;
 
   // ---- per-step direct-space buffers -----------------------------------------
   DevBuf<int4> posq;       // fixed-point coords (x,y,z as uint32) + scaled q
@@ -102,6 +106,7 @@ struct NbSystem {
   // calc_force runs PME and the bonded/1-4 terms on two side streams that
   // fork from and join back into the caller's stream
   hipStream_t side[2] = {nullptr, nullptr};
+  bool single_stream = false;  // everything on the caller's stream (batched systems)
   hipEvent_t ev_fork = nullptr, ev_sorted = nullptr, ev_pme = nullptr, ev_bond = nullptr;
 
   // HIP-event kernel timers (apo_nb_profile)

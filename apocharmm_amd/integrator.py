@@ -191,6 +191,18 @@ class LangevinThermostatIntegrator:
             pass
 
 
+def propagate_batch(integrators, numSteps):
+    """Advances several LangevinThermostatIntegrator objects (each with its own
+    ForceManager) by numSteps together, their kernels overlapping on the GPU
+    (apo_lt_propagate_batch).  Each trajectory is bitwise the one propagate()
+    gives the system alone."""
+    torch = _torch()
+    arr = (ctypes.c_void_p * len(integrators))(*[lt._handle.value for lt in integrators])
+    _lib.check(_lib.lib().apo_lt_propagate_batch(
+        arr, len(integrators), int(numSteps),
+        ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+
 def maxwell_velocities(masses, temperature, seed):
     """Gaussian velocities with variance kB T / m per component (A/AKMA) and
     zero mass-weighted mean, as CharmmContext::assignVelocitiesAtTemperature

@@ -1,7 +1,8 @@
 """Benchmark: ns/day of NVT Langevin dynamics on the 98,304-atom TIP3P PME box.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1|c2|c3|c4]
                     [--precision mixed|fp64] [--step md|force] [--no-cpu-baseline]
+                    [--nsys 64] [--hw-queues 16]      (c4: batched systems)
 
 One "step" (default --step md) is one leapfrog Langevin step at dt = 2 fs,
 friction 1/ps, 300 K with SETTLE rigid waters: the full force path (bonded +
@@ -80,6 +81,89 @@ def cpu_baseline(spec, threads, md):
                       "threads), %.3f s each; ns/day = dt*86.4/t_step" % (reps, what, threads, t_eval)}
 
 
+def bench_batched(a):
+    """C4: 64 independent TIP3P boxes of 500-8,000 waters (log-uniform, seed
+    4001), each with its own box and ~1 A PME grid, NVT Langevin + SETTLE at
+    2 fs.  Aggregate ns/day = systems x steps x dt / wall, batched (all systems
+    advanced together, apo_lt_propagate_batch) against the same systems run
+    one after another through the single-system path."""
+    import numpy as np
+    import torch
+    from apocharmm_amd import systems
+    from apocharmm_amd.force_manager import ForceManager
+    from apocharmm_amd.integrator import (LangevinThermostatIntegrator, maxwell_velocities,
+                                          propagate_batch)
+
+    torch.cuda.set_device(0)
+    sizes = systems.batched_sizes(a.nsys)
+    specs = [systems.batched_system(i, int(w)) for i, w in enumerate(sizes)]
+    natoms = [len(sp["charges"]) for sp in specs]
+
+    def make():
+        out = []
+        for i, sp in enumerate(specs):
+            fm = ForceManager(sp, a.precision).initialize()
+            lt = LangevinThermostatIntegrator(DT_PS, friction=FRICTION, temperature=TEMP,
+                                              seed=SEED + i)
+            lt.setNonbondedListUpdateFrequency(REBUILD_EVERY)
+            lt.setCharmmContext(fm, np.asarray(sp["xyz"], np.float64),
+                                maxwell_velocities(sp["masses"], TEMP, 4200 + i))
+            lt.initialize()
+            out.append((fm, lt))
+        torch.cuda.synchronize()
+        return out
+
+    runs = make()
+    # warm both paths (graph capture, first rebuilds)
+    for fm, lt in runs:
+        lt.propagate(a.warmup)
+    propagate_batch([lt for _, lt in runs], a.warmup)
+    torch.cuda.synchronize()
+    t_seq = 0.0
+    for fm, lt in runs:
+        t0 = time.perf_counter()
+        lt.propagate(a.steps)
+        torch.cuda.synchronize()
+        t_seq += time.perf_counter() - t0
+    t0 = time.perf_counter()
+    propagate_batch([lt for _, lt in runs], a.steps)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    n = len(runs)
+    agg = lambda t: n * a.steps * DT_PS / 1000.0 / (t / 86400.0)
+    for fm, lt in runs:
+        lt.close()
+        fm.close()
+    out = {
+        "metric": "aggregate ns/day, %d uneven TIP3P PME boxes (NVT Langevin MD, 2 fs), batched"
+                  % n,
+        "value": round(agg(t_batch), 3),
+        "unit": "ns/day",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(t_batch / a.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+int64" if a.precision == "mixed" else "fp64+int64",
+        "data": "synthetic (seeded TIP3P lattices, random orientations, Maxwell velocities)",
+        "config": {"workload": "c4_%d_tip3p_boxes_%d_to_%d_atoms" % (n, min(natoms), max(natoms)),
+                   "systems": n, "atoms_total": int(sum(natoms)),
+                   "atoms_min_max": [int(min(natoms)), int(max(natoms))],
+                   "dt_fs": 2.0, "friction_per_ps": FRICTION, "temperature_K": TEMP,
+                   "list_rebuild_every": REBUILD_EVERY, "precision": a.precision,
+                   "gpu_max_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
+                   "parallelism": "single GPU, one stream per system"},
+        "sequential": {"value": round(agg(t_seq), 3), "unit": "ns/day",
+                       "ms_per_step_all_systems": round(t_seq / a.steps * 1e3, 4)},
+        "speedup_vs_sequential": round(t_seq / t_batch, 3),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,11 +173,19 @@ def main():
     ap.add_argument("--precision", default="mixed")
     ap.add_argument("--step", choices=("md", "force"), default="md")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--nsys", type=int, default=64, help="systems in the c4 batch")
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="GPU_MAX_HW_QUEUES for the c4 batch (streams that run concurrently)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build runs one system per process on a "
               "single MI355X; --gpus %d is not supported" % a.gpus, file=sys.stderr)
         sys.exit(2)
+    if a.config == "c4":
+        # hardware queues are fixed when the HIP runtime starts in this process
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(max(a.hw_queues, 1), 32))
+        bench_batched(a)
+        return
 
     import ctypes
 

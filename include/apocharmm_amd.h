@@ -292,6 +292,17 @@ APO_API apo_status apo_lt_initialize(apo_lt_integrator *lt, void *stream);
 APO_API apo_status apo_lt_propagate(apo_lt_integrator *lt, int32_t nsteps,
                                     void *stream);
 
+/* Advances n independent integrators (each bound to its own force system) by
+ * nsteps together: step k of every system is queued before step k+1 of any,
+ * each system on its own stream, so their kernels overlap on the GPU and no
+ * host synchronisation happens between list rebuilds.  Each system's
+ * trajectory is bitwise the one apo_lt_propagate gives it alone.  No
+ * counterpart in the reference, which runs one CharmmContext per process; it
+ * serves many-system workloads (replica exchange, FEP windows, separate
+ * jobs). */
+APO_API apo_status apo_lt_propagate_batch(apo_lt_integrator *const *lts, int32_t n,
+                                          int32_t nsteps, void *stream);
+
 /* Copies the state to the host (synchronises): any output may be NULL.
  *  xyz, vel, dprev, grad: double[3*natom] AoS; vel is the on-step velocity of
  *    the last step, dprev the last half-step displacement (getCoordsDelta),

@@ -162,3 +162,47 @@ def test_md_langevin_temperature(c1):
     assert abs(T[:, 1].mean() / TEMP - 1) < 0.03
     lt.close()
     fm.close()
+
+
+def test_md_batch_bitwise_equals_alone():
+    """apo_lt_propagate_batch: each system's trajectory is bitwise the one it
+    gets alone (same kernels, same order per system, integer force sums)."""
+    from apocharmm_amd.integrator import maxwell_velocities, propagate_batch
+    sizes = [500, 650, 830, 1100]
+    specs = [systems.batched_system(i, w) for i, w in enumerate(sizes)]
+
+    def fresh():
+        out = []
+        for i, sp in enumerate(specs):
+            v = maxwell_velocities(sp["masses"], TEMP, 4200 + i)
+            out.append(_make(sp, np.asarray(sp["xyz"], np.float64), v, seed=SEED + i, nbfreq=10))
+        return out
+
+    alone = []
+    for fm, lt in fresh():
+        lt.propagate(25)
+        alone.append(lt.getState())
+        lt.close()
+        fm.close()
+    runs = fresh()
+    propagate_batch([lt for _, lt in runs], 25)
+    for (fm, lt), ref in zip(runs, alone):
+        st = lt.getState()
+        assert st["step"] == ref["step"] == 25
+        for k in ("xyz", "vel", "dprev", "grad"):
+            assert np.array_equal(st[k], ref[k]), k
+        lt.close()
+        fm.close()
+
+
+def test_reset_list_rejects_non_finite_coordinates():
+    from apocharmm_amd import _lib
+    from apocharmm_amd.force_manager import ForceManager
+    sp, xyz = _centered_c1()
+    fm = ForceManager(sp).initialize()
+    x = systems.xyzq_of(sp, xyz)
+    x[17, 1] = np.nan
+    t = torch.tensor(x, device="cuda")
+    with pytest.raises(_lib.ApoError, match="non-finite"):
+        fm.resetNeighborList(t)
+    fm.close()
