@@ -100,13 +100,17 @@ def lib():
         L.apo_lt_initialize.argtypes = [vp, vp]
         L.apo_lt_propagate.argtypes = [vp, i32, vp]
         L.apo_lt_propagate_batch.argtypes = [ctypes.POINTER(vp), i32, i32, vp]
+        L.apo_lt_batch_create.argtypes = [ctypes.POINTER(vp), i32, i32, ctypes.POINTER(vp)]
+        L.apo_lt_batch_propagate.argtypes = [vp, i32, vp]
+        L.apo_lt_batch_destroy.argtypes = [vp]
         L.apo_lt_get_state.argtypes = [vp, vp, vp, vp, vp, vp, i64p]
         L.apo_lt_xyzq.argtypes = [vp, ctypes.POINTER(vp)]
         for name in ("apo_nb_create", "apo_nb_destroy", "apo_nb_set_box", "apo_nb_reset_list",
                      "apo_nb_calc_force", "apo_nb_force_stride", "apo_nb_num_atoms",
                      "apo_nb_force_to_double", "apo_nb_list_stats", "apo_topology_exclusions",
                      "apo_lt_create", "apo_lt_destroy", "apo_lt_initialize", "apo_lt_propagate",
-                     "apo_lt_propagate_batch",
+                     "apo_lt_propagate_batch", "apo_lt_batch_create", "apo_lt_batch_propagate",
+                     "apo_lt_batch_destroy",
                      "apo_lt_get_state", "apo_lt_xyzq"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L

@@ -106,14 +106,20 @@ __device__ __forceinline__ void dihedral_geom(const float4 *x, int i, int j, int
   st = rg * rab * (A[0] * H[0] + A[1] * H[1] + A[2] * H[2]);
 }
 
+struct BondedBatchArgs {
+  BondedArgs a;
+  unsigned long long *f;
+};
+
 template <typename CT, bool CALC_E, bool CALC_V>
-__global__ __launch_bounds__(256) void k_bonded(BondedArgs a, unsigned long long *__restrict__ f,
-                                                double *__restrict__ part) {
+__device__ __forceinline__ void bonded_body(int bid, const BondedArgs &a,
+                                            unsigned long long *__restrict__ f,
+                                            double *__restrict__ part) {
   __shared__ double sh[4 * 16];
   double acc[kBondPart];
 #pragma unroll
   for (int k = 0; k < kBondPart; k++) acc[k] = 0.0;
-  int idx = blockIdx.x * 256 + threadIdx.x;
+  int idx = bid * 256 + threadIdx.x;
   const float4 *x = a.xyzq;
   if (idx < a.nb + a.nub) {
     const bool ub = idx >= a.nb;
@@ -222,8 +228,20 @@ __global__ __launch_bounds__(256) void k_bonded(BondedArgs a, unsigned long long
       vir_add<CT>(acc + 5, H[0] - G[0], H[1] - G[1], H[2] - G[2], gl[0], gl[1], gl[2]);
     }
   }
-  if (CALC_E || CALC_V) block_partials<256>(acc, kBondPart, sh, part + blockIdx.x * kBondPart);
+  if (CALC_E || CALC_V) block_partials<256>(acc, kBondPart, sh, part + bid * kBondPart);
 }
+template <typename CT, bool CALC_E, bool CALC_V>
+__global__ __launch_bounds__(256) void k_bonded(BondedArgs a, unsigned long long *__restrict__ f,
+                                                double *__restrict__ part) {
+  bonded_body<CT, CALC_E, CALC_V>(blockIdx.x, a, f, part);
+}
+__global__ __launch_bounds__(256) void kb_bonded(const BondedBatchArgs *__restrict__ tab,
+                                                 const int *__restrict__ bstart, int nsys) {
+  const int s = batch_system(bstart, nsys);
+  const BondedBatchArgs &a = tab[s];
+  bonded_body<float, false, false>(blockIdx.x - bstart[s], a.a, a.f, nullptr);
+}
+
 
 // 1-4 pairs and 1-2/1-3 Ewald exclusion corrections.
 struct P14Args {
@@ -234,6 +252,9 @@ struct P14Args {
   int ntypes;
   double e14fac;
 };
+
+struct BondedBatchArgs;
+struct P14BatchArgs;
 
 template <typename CT> struct Tab2;
 template <> struct Tab2<float> { using T = float2; };
@@ -246,16 +267,23 @@ template <typename CT> __device__ __forceinline__ CT exp_ct(CT x);
 template <> __device__ __forceinline__ float exp_ct<float>(float x) { return expf(x); }
 template <> __device__ __forceinline__ double exp_ct<double>(double x) { return exp(x); }
 
+struct P14BatchArgs {
+  P14Args a;
+  PairConsts<float> P;
+  const float2 *tab14;
+  unsigned long long *f;
+};
+
 template <typename CT, bool CALC_E, bool CALC_V>
-__global__ __launch_bounds__(256) void k_pairs14(P14Args a, const PairConsts<CT> P,
-                                                 const typename Tab2<CT>::T *__restrict__ tab14,
-                                                 unsigned long long *__restrict__ f,
-                                                 double *__restrict__ part) {
+__device__ __forceinline__ void pairs14_body(int bid, const P14Args &a, const PairConsts<CT> &P,
+                                             const typename Tab2<CT>::T *__restrict__ tab14,
+                                             unsigned long long *__restrict__ f,
+                                             double *__restrict__ part) {
   __shared__ double sh[4 * 16];
   double acc[kP14Part];
 #pragma unroll
   for (int k = 0; k < kP14Part; k++) acc[k] = 0.0;
-  const int idx = blockIdx.x * 256 + threadIdx.x;
+  const int idx = bid * 256 + threadIdx.x;
   if (idx < a.nin + a.nex) {
     const bool ex = idx >= a.nin;
     const int2 pr = ex ? a.ex14[idx - a.nin] : a.in14[idx];
@@ -291,8 +319,22 @@ __global__ __launch_bounds__(256) void k_pairs14(P14Args a, const PairConsts<CT>
     add_grad<CT>(f, a.stride, pr.y, -fij * dx, -fij * dy, -fij * dz);
     if (CALC_V) vir_add<CT>(acc + 3, dx, dy, dz, fij * dx, fij * dy, fij * dz);
   }
-  if (CALC_E || CALC_V) block_partials<256>(acc, kP14Part, sh, part + blockIdx.x * kP14Part);
+  if (CALC_E || CALC_V) block_partials<256>(acc, kP14Part, sh, part + bid * kP14Part);
 }
+template <typename CT, bool CALC_E, bool CALC_V>
+__global__ __launch_bounds__(256) void k_pairs14(P14Args a, const PairConsts<CT> P,
+                                                 const typename Tab2<CT>::T *__restrict__ tab14,
+                                                 unsigned long long *__restrict__ f,
+                                                 double *__restrict__ part) {
+  pairs14_body<CT, CALC_E, CALC_V>(blockIdx.x, a, P, tab14, f, part);
+}
+__global__ __launch_bounds__(256) void kb_pairs14(const P14BatchArgs *__restrict__ tab,
+                                                  const int *__restrict__ bstart, int nsys) {
+  const int s = batch_system(bstart, nsys);
+  const P14BatchArgs &a = tab[s];
+  pairs14_body<float, false, false>(blockIdx.x - bstart[s], a.a, a.P, a.tab14, a.f, nullptr);
+}
+
 
 template <typename CT, typename F>
 static void dispatch_ev(bool E, bool V, F &&fn) {
@@ -346,6 +388,47 @@ void pairs14_launch(NbSystem &S, const float4 *x, int64_t *force, int stride, bo
           <<<G, 256, 0, s>>>(a, S.pc_f, S.vdwtab14.get(), f, part);
   };
   dispatch_ev<float>(E, V, go);
+  APO_LAUNCH_CHECK();
+}
+
+// ---- fused launches over several systems (mixed precision, no energy) ------
+struct BondedBatch {
+  BatchLaunch<BondedBatchArgs> bonded;
+  BatchLaunch<P14BatchArgs> p14;
+};
+BondedBatch *bonded_batch_create() { return new BondedBatch(); }
+void bonded_batch_destroy(BondedBatch *b) { delete b; }
+void bonded_batch_clear(BondedBatch &b) {
+  b.bonded.clear();
+  b.p14.clear();
+}
+void bonded_batch_add(BondedBatch &b, NbSystem &S, const float4 *x, int64_t *force, int stride) {
+  if (S.precision != 0) throw InvalidArgument("fused batches run the mixed-precision path");
+  auto *f = reinterpret_cast<unsigned long long *>(force);
+  const int nt = S.nbond + S.nub + S.nangle + S.ndihe + S.nimdihe;
+  if (nt > 0)
+    b.bonded.add(BondedBatchArgs{BondedArgs{x, S.bonds.get(), S.ureyb.get(), S.angles.get(),
+                                            S.dihe.get(), S.imdihe.get(), S.nbond, S.nub,
+                                            S.nangle, S.ndihe, S.nimdihe, stride},
+                                 f},
+                 S.part.bond_n);
+  if (S.nin14 + S.nex14 > 0)
+    b.p14.add(P14BatchArgs{P14Args{x, S.vdwtype.get(), S.in14.get(), S.ex14.get(), S.nin14,
+                                   S.nex14, stride, S.ntypes, S.e14fac},
+                           S.pc_f, S.vdwtab14.get(), f},
+              S.part.p14_n);
+}
+void bonded_batch_upload(BondedBatch &b, hipStream_t s) {
+  b.bonded.upload(s);
+  b.p14.upload(s);
+}
+void bonded_batch_launch(BondedBatch &b, hipStream_t s) {
+  if (b.bonded.nsys())
+    kb_bonded<<<b.bonded.blocks(), 256, 0, s>>>(b.bonded.d_args.get(), b.bonded.d_start.get(),
+                                                b.bonded.nsys());
+  if (b.p14.nsys())
+    kb_pairs14<<<b.p14.blocks(), 256, 0, s>>>(b.p14.d_args.get(), b.p14.d_start.get(),
+                                              b.p14.nsys());
   APO_LAUNCH_CHECK();
 }
 

@@ -23,6 +23,10 @@ void destroy_lt(LtSystem *L);
 void lt_initialize(LtSystem &L, hipStream_t caller);
 void lt_propagate(LtSystem &L, int nsteps, hipStream_t caller);
 void lt_propagate_batch(LtSystem **Ls, int n, int nsteps, hipStream_t caller);
+struct LtBatch;
+LtBatch *create_batch(LtSystem **Ls, int n, int lanes);
+void destroy_batch(LtBatch *B);
+void batch_propagate(LtBatch &B, int nsteps, hipStream_t caller);
 void lt_get_state(LtSystem &L, double *xyz, double *vel, double *dprev, double *grad,
                   double *thermo, int64_t *step);
 const float *lt_xyzq(LtSystem &L);
@@ -34,6 +38,10 @@ struct apo_nb_system {
 
 struct apo_lt_integrator {
   apo::LtSystem *lt;
+};
+
+struct apo_lt_batch {
+  apo::LtBatch *b;
 };
 
 namespace {
@@ -234,6 +242,33 @@ APO_API apo_status apo_lt_propagate_batch(apo_lt_integrator *const *lts, int32_t
     std::vector<apo::LtSystem *> v((size_t)(n > 0 ? n : 0));
     for (int i = 0; i < n; i++) v[i] = &get(lts[i]);
     apo::lt_propagate_batch(v.data(), n, nsteps, static_cast<hipStream_t>(stream));
+  });
+}
+
+APO_API apo_status apo_lt_batch_create(apo_lt_integrator *const *lts, int32_t n, int32_t lanes,
+                                       apo_lt_batch **out) {
+  return guard([&] {
+    if (!out || (n > 0 && !lts)) throw apo::InvalidArgument("null argument");
+    *out = nullptr;
+    std::vector<apo::LtSystem *> v((size_t)(n > 0 ? n : 0));
+    for (int i = 0; i < n; i++) v[i] = &get(lts[i]);
+    apo::LtBatch *B = apo::create_batch(v.data(), n, lanes);
+    *out = new apo_lt_batch{B};
+  });
+}
+
+APO_API apo_status apo_lt_batch_propagate(apo_lt_batch *b, int32_t nsteps, void *stream) {
+  return guard([&] {
+    if (!b || !b->b) throw apo::InvalidArgument("null batch handle");
+    apo::batch_propagate(*b->b, nsteps, static_cast<hipStream_t>(stream));
+  });
+}
+
+APO_API apo_status apo_lt_batch_destroy(apo_lt_batch *b) {
+  return guard([&] {
+    if (!b) return;
+    apo::destroy_batch(b->b);
+    delete b;
   });
 }
 

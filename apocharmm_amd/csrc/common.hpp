@@ -52,6 +52,22 @@ constexpr int kBondPart = 11;
 // Reciprocal partials: ewks, virial(6).
 constexpr int kRecipPart = 7;
 
+// ---- fused launches over several systems -------------------------------------
+// A batched kernel's grid is the concatenation of the per-system grids:
+// bstart[s] is the first block of system s (nsys + 1 entries).  The block
+// finds its system (a wave-uniform binary search), loads that system's
+// argument struct and runs the single-system body with its local block index.
+__device__ __forceinline__ int batch_system(const int *__restrict__ bstart, int nsys) {
+  const int b = blockIdx.x;
+  int lo = 0, hi = nsys;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (bstart[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 struct InvalidArgument : std::invalid_argument {
   using std::invalid_argument::invalid_argument;
 };
@@ -103,6 +119,32 @@ template <typename T> struct DevBuf {
   T *get() const { return p; }
 };
 
+// Host side of one fused launch: the per-system argument structs and block
+// ranges, uploaded once per (re)capture of the batch step.
+template <typename A> struct BatchLaunch {
+  std::vector<A> args;
+  std::vector<int> start{0};
+  DevBuf<A> d_args;
+  DevBuf<int> d_start;
+  size_t lds = 0;
+  void clear() {
+    args.clear();
+    start.assign(1, 0);
+    lds = 0;
+  }
+  void add(const A &a, int nblocks, size_t lds_bytes = 0) {
+    args.push_back(a);
+    start.push_back(start.back() + nblocks);
+    if (lds_bytes > lds) lds = lds_bytes;
+  }
+  int nsys() const { return (int)args.size(); }
+  int blocks() const { return start.back(); }
+  void upload(hipStream_t s) {
+    d_args.upload(args, s);
+    d_start.upload(start, s);
+  }
+};
+
 // Pair-kernel constants (VFSW + Ewald), CT = float (mixed) or double (FP64).
 // Switching constants follow src/CudaPMEDirectForce.cu:321-331.
 template <typename CT> struct PairConsts {
@@ -112,6 +154,19 @@ template <typename CT> struct PairConsts {
   CT kappa, kappa2, k2log2e, two_kappa_sqrtpi;
   CT erfc_p;              // t = 1/(1 + erfc_p * kappa r)
   CT erfc_c[9];           // erfc(x) ~ exp(-x^2) * sum_k c_k t^k
+};
+struct PairArgs {
+  const int4 *posq;
+  const int *types;
+  const int4 *items;
+  const uint32_t *ent;
+  const int *cand_j, *meta;
+  const float2 *vdwtab, *vdwsq;
+  int ntypes;
+  PairConsts<float> P;
+  float qfac;
+  unsigned long long *fsort;
+  int ss;
 };
 
 // Offsets of per-block partial regions in the partials buffer.
@@ -128,6 +183,36 @@ struct PartLayout {
 struct NbSystem;
 
 void launch_fill_i64(int64_t *p, size_t n, hipStream_t s);
+struct FillArgs {
+  int64_t *p;
+  size_t n;
+};
+void batch_fill_i64(BatchLaunch<FillArgs> &b, hipStream_t s);
+int fill_blocks(size_t n);
+
+// Argument structs of the fused (multi-system) launches of the step kernels.
+struct SortArgs {
+  const float4 *xyzq;
+  const int *vdwtype, *loc2glo, *meta;
+  double3 inv_len;
+  int4 *posq;
+  int *type_s;
+  int64_t *fsort;
+  int ss;
+};
+struct CellbbArgs {
+  const int4 *posq;
+  const int *loc2glo, *meta;
+  float3 unit;
+  float4 *bb;
+};
+struct FinalizeArgs {
+  const int64_t *fsort;
+  int ss;
+  const int *loc2glo, *meta;
+  int64_t *force;
+  int stride;
+};
 void launch_fill_f64(double *p, size_t n, hipStream_t s);
 void launch_fill_i32(int *p, size_t n, hipStream_t s);
 void launch_scan(const int *in, int *out, const int *n_dev, int n_max,

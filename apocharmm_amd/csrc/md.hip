@@ -16,6 +16,7 @@
 // so the integrator costs one pass over the state instead of the reference's
 // five kernels and two device copies.  Steps without a rebuild or centre of
 // mass removal replay a hipGraph of that sequence.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -27,6 +28,31 @@ namespace apo {
 void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int stride,
                 double *energy, double *virial, hipStream_t s);
 void list_export_check(NbSystem &S, hipStream_t s);
+
+// fused (multi-system) launches of the step kernels, one per kernel type
+void batch_add_sort(BatchLaunch<SortArgs> &b, NbSystem &S, const float4 *x);
+void batch_sort(BatchLaunch<SortArgs> &b, hipStream_t s);
+void batch_add_cellbb(BatchLaunch<CellbbArgs> &b, NbSystem &S);
+void batch_cellbb(BatchLaunch<CellbbArgs> &b, hipStream_t s);
+void batch_add_pair(BatchLaunch<PairArgs> &b, NbSystem &S, int blocks);
+void batch_pair(BatchLaunch<PairArgs> &b, bool tab_lds, hipStream_t s);
+bool pair_tab_lds(const NbSystem &S);
+void batch_add_finalize(BatchLaunch<FinalizeArgs> &b, NbSystem &S, int64_t *force, int stride);
+void batch_finalize(BatchLaunch<FinalizeArgs> &b, hipStream_t s);
+struct PmeBatch;
+PmeBatch *pme_batch_create();
+void pme_batch_destroy(PmeBatch *b);
+void pme_batch_clear(PmeBatch &b);
+void pme_batch_add(PmeBatch &b, NbSystem &S, int64_t *force, int stride);
+void pme_batch_upload(PmeBatch &b, hipStream_t s);
+void pme_batch_launch(PmeBatch &b, hipStream_t s);
+struct BondedBatch;
+BondedBatch *bonded_batch_create();
+void bonded_batch_destroy(BondedBatch *b);
+void bonded_batch_clear(BondedBatch &b);
+void bonded_batch_add(BondedBatch &b, NbSystem &S, const float4 *x, int64_t *force, int stride);
+void bonded_batch_upload(BondedBatch &b, hipStream_t s);
+void bonded_batch_launch(BondedBatch &b, hipStream_t s);
 
 namespace {
 
@@ -219,10 +245,10 @@ __device__ inline double wave_sum(double v) {
 }
 
 // One thread per water (first nwater threads) or free atom (the rest).
-__global__ __launch_bounds__(kStepBlock) void k_lt_step(LtState S, LtCoef C,
-                                                        const unsigned long long *__restrict__ stepc,
-                                                        double2 *__restrict__ part) {
-  const int u = blockIdx.x * kStepBlock + threadIdx.x;
+__device__ __forceinline__ void lt_step_body(int bid, const LtState &S, const LtCoef &C,
+                                             const unsigned long long *__restrict__ stepc,
+                                             double2 *__restrict__ part) {
+  const int u = bid * kStepBlock + threadIdx.x;
   const unsigned long long step = stepc[0];
   double ke0 = 0.0, ke1 = 0.0;
   if (u < S.nwater) {
@@ -244,16 +270,34 @@ __global__ __launch_bounds__(kStepBlock) void k_lt_step(LtState S, LtCoef C,
   }
   ke0 = wave_sum(ke0);
   ke1 = wave_sum(ke1);
-  if (threadIdx.x == 0) part[blockIdx.x] = make_double2(ke0, ke1);
+  if (threadIdx.x == 0) part[bid] = make_double2(ke0, ke1);
 }
+__global__ __launch_bounds__(kStepBlock) void k_lt_step(LtState S, LtCoef C,
+                                                        const unsigned long long *__restrict__ stepc,
+                                                        double2 *__restrict__ part) {
+  lt_step_body(blockIdx.x, S, C, stepc, part);
+}
+struct LtStepArgs {
+  LtState S;
+  LtCoef C;
+  const unsigned long long *stepc;
+  double2 *part;
+};
+__global__ __launch_bounds__(kStepBlock) void kb_lt_step(const LtStepArgs *__restrict__ tab,
+                                                         const int *__restrict__ bstart, int nsys) {
+  const int s = batch_system(bstart, nsys);
+  const LtStepArgs &a = tab[s];
+  lt_step_body(blockIdx.x - bstart[s], a.S, a.C, a.stepc, a.part);
+}
+
 
 // Fixed-order kinetic energy sum, temperatures and their running means
 // (src/CudaLangevinThermostatIntegrator.cu:684-701), step counter + 1.
 // thermo: [0] ke (three-estimate mean) [1] ke (on-step) [2] T [3] T on-step
 //         [4] <T> [5] <T on-step>; counters: [0] step [1] averaging window.
-__global__ __launch_bounds__(256) void k_lt_finalize(const double2 *__restrict__ part, int npart,
-                                                     double ndof, double *__restrict__ thermo,
-                                                     unsigned long long *__restrict__ counters) {
+__device__ __forceinline__ void lt_finalize_body(const double2 *__restrict__ part, int npart,
+                                                 double ndof, double *__restrict__ thermo,
+                                                 unsigned long long *__restrict__ counters) {
   __shared__ double s0[256], s1[256];
   double a = 0.0, b = 0.0;
   for (int k = threadIdx.x; k < npart; k += 256) {
@@ -283,6 +327,24 @@ __global__ __launch_bounds__(256) void k_lt_finalize(const double2 *__restrict__
     counters[1] += 1;
   }
 }
+__global__ __launch_bounds__(256) void k_lt_finalize(const double2 *__restrict__ part, int npart,
+                                                     double ndof, double *__restrict__ thermo,
+                                                     unsigned long long *__restrict__ counters) {
+  lt_finalize_body(part, npart, ndof, thermo, counters);
+}
+struct LtFinArgs {
+  const double2 *part;
+  int npart;
+  double ndof;
+  double *thermo;
+  unsigned long long *counters;
+};
+// one block per system
+__global__ __launch_bounds__(256) void kb_lt_finalize(const LtFinArgs *__restrict__ tab) {
+  const LtFinArgs &a = tab[blockIdx.x];
+  lt_finalize_body(a.part, a.npart, a.ndof, a.thermo, a.counters);
+}
+
 
 // ---- initialisation (src/CudaLangevinThermostatIntegrator.cu:206-258) ------
 // Constrain x against itself, fold the correction into v, refresh xyzq.
@@ -795,44 +857,255 @@ void lt_propagate(LtSystem &L, int nsteps, hipStream_t caller) {
   on_stream(L, caller, [&](hipStream_t s) { L.propagate(nsteps, s); });
 }
 
-// Many independent systems advanced together: step k of every system is
-// queued before step k+1 of any, each system on its own stream (so their
-// kernels overlap on the device), with no host synchronisation between
-// list rebuilds.  Per system the work and its order are exactly those of
-// lt_propagate, so every system's trajectory is bitwise the same as alone.
-void lt_propagate_batch(LtSystem **Ls, int n, int nsteps, hipStream_t caller) {
-  if (n < 0 || nsteps < 0) throw InvalidArgument("negative count");
+// Many independent systems advanced together (no counterpart in the
+// reference, which runs one CharmmContext per process).  Each batch step is
+// one graph launch: the systems' steps are `lanes` independent chains of
+// branches (systems i, i + lanes, ... back to back in chain i mod lanes), so
+// the chains overlap on the GPU's hardware queues and no host synchronisation
+// or per-step event happens between list rebuilds.  Per system the kernels
+// and their order are those of lt_propagate, so every trajectory is bitwise
+// the one the system gets alone.
+struct LtBatch {
+  std::vector<LtSystem *> sys;
+  std::vector<bool> was_single;
+  int lanes = 4;
+  hipStream_t m = nullptr;
+  std::vector<hipStream_t> lane;
+  hipEvent_t fork_ev = nullptr, m_ev = nullptr;
+  std::vector<hipEvent_t> lane_ev;
+  hipGraphExec_t ex[2] = {nullptr, nullptr};
+  std::vector<unsigned long long> gens[2];
+  // fused step: one launch per kernel type for all systems
+  BatchLaunch<FillArgs> fill;
+  BatchLaunch<SortArgs> sort;
+  BatchLaunch<PairArgs> pair;
+  BatchLaunch<CellbbArgs> cellbb;
+  BatchLaunch<FinalizeArgs> fin;
+  BatchLaunch<LtStepArgs> lstep;
+  BatchLaunch<LtFinArgs> lfin;
+  PmeBatch *pme = nullptr;
+  BondedBatch *bond = nullptr;
+  bool tab_lds = true;
+  std::vector<unsigned long long> tab_gens;
+
+  ~LtBatch() {
+    for (hipGraphExec_t g : ex)
+      if (g) (void)hipGraphExecDestroy(g);
+    for (hipEvent_t e : lane_ev) (void)hipEventDestroy(e);
+    for (hipStream_t st : lane) (void)hipStreamDestroy(st);
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    if (m_ev) (void)hipEventDestroy(m_ev);
+    if (m) (void)hipStreamDestroy(m);
+    if (pme) pme_batch_destroy(pme);
+    if (bond) bonded_batch_destroy(bond);
+    for (size_t i = 0; i < sys.size(); i++)
+      if (!was_single[i]) {
+        sys[i]->nb->single_stream = false;
+        sys[i]->nb->generation++;
+      }
+  }
+
+  // per-system argument tables of the fused launches (pointers and launch
+  // constants move only with a system's generation)
+  void build_tables() {
+    fill.clear();
+    sort.clear();
+    pair.clear();
+    cellbb.clear();
+    fin.clear();
+    lstep.clear();
+    lfin.clear();
+    pme_batch_clear(*pme);
+    bonded_batch_clear(*bond);
+    tab_lds = true;
+    double ntot = 0;
+    for (LtSystem *L : sys) ntot += L->n;
+    const int ncu = sys[0]->nb->ncu;
+    for (LtSystem *L : sys) {
+      NbSystem &S = *L->nb;
+      const float4 *x = L->xyzq.get();
+      int64_t *g = L->grad.get();
+      fill.add(FillArgs{g, 3 * (size_t)L->gstride}, fill_blocks(3 * (size_t)L->gstride));
+      batch_add_sort(sort, S, x);
+      const int pb = std::max(8, (int)std::lround(kPairBlocksPerCu * ncu * (L->n / ntot)));
+      batch_add_pair(pair, S, pb);
+      tab_lds = tab_lds && pair_tab_lds(S);
+      batch_add_cellbb(cellbb, S);
+      pme_batch_add(*pme, S, g, L->gstride);
+      bonded_batch_add(*bond, S, x, g, L->gstride);
+      batch_add_finalize(fin, S, g, L->gstride);
+      lstep.add(LtStepArgs{L->state(), L->C, L->counters.get(), L->ke_part.get()}, L->nunit_blocks);
+      lfin.add(LtFinArgs{L->ke_part.get(), L->nunit_blocks, L->ndof, L->thermo.get(),
+                         L->counters.get()},
+               1);
+    }
+    fill.upload(m);
+    sort.upload(m);
+    pair.upload(m);
+    cellbb.upload(m);
+    fin.upload(m);
+    lstep.upload(m);
+    lfin.upload(m);
+    pme_batch_upload(*pme, m);
+    bonded_batch_upload(*bond, m);
+    APO_HIP(hipStreamSynchronize(m));
+    tab_gens.resize(sys.size());
+    for (size_t i = 0; i < sys.size(); i++) tab_gens[i] = sys[i]->nb->generation;
+  }
+
+  // force path of every system + integrator update: 16 launches in all
+  void fused_step(hipStream_t s) {
+    batch_fill_i64(fill, s);
+    batch_sort(sort, s);
+    batch_pair(pair, tab_lds, s);
+    batch_cellbb(cellbb, s);
+    pme_batch_launch(*pme, s);
+    bonded_batch_launch(*bond, s);
+    batch_finalize(fin, s);
+    kb_lt_step<<<lstep.blocks(), kStepBlock, 0, s>>>(lstep.d_args.get(), lstep.d_start.get(),
+                                                     lstep.nsys());
+    kb_lt_finalize<<<lfin.nsys(), 256, 0, s>>>(lfin.d_args.get());
+    APO_LAUNCH_CHECK();
+  }
+
+  void capture(int rb) {
+    bool stale_tab = tab_gens.size() != sys.size();
+    for (size_t i = 0; i < sys.size() && !stale_tab; i++)
+      stale_tab = tab_gens[i] != sys[i]->nb->generation;
+    if (stale_tab) build_tables();
+    if (ex[rb]) {
+      APO_HIP(hipGraphExecDestroy(ex[rb]));
+      ex[rb] = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    APO_HIP(hipStreamBeginCapture(m, hipStreamCaptureModeThreadLocal));
+    try {
+      if (rb) {
+        // image centering and list rebuild of every system, in parallel lanes
+        APO_HIP(hipEventRecord(fork_ev, m));
+        for (int l = 0; l < lanes; l++) {
+          APO_HIP(hipStreamWaitEvent(lane[l], fork_ev, 0));
+          for (size_t i = l; i < sys.size(); i += lanes) {
+            LtSystem &L = *sys[i];
+            L.center(lane[l]);
+            nblist_build(*L.nb, L.xyzq.get(), lane[l], false);
+          }
+          APO_HIP(hipEventRecord(lane_ev[l], lane[l]));
+          APO_HIP(hipStreamWaitEvent(m, lane_ev[l], 0));
+        }
+      }
+      fused_step(m);
+    } catch (...) {
+      (void)hipStreamEndCapture(m, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    APO_HIP(hipStreamEndCapture(m, &g));
+    hipError_t e = hipGraphInstantiate(&ex[rb], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    APO_HIP(e);
+    gens[rb].resize(sys.size());
+    for (size_t i = 0; i < sys.size(); i++) gens[rb][i] = sys[i]->nb->generation;
+  }
+  void propagate(int nsteps, hipStream_t caller) {
+    const int n = (int)sys.size();
+    APO_HIP(hipEventRecord(m_ev, caller));
+    APO_HIP(hipStreamWaitEvent(m, m_ev, 0));
+    for (LtSystem *L : sys) {
+      APO_HIP(hipStreamWaitEvent(L->st, m_ev, 0));
+      L->begin(L->st);
+      APO_HIP(hipEventRecord(L->ev_out, L->st));
+      APO_HIP(hipStreamWaitEvent(m, L->ev_out, 0));
+    }
+    bool whole = true;
+    for (LtSystem *L : sys)
+      whole = whole && L->graph_usable() && L->nbfreq == sys[0]->nbfreq &&
+              L->steps_since_rebuild == sys[0]->steps_since_rebuild;
+    for (int k = 0; k < nsteps; k++) {
+      if (!whole) {  // systems out of phase: per-system steps on the batch stream
+        for (LtSystem *L : sys) L->one_step(m);
+        continue;
+      }
+      const int rb = sys[0]->steps_since_rebuild % sys[0]->nbfreq == 0 ? 1 : 0;
+      for (LtSystem *L : sys)
+        if (L->comfreq > 0 && L->step % (unsigned long long)L->comfreq == 0) L->remove_com(m);
+      if (rb)
+        for (LtSystem *L : sys) list_export_check(*L->nb, m);
+      bool stale = !ex[rb];
+      for (int i = 0; i < n && !stale; i++) stale = gens[rb][i] != sys[i]->nb->generation;
+      if (stale) capture(rb);
+      APO_HIP(hipGraphLaunch(ex[rb], m));
+      for (LtSystem *L : sys) {
+        L->steps_since_rebuild++;
+        L->step++;
+      }
+    }
+    APO_HIP(hipEventRecord(m_ev, m));
+    for (LtSystem *L : sys) APO_HIP(hipStreamWaitEvent(L->st, m_ev, 0));
+    APO_HIP(hipStreamWaitEvent(caller, m_ev, 0));
+  }
+};
+
+LtBatch *create_batch(LtSystem **Ls, int n, int lanes) {
+  if (n <= 0) throw InvalidArgument("a batch needs at least one system");
   for (int i = 0; i < n; i++)
     if (!Ls[i]) throw InvalidArgument("null integrator in batch");
   for (int i = 0; i < n; i++)
     for (int j = 0; j < i; j++)
       if (Ls[i] == Ls[j] || Ls[i]->nb == Ls[j]->nb)
         throw InvalidArgument("a batch must not hold one system twice");
-  for (int i = 0; i < n; i++) {
-    APO_HIP(hipEventRecord(Ls[i]->ev_in, caller));
-    APO_HIP(hipStreamWaitEvent(Ls[i]->st, Ls[i]->ev_in, 0));
-  }
-  // one stream per system: no cross-stream fork/join inside a system's step
-  std::vector<bool> mode(n);
-  for (int i = 0; i < n; i++) {
-    mode[i] = Ls[i]->nb->single_stream;
-    if (!mode[i]) {
-      Ls[i]->nb->single_stream = true;
-      Ls[i]->nb->generation++;  // captured steps have the other shape
+  auto *B = new LtBatch();
+  try {
+    B->sys.assign(Ls, Ls + n);
+    B->lanes = std::max(1, std::min(lanes > 0 ? lanes : 4, n));
+    // one stream per system inside a step: no fork/join within a system
+    for (LtSystem *L : B->sys) {
+      B->was_single.push_back(L->nb->single_stream);
+      if (!L->nb->single_stream) {
+        L->nb->single_stream = true;
+        L->nb->generation++;
+      }
     }
-  }
-  for (int i = 0; i < n; i++) Ls[i]->begin(Ls[i]->st);
-  for (int k = 0; k < nsteps; k++)
-    for (int i = 0; i < n; i++) Ls[i]->one_step(Ls[i]->st);
-  for (int i = 0; i < n; i++)
-    if (!mode[i]) {
-      Ls[i]->nb->single_stream = false;
-      Ls[i]->nb->generation++;
+    APO_HIP(hipStreamCreateWithFlags(&B->m, hipStreamNonBlocking));
+    APO_HIP(hipEventCreateWithFlags(&B->fork_ev, hipEventDisableTiming));
+    APO_HIP(hipEventCreateWithFlags(&B->m_ev, hipEventDisableTiming));
+    B->lane.resize(B->lanes, nullptr);
+    B->lane_ev.resize(B->lanes, nullptr);
+    for (int l = 0; l < B->lanes; l++) {
+      APO_HIP(hipStreamCreateWithFlags(&B->lane[l], hipStreamNonBlocking));
+      APO_HIP(hipEventCreateWithFlags(&B->lane_ev[l], hipEventDisableTiming));
     }
-  for (int i = 0; i < n; i++) {
-    APO_HIP(hipEventRecord(Ls[i]->ev_out, Ls[i]->st));
-    APO_HIP(hipStreamWaitEvent(caller, Ls[i]->ev_out, 0));
+    for (LtSystem *L : B->sys)
+      if (L->nb->precision != 0)
+        throw InvalidArgument("a batch runs the mixed-precision force path only");
+    B->pme = pme_batch_create();
+    B->bond = bonded_batch_create();
+  } catch (...) {
+    delete B;
+    throw;
   }
+  return B;
+}
+
+void destroy_batch(LtBatch *B) { delete B; }
+
+void batch_propagate(LtBatch &B, int nsteps, hipStream_t caller) {
+  if (nsteps < 0) throw InvalidArgument("nsteps must be >= 0");
+  B.propagate(nsteps, caller);
+}
+
+void lt_propagate_batch(LtSystem **Ls, int n, int nsteps, hipStream_t caller) {
+  if (n == 0) return;
+  LtBatch *B = create_batch(Ls, n, 0);
+  try {
+    batch_propagate(*B, nsteps, caller);
+  } catch (...) {
+    delete B;
+    throw;
+  }
+  // the batch's streams may still hold queued work: release after it drains
+  APO_HIP(hipStreamSynchronize(B->m));
+  delete B;
 }
 
 void lt_get_state(LtSystem &L, double *xyz, double *vel, double *dprev, double *grad,

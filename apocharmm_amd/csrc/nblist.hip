@@ -150,11 +150,14 @@ __global__ __launch_bounds__(256) void k_colsort(const unsigned long long *__res
 // Per-step gather of xyzq into cell order as 32-bit fixed-point coordinates
 // (setSortedCoords, src/CudaPMEDirectForce.cu:1389-1400), fused with the
 // clear of the sorted force buffer.
-__global__ void k_sort_coords(const float4 *__restrict__ xyzq, const int *__restrict__ vdwtype,
-                              const int *__restrict__ loc2glo, const int *__restrict__ meta,
-                              double3 inv_len, float qscale, int4 *__restrict__ posq,
-                              int *__restrict__ type_s, int64_t *__restrict__ fsort, int ss) {
-  int loc = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void sort_coords_body(int bid, const float4 *__restrict__ xyzq,
+                                                 const int *__restrict__ vdwtype,
+                                                 const int *__restrict__ loc2glo,
+                                                 const int *__restrict__ meta, double3 inv_len,
+                                                 float qscale, int4 *__restrict__ posq,
+                                                 int *__restrict__ type_s,
+                                                 int64_t *__restrict__ fsort, int ss) {
+  int loc = bid * blockDim.x + threadIdx.x;
   if (loc >= meta[0] * kCell) return;
   int g = loc2glo[loc];
   int4 o = make_int4(0, 0, 0, 0);
@@ -173,14 +176,28 @@ __global__ void k_sort_coords(const float4 *__restrict__ xyzq, const int *__rest
   fsort[ss + loc] = 0;
   fsort[2 * (size_t)ss + loc] = 0;
 }
+__global__ void k_sort_coords(const float4 *__restrict__ xyzq, const int *__restrict__ vdwtype,
+                              const int *__restrict__ loc2glo, const int *__restrict__ meta,
+                              double3 inv_len, float qscale, int4 *__restrict__ posq,
+                              int *__restrict__ type_s, int64_t *__restrict__ fsort, int ss) {
+  sort_coords_body(blockIdx.x, xyzq, vdwtype, loc2glo, meta, inv_len, qscale, posq, type_s, fsort,
+                   ss);
+}
+__global__ __launch_bounds__(256) void kb_sort_coords(const SortArgs *__restrict__ tab,
+                                                      const int *__restrict__ bstart, int nsys) {
+  const int s = batch_system(bstart, nsys);
+  const SortArgs a = tab[s];
+  sort_coords_body(blockIdx.x - bstart[s], a.xyzq, a.vdwtype, a.loc2glo, a.meta, a.inv_len, 1.0f,
+                   a.posq, a.type_s, a.fsort, a.ss);
+}
 
 // 5. cell bounding boxes from the fixed-point coordinates (two cells per wave).
-__global__ __launch_bounds__(64) void k_cellbb(const int4 *__restrict__ posq,
-                                               const int *__restrict__ loc2glo,
-                                               const int *__restrict__ meta, float3 unit,
-                                               float4 *__restrict__ bb) {
+__device__ __forceinline__ void cellbb_body(int bid, const int4 *__restrict__ posq,
+                                            const int *__restrict__ loc2glo,
+                                            const int *__restrict__ meta, float3 unit,
+                                            float4 *__restrict__ bb) {
   const int lane = threadIdx.x;
-  const int cell = blockIdx.x * 2 + (lane >> 5);
+  const int cell = bid * 2 + (lane >> 5);
   const bool cv = cell < meta[0];
   const int loc = cell * kCell + (lane & 31);
   bool valid = cv && loc2glo[loc] >= 0;
@@ -205,6 +222,18 @@ __global__ __launch_bounds__(64) void k_cellbb(const int4 *__restrict__ posq,
     bb[2 * cell] = make_float4(0.5f * (mnx + mxx), 0.5f * (mny + mxy), 0.5f * (mnz + mxz), 0.f);
     bb[2 * cell + 1] = make_float4(0.5f * (mxx - mnx), 0.5f * (mxy - mny), 0.5f * (mxz - mnz), 0.f);
   }
+}
+__global__ __launch_bounds__(64) void k_cellbb(const int4 *__restrict__ posq,
+                                               const int *__restrict__ loc2glo,
+                                               const int *__restrict__ meta, float3 unit,
+                                               float4 *__restrict__ bb) {
+  cellbb_body(blockIdx.x, posq, loc2glo, meta, unit, bb);
+}
+__global__ __launch_bounds__(64) void kb_cellbb(const CellbbArgs *__restrict__ tab,
+                                                const int *__restrict__ bstart, int nsys) {
+  const int s = batch_system(bstart, nsys);
+  const CellbbArgs a = tab[s];
+  cellbb_body(blockIdx.x - bstart[s], a.posq, a.loc2glo, a.meta, a.unit, a.bb);
 }
 
 __device__ __forceinline__ float bb_gap2(float4 ci, float4 hi, float4 cj, float4 hj, float3 L) {
@@ -591,6 +620,31 @@ void nblist_stats(NbSystem &S, int64_t *out6, hipStream_t s) {
   out6[3] = (int64_t)pairs;
   out6[4] = S.ncol;
   out6[5] = hmeta[4];
+}
+
+// ---- fused launches over several systems ------------------------------------
+void batch_add_sort(BatchLaunch<SortArgs> &b, NbSystem &S, const float4 *x) {
+  SortArgs a{x, S.vdwtype.get(), S.loc2glo.get(), S.meta.get(),
+             make_double3(1.0 / S.box[0], 1.0 / S.box[1], 1.0 / S.box[2]), S.posq.get(),
+             S.type_s.get(), S.fsort.get(), S.ssort};
+  b.add(a, nblk(S.ssort, 256));
+}
+void batch_sort(BatchLaunch<SortArgs> &b, hipStream_t s) {
+  if (!b.blocks()) return;
+  kb_sort_coords<<<b.blocks(), 256, 0, s>>>(b.d_args.get(), b.d_start.get(), b.nsys());
+  APO_LAUNCH_CHECK();
+}
+void batch_add_cellbb(BatchLaunch<CellbbArgs> &b, NbSystem &S) {
+  CellbbArgs a{S.posq.get(), S.loc2glo.get(), S.meta.get(),
+               make_float3((float)(S.box[0] / 4294967296.0), (float)(S.box[1] / 4294967296.0),
+                           (float)(S.box[2] / 4294967296.0)),
+               S.cell_bb.get()};
+  b.add(a, nblk(S.ncell_cap, 2));
+}
+void batch_cellbb(BatchLaunch<CellbbArgs> &b, hipStream_t s) {
+  if (!b.blocks()) return;
+  kb_cellbb<<<b.blocks(), 64, 0, s>>>(b.d_args.get(), b.d_start.get(), b.nsys());
+  APO_LAUNCH_CHECK();
 }
 
 }  // namespace apo

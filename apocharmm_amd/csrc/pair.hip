@@ -91,11 +91,12 @@ constexpr int kMaxTypesLds = 32;  // square vdW table in LDS up to 32 types
 // TAB_LDS: vdW table square (ntypes x ntypes) in LDS; otherwise the
 // triangular table is read from global memory.
 template <typename CT, bool CALC_E, bool CALC_V, bool TAB_LDS>
-__global__ __launch_bounds__(64) void k_pair(
-    const int4 *__restrict__ posq, const int *__restrict__ types, const int4 *__restrict__ items,
-    const uint32_t *__restrict__ ent, const int *__restrict__ cand_j, const int *__restrict__ meta,
+__device__ __forceinline__ void pair_body(
+    int bid, int nb, const int4 *__restrict__ posq, const int *__restrict__ types,
+    const int4 *__restrict__ items, const uint32_t *__restrict__ ent,
+    const int *__restrict__ cand_j, const int *__restrict__ meta,
     const typename Vec2<CT>::T *__restrict__ vdwtab, const typename Vec2<CT>::T *__restrict__ vdwsq,
-    int ntypes, const PairConsts<CT> P, CT qfac, unsigned long long *__restrict__ fsort, int ss,
+    int ntypes, const PairConsts<CT> &P, CT qfac, unsigned long long *__restrict__ fsort, int ss,
     double *__restrict__ part) {
   using T2 = typename Vec2<CT>::T;
   extern __shared__ char smem_pair[];
@@ -110,7 +111,7 @@ __global__ __launch_bounds__(64) void k_pair(
 #pragma unroll
   for (int k = 0; k < kPairPart; k++) acc[k] = 0.0;
 
-  for (int item = blockIdx.x; item < nitem; item += gridDim.x) {
+  for (int item = bid; item < nitem; item += nb) {
     const int4 it = items[item];
     const int ibase = it.x * kGroup;
     // i-data to SGPRs: lane k < kGroup loads slot ibase + k
@@ -251,8 +252,32 @@ __global__ __launch_bounds__(64) void k_pair(
     }
     if (lane == 0)
 #pragma unroll
-      for (int k = 0; k < kPairPart; k++) part[blockIdx.x * kPairPart + k] = acc[k];
+      for (int k = 0; k < kPairPart; k++) part[bid * kPairPart + k] = acc[k];
   }
+}
+
+template <typename CT, bool CALC_E, bool CALC_V, bool TAB_LDS>
+__global__ __launch_bounds__(64) void k_pair(
+    const int4 *__restrict__ posq, const int *__restrict__ types, const int4 *__restrict__ items,
+    const uint32_t *__restrict__ ent, const int *__restrict__ cand_j, const int *__restrict__ meta,
+    const typename Vec2<CT>::T *__restrict__ vdwtab, const typename Vec2<CT>::T *__restrict__ vdwsq,
+    int ntypes, const PairConsts<CT> P, CT qfac, unsigned long long *__restrict__ fsort, int ss,
+    double *__restrict__ part) {
+  pair_body<CT, CALC_E, CALC_V, TAB_LDS>(blockIdx.x, gridDim.x, posq, types, items, ent, cand_j,
+                                         meta, vdwtab, vdwsq, ntypes, P, qfac, fsort, ss, part);
+}
+
+// Fused launch over several systems: mixed precision, no energy/virial (the
+// MD step).
+template <bool TAB_LDS>
+__global__ __launch_bounds__(64) void kb_pair(const PairArgs *__restrict__ tab,
+                                              const int *__restrict__ bstart, int nsys) {
+  const int s = batch_system(bstart, nsys);
+  const PairArgs &a = tab[s];
+  pair_body<float, false, false, TAB_LDS>(blockIdx.x - bstart[s], bstart[s + 1] - bstart[s],
+                                          a.posq, a.types, a.items, a.ent, a.cand_j, a.meta,
+                                          a.vdwtab, a.vdwsq, a.ntypes, a.P, a.qfac, a.fsort, a.ss,
+                                          nullptr);
 }
 
 template <typename CT>
@@ -295,10 +320,11 @@ void direct_launch(NbSystem &S, bool E, bool V, hipStream_t s) {
 
 // Sorted -> global order (addSortedForce, src/CudaPMEDirectForce.cu:1375-1386).
 // Each atom has exactly one sorted slot, so this is a plain add.
-__global__ void k_direct_finalize(const int64_t *__restrict__ fsort, int ss,
-                                  const int *__restrict__ loc2glo, const int *__restrict__ meta,
-                                  int64_t *__restrict__ force, int stride) {
-  int loc = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void finalize_body(int bid, const int64_t *__restrict__ fsort, int ss,
+                                              const int *__restrict__ loc2glo,
+                                              const int *__restrict__ meta,
+                                              int64_t *__restrict__ force, int stride) {
+  int loc = bid * blockDim.x + threadIdx.x;
   if (loc >= meta[0] * kCell) return;
   int g = loc2glo[loc];
   if (g < 0) return;
@@ -306,10 +332,48 @@ __global__ void k_direct_finalize(const int64_t *__restrict__ fsort, int ss,
   force[(size_t)stride + g] += fsort[ss + loc];
   force[2 * (size_t)stride + g] += fsort[2 * (size_t)ss + loc];
 }
+__global__ void k_direct_finalize(const int64_t *__restrict__ fsort, int ss,
+                                  const int *__restrict__ loc2glo, const int *__restrict__ meta,
+                                  int64_t *__restrict__ force, int stride) {
+  finalize_body(blockIdx.x, fsort, ss, loc2glo, meta, force, stride);
+}
+__global__ __launch_bounds__(256) void kb_direct_finalize(const FinalizeArgs *__restrict__ tab,
+                                                          const int *__restrict__ bstart, int nsys) {
+  const int s = batch_system(bstart, nsys);
+  const FinalizeArgs a = tab[s];
+  finalize_body(blockIdx.x - bstart[s], a.fsort, a.ss, a.loc2glo, a.meta, a.force, a.stride);
+}
 
 void direct_finalize(NbSystem &S, int64_t *force, int stride, hipStream_t s) {
   k_direct_finalize<<<(S.ssort + 255) / 256, 256, 0, s>>>(S.fsort.get(), S.ssort, S.loc2glo.get(),
                                                           S.meta.get(), force, stride);
+  APO_LAUNCH_CHECK();
+}
+
+// ---- fused launches over several systems ------------------------------------
+void batch_add_pair(BatchLaunch<PairArgs> &b, NbSystem &S, int blocks) {
+  if (S.precision != 0) throw InvalidArgument("fused batches run the mixed-precision path");
+  PairArgs a{S.posq.get(), S.type_s.get(), S.items.get(), S.ent.get(), S.cand_j.get(),
+             S.meta.get(), S.vdwtab.get(), S.vdwsq.get(), S.ntypes, S.pc_f, (float)kCcelec,
+             reinterpret_cast<unsigned long long *>(S.fsort.get()), S.ssort};
+  b.add(a, blocks, (size_t)S.ntypes * S.ntypes * sizeof(float2));
+}
+void batch_pair(BatchLaunch<PairArgs> &b, bool tab_lds, hipStream_t s) {
+  if (!b.blocks()) return;
+  if (tab_lds)
+    kb_pair<true><<<b.blocks(), 64, b.lds, s>>>(b.d_args.get(), b.d_start.get(), b.nsys());
+  else
+    kb_pair<false><<<b.blocks(), 64, 0, s>>>(b.d_args.get(), b.d_start.get(), b.nsys());
+  APO_LAUNCH_CHECK();
+}
+bool pair_tab_lds(const NbSystem &S) { return S.ntypes <= kMaxTypesLds; }
+void batch_add_finalize(BatchLaunch<FinalizeArgs> &b, NbSystem &S, int64_t *force, int stride) {
+  FinalizeArgs a{S.fsort.get(), S.ssort, S.loc2glo.get(), S.meta.get(), force, stride};
+  b.add(a, (S.ssort + 255) / 256);
+}
+void batch_finalize(BatchLaunch<FinalizeArgs> &b, hipStream_t s) {
+  if (!b.blocks()) return;
+  kb_direct_finalize<<<b.blocks(), 256, 0, s>>>(b.d_args.get(), b.d_start.get(), b.nsys());
   APO_LAUNCH_CHECK();
 }
 

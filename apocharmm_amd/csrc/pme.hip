@@ -214,16 +214,15 @@ __device__ __forceinline__ void frac_fixed(int ufix, int K, int &ix, double &w) 
 }
 
 template <typename CT>
-__global__ __launch_bounds__(256) void k_spread_brick(const int4 *__restrict__ posq,
-                                                      const float4 *__restrict__ bb,
-                                                      const int *__restrict__ meta, BrickGeom g,
-                                                      double qunscale, CT *__restrict__ grid) {
+__device__ __forceinline__ void spread_body(int bid, const int4 *__restrict__ posq,
+                                            const float4 *__restrict__ bb,
+                                            const int *__restrict__ meta, const BrickGeom &g,
+                                            double qunscale, CT *__restrict__ grid) {
   __shared__ unsigned long long brick[kBx * kBy * kBz];
   __shared__ int cand[256];
   __shared__ int ncand;
   const int tid = threadIdx.x;
-  const int bxi = blockIdx.x % g.nbx, byi = (blockIdx.x / g.nbx) % g.nby,
-            bzi = blockIdx.x / (g.nbx * g.nby);
+  const int bxi = bid % g.nbx, byi = (bid / g.nbx) % g.nby, bzi = bid / (g.nbx * g.nby);
   const int x0 = bxi * kBx, y0 = byi * kBy, z0 = bzi * kBz;
   for (int k = tid; k < kBx * kBy * kBz; k += 256) brick[k] = 0ull;
   const int ncell = meta[0];
@@ -301,18 +300,25 @@ __global__ __launch_bounds__(256) void k_spread_brick(const int4 *__restrict__ p
           (CT)((double)(long long)brick[k] * (1.0 / kGridScale));
   }
 }
+template <typename CT>
+__global__ __launch_bounds__(256) void k_spread_brick(const int4 *__restrict__ posq,
+                                                      const float4 *__restrict__ bb,
+                                                      const int *__restrict__ meta, BrickGeom g,
+                                                      double qunscale, CT *__restrict__ grid) {
+  spread_body<CT>(blockIdx.x, posq, bb, meta, g, qunscale, grid);
+}
+
 
 // Gather in cell order: 16 lanes per atom; the gradient goes straight into
 // the global-order output with int64 atomics (three per atom), so the gather
 // can run while the pair kernel and the bonded terms write the same atoms.
 template <typename CT>
-__global__ __launch_bounds__(256) void k_gather_sorted(const int4 *__restrict__ posq,
-                                                       const int *__restrict__ meta,
-                                                       const int *__restrict__ loc2glo, GridGeom g,
-                                                       double qunscale, const CT *__restrict__ phi,
-                                                       unsigned long long *__restrict__ force,
-                                                       int stride) {
-  const int gt = blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void gather_body(int bid, const int4 *__restrict__ posq,
+                                            const int *__restrict__ meta,
+                                            const int *__restrict__ loc2glo, const GridGeom &g,
+                                            double qunscale, const CT *__restrict__ phi,
+                                            unsigned long long *__restrict__ force, int stride) {
+  const int gt = bid * 256 + threadIdx.x;
   const int loc = gt >> 4, row = gt & 15;
   const bool valid = loc < meta[0] * kCell;
   const int4 p = valid ? posq[loc] : make_int4(0, 0, 0, 0);
@@ -360,13 +366,23 @@ __global__ __launch_bounds__(256) void k_gather_sorted(const int4 *__restrict__ 
     }
   }
 }
+template <typename CT>
+__global__ __launch_bounds__(256) void k_gather_sorted(const int4 *__restrict__ posq,
+                                                       const int *__restrict__ meta,
+                                                       const int *__restrict__ loc2glo, GridGeom g,
+                                                       double qunscale, const CT *__restrict__ phi,
+                                                       unsigned long long *__restrict__ force,
+                                                       int stride) {
+  gather_body<CT>(blockIdx.x, posq, meta, loc2glo, g, qunscale, phi, force, stride);
+}
+
 
 // x forward: int64 rows -> complex half rows, two real rows per complex FFT.
 template <typename CT>
-__global__ __launch_bounds__(256) void k_fftx_fwd(const CT *__restrict__ grid, int nrows,
-                                                  FftPlan p, int rb,
-                                                  const typename Cx<CT>::T *__restrict__ twg,
-                                                  typename Cx<CT>::T *__restrict__ out) {
+__device__ __forceinline__ void fftx_fwd_body(int bid, const CT *__restrict__ grid, int nrows,
+                                              const FftPlan &p, int rb,
+                                              const typename Cx<CT>::T *__restrict__ twg,
+                                              typename Cx<CT>::T *__restrict__ out) {
   using C2 = typename Cx<CT>::T;
   extern __shared__ char smem[];
   const int n = p.n, H = n / 2 + 1;
@@ -374,7 +390,7 @@ __global__ __launch_bounds__(256) void k_fftx_fwd(const CT *__restrict__ grid, i
   C2 *a = tw + n;
   C2 *b = a + rb * n;
   for (int e = threadIdx.x; e < n; e += blockDim.x) tw[e] = twg[e];
-  const int r0 = blockIdx.x * rb;  // first row pair
+  const int r0 = bid * rb;  // first row pair
   for (int e = threadIdx.x; e < rb * n; e += blockDim.x) {
     const int r = e / n, x = e - r * n;
     const int ra = 2 * (r0 + r), rbw = ra + 1;
@@ -398,14 +414,22 @@ __global__ __launch_bounds__(256) void k_fftx_fwd(const CT *__restrict__ grid, i
     if (rbw < nrows) out[(size_t)rbw * H + k] = B;
   }
 }
+template <typename CT>
+__global__ __launch_bounds__(256) void k_fftx_fwd(const CT *__restrict__ grid, int nrows,
+                                                  FftPlan p, int rb,
+                                                  const typename Cx<CT>::T *__restrict__ twg,
+                                                  typename Cx<CT>::T *__restrict__ out) {
+  fftx_fwd_body<CT>(blockIdx.x, grid, nrows, p, rb, twg, out);
+}
+
 
 // x inverse: complex half rows -> real rows (C2R semantics: the imaginary
 // parts of the k = 0 and Nyquist bins are ignored).
 template <typename CT>
-__global__ __launch_bounds__(256) void k_fftx_inv(const typename Cx<CT>::T *__restrict__ in,
-                                                  int nrows, FftPlan p, int rb,
-                                                  const typename Cx<CT>::T *__restrict__ twg,
-                                                  CT *__restrict__ phi) {
+__device__ __forceinline__ void fftx_inv_body(int bid, const typename Cx<CT>::T *__restrict__ in,
+                                              int nrows, const FftPlan &p, int rb,
+                                              const typename Cx<CT>::T *__restrict__ twg,
+                                              CT *__restrict__ phi) {
   using C2 = typename Cx<CT>::T;
   extern __shared__ char smem[];
   const int n = p.n, H = n / 2 + 1;
@@ -413,7 +437,7 @@ __global__ __launch_bounds__(256) void k_fftx_inv(const typename Cx<CT>::T *__re
   C2 *a = tw + n;
   C2 *b = a + rb * n;
   for (int e = threadIdx.x; e < n; e += blockDim.x) tw[e] = twg[e];
-  const int r0 = blockIdx.x * rb;
+  const int r0 = bid * rb;
   for (int e = threadIdx.x; e < rb * n; e += blockDim.x) {
     const int r = e / n, k = e - r * n;
     const int ra = 2 * (r0 + r), rbw = ra + 1;
@@ -444,12 +468,20 @@ __global__ __launch_bounds__(256) void k_fftx_inv(const typename Cx<CT>::T *__re
     if (rbw < nrows) phi[(size_t)rbw * n + x] = res[e].y;
   }
 }
+template <typename CT>
+__global__ __launch_bounds__(256) void k_fftx_inv(const typename Cx<CT>::T *__restrict__ in,
+                                                  int nrows, FftPlan p, int rb,
+                                                  const typename Cx<CT>::T *__restrict__ twg,
+                                                  CT *__restrict__ phi) {
+  fftx_inv_body<CT>(blockIdx.x, in, nrows, p, rb, twg, phi);
+}
+
 
 // y pass (either direction): block = (z, chunk of W kx columns).
 template <typename CT>
-__global__ __launch_bounds__(256) void k_ffty(typename Cx<CT>::T *__restrict__ data, int H, int K1,
-                                              int W, FftPlan p,
-                                              const typename Cx<CT>::T *__restrict__ twg, int inv) {
+__device__ __forceinline__ void ffty_body(int bid, typename Cx<CT>::T *__restrict__ data, int H,
+                                          int K1, int W, const FftPlan &p,
+                                          const typename Cx<CT>::T *__restrict__ twg, int inv) {
   using C2 = typename Cx<CT>::T;
   extern __shared__ char smem[];
   const int n = p.n;
@@ -458,7 +490,7 @@ __global__ __launch_bounds__(256) void k_ffty(typename Cx<CT>::T *__restrict__ d
   C2 *b = a + W * n;
   for (int e = threadIdx.x; e < n; e += blockDim.x) tw[e] = twg[e];
   const int nchunk = (H + W - 1) / W;
-  const int z = blockIdx.x / nchunk, kx0 = (blockIdx.x % nchunk) * W;
+  const int z = bid / nchunk, kx0 = (bid % nchunk) * W;
   const int wc = min(W, H - kx0);
   for (int e = threadIdx.x; e < W * n; e += blockDim.x) {
     const int y = e / W, c = e - y * W;
@@ -471,6 +503,13 @@ __global__ __launch_bounds__(256) void k_ffty(typename Cx<CT>::T *__restrict__ d
     if (c < wc) data[((size_t)z * K1 + y) * H + kx0 + c] = res[c * n + y];
   }
 }
+template <typename CT>
+__global__ __launch_bounds__(256) void k_ffty(typename Cx<CT>::T *__restrict__ data, int H, int K1,
+                                              int W, FftPlan p,
+                                              const typename Cx<CT>::T *__restrict__ twg, int inv) {
+  ffty_body<CT>(blockIdx.x, data, H, K1, W, p, twg, inv);
+}
+
 
 struct ConvArgs {
   int K0, K1, K2, H, W;
@@ -485,11 +524,11 @@ template <> __device__ __forceinline__ double exp_k<double>(double x) { return e
 
 // z pass with the k-space convolution in the middle: block = (y, W columns).
 template <typename CT, bool CALC_EV>
-__global__ __launch_bounds__(256) void k_fftz_conv(typename Cx<CT>::T *__restrict__ data, ConvArgs c,
-                                                   FftPlan p,
-                                                   const typename Cx<CT>::T *__restrict__ twg,
-                                                   const CT *__restrict__ prefac,
-                                                   double *__restrict__ part) {
+__device__ __forceinline__ void fftz_conv_body(int bid, typename Cx<CT>::T *__restrict__ data,
+                                               const ConvArgs &c, const FftPlan &p,
+                                               const typename Cx<CT>::T *__restrict__ twg,
+                                               const CT *__restrict__ prefac,
+                                               double *__restrict__ part) {
   using C2 = typename Cx<CT>::T;
   extern __shared__ char smem[];
   const int n = p.n, H = c.H, W = c.W;
@@ -501,7 +540,7 @@ __global__ __launch_bounds__(256) void k_fftz_conv(typename Cx<CT>::T *__restric
   for (int e = threadIdx.x; e < n; e += blockDim.x) tw[e] = twg[e];
   for (int e = threadIdx.x; e < c.K0 + c.K1 + c.K2; e += blockDim.x) pre[e] = prefac[e];
   const int nchunk = (H + W - 1) / W;
-  const int y = blockIdx.x / nchunk, kx0 = (blockIdx.x % nchunk) * W;
+  const int y = bid / nchunk, kx0 = (bid % nchunk) * W;
   const int wc = min(W, H - kx0);
   const size_t zs = (size_t)c.K1 * H;
   for (int e = threadIdx.x; e < W * n; e += blockDim.x) {
@@ -566,10 +605,57 @@ __global__ __launch_bounds__(256) void k_fftz_conv(typename Cx<CT>::T *__restric
       double s = 0;
       for (int w = 0; w < 4; w++) s += red[w * 8 + threadIdx.x];
       // energy +1/2 ccelec sum, virial -1/2 ccelec sum
-      part[blockIdx.x * kRecipPart + threadIdx.x] =
+      part[bid * kRecipPart + threadIdx.x] =
           threadIdx.x == 0 ? s * c.half_ccelec : -s * c.half_ccelec;
     }
   }
+}
+template <typename CT, bool CALC_EV>
+__global__ __launch_bounds__(256) void k_fftz_conv(typename Cx<CT>::T *__restrict__ data, ConvArgs c,
+                                                   FftPlan p,
+                                                   const typename Cx<CT>::T *__restrict__ twg,
+                                                   const CT *__restrict__ prefac,
+                                                   double *__restrict__ part) {
+  fftz_conv_body<CT, CALC_EV>(blockIdx.x, data, c, p, twg, prefac, part);
+}
+
+
+// Launch shapes of the reciprocal passes for one system (shared by the
+// single-system launcher and the fused multi-system launch).
+template <typename CT> struct PmeShapes {
+  GridGeom g;
+  BrickGeom bg;
+  int nbricks, nrows, npair, rb, wy, ncy, wz, ncz;
+  size_t shx, shy, shz;
+  ConvArgs c;
+};
+
+template <typename CT> static PmeShapes<CT> pme_shapes(const NbSystem &S) {
+  using C2 = typename Cx<CT>::T;
+  PmeShapes<CT> P;
+  const int K0 = S.K[0], K1 = S.K[1], K2 = S.K[2], H = K0 / 2 + 1;
+  P.g = GridGeom{K0, K1, K2, 1.0 / S.box[0], 1.0 / S.box[1], 1.0 / S.box[2]};
+  P.bg = BrickGeom{K0, K1, K2, (K0 + kBx - 1) / kBx, (K1 + kBy - 1) / kBy, (K2 + kBz - 1) / kBz,
+                   (float)S.box[0], (float)S.box[1], (float)S.box[2]};
+  P.nbricks = P.bg.nbx * P.bg.nby * P.bg.nbz;
+  const size_t lds_cap = 64 * 1024;
+  P.nrows = K1 * K2;
+  P.rb = (int)std::max<size_t>(
+      1, std::min<size_t>(kFftRowsPerBlock, (lds_cap / sizeof(C2) - K0) / (2 * K0)));
+  P.shx = (size_t)(K0 + 2 * P.rb * K0) * sizeof(C2);
+  P.npair = (P.nrows + 1) / 2;
+  const int W = kFftColsPerBlock;
+  P.wy = (int)std::max<size_t>(1, std::min<size_t>(W, (lds_cap / sizeof(C2) - K1) / (2 * K1)));
+  P.shy = (size_t)(K1 + 2 * P.wy * K1) * sizeof(C2);
+  P.ncy = (H + P.wy - 1) / P.wy;
+  P.wz = (int)std::max<size_t>(
+      1, std::min<size_t>(W, (lds_cap / sizeof(C2) - K2 - (K0 + K1 + K2)) / (2 * K2)));
+  P.shz = (size_t)(K2 + 2 * P.wz * K2) * sizeof(C2) + (size_t)(K0 + K1 + K2) * sizeof(CT);
+  P.ncz = (H + P.wz - 1) / P.wz;
+  P.c = ConvArgs{K0, K1, K2, H, P.wz, 1.0 / S.box[0], 1.0 / S.box[1], 1.0 / S.box[2],
+                 kPi * kPi / (S.kappa * S.kappa),
+                 1.0 / (S.box[0] * S.box[1] * S.box[2]) / kPi, 0.5 * kCcelec};
+  return P;
 }
 
 template <typename CT>
@@ -577,64 +663,42 @@ static void recip_t(NbSystem &S, const float4 *x, int64_t *force, int stride, bo
                     typename Cx<CT>::T *cgrid, CT *phi, const typename Cx<CT>::T *twx,
                     const typename Cx<CT>::T *twy, const typename Cx<CT>::T *twz, const CT *pre,
                     hipStream_t s) {
-  using C2 = typename Cx<CT>::T;
-  const int K0 = S.K[0], K1 = S.K[1], K2 = S.K[2], H = K0 / 2 + 1;
-  GridGeom g{K0, K1, K2, 1.0 / S.box[0], 1.0 / S.box[1], 1.0 / S.box[2]};
+  (void)x;
+  const PmeShapes<CT> P = pme_shapes<CT>(S);
+  const int K1 = S.K[1], K2 = S.K[2], H = S.K[0] / 2 + 1;
   const double qunscale = 1.0;  // sorted coordinates carry the raw charge
   hipEvent_t t0 = nullptr;
   S.tic(1, s, t0);
-  BrickGeom bg{K0, K1, K2, (K0 + kBx - 1) / kBx, (K1 + kBy - 1) / kBy, (K2 + kBz - 1) / kBz,
-               (float)S.box[0], (float)S.box[1], (float)S.box[2]};
   CT *rgrid = reinterpret_cast<CT *>(S.qgrid.get());
-  k_spread_brick<CT><<<bg.nbx * bg.nby * bg.nbz, 256, 0, s>>>(S.posq.get(), S.cell_bb.get(),
-                                                              S.meta.get(), bg, qunscale, rgrid);
+  k_spread_brick<CT><<<P.nbricks, 256, 0, s>>>(S.posq.get(), S.cell_bb.get(), S.meta.get(), P.bg,
+                                               qunscale, rgrid);
   APO_LAUNCH_CHECK();
   S.toc(1, s, t0);
   S.tic(2, s, t0);
-  const size_t lds_cap = 64 * 1024;
-  // x forward
-  const int nrows = K1 * K2;
-  int rb = (int)std::max<size_t>(
-      1, std::min<size_t>(kFftRowsPerBlock, (lds_cap / sizeof(C2) - K0) / (2 * K0)));
-  size_t shx = (size_t)(K0 + 2 * rb * K0) * sizeof(C2);
-  const int npair = (nrows + 1) / 2;
-  k_fftx_fwd<CT><<<(npair + rb - 1) / rb, 256, shx, s>>>(rgrid, nrows, plan_of(S.fx), rb, twx,
-                                                        cgrid);
+  k_fftx_fwd<CT><<<(P.npair + P.rb - 1) / P.rb, 256, P.shx, s>>>(rgrid, P.nrows, plan_of(S.fx),
+                                                                  P.rb, twx, cgrid);
   APO_LAUNCH_CHECK();
-  // y forward
-  const int W = kFftColsPerBlock;
-  const int wy = (int)std::max<size_t>(1, std::min<size_t>(W, (lds_cap / sizeof(C2) - K1) / (2 * K1)));
-  size_t shy = (size_t)(K1 + 2 * wy * K1) * sizeof(C2);
-  const int ncy = (H + wy - 1) / wy;
-  k_ffty<CT><<<K2 * ncy, 256, shy, s>>>(cgrid, H, K1, wy, plan_of(S.fy), twy, 0);
+  k_ffty<CT><<<K2 * P.ncy, 256, P.shy, s>>>(cgrid, H, K1, P.wy, plan_of(S.fy), twy, 0);
   APO_LAUNCH_CHECK();
-  // z forward + convolution + z inverse
-  const int wz = (int)std::max<size_t>(
-      1, std::min<size_t>(W, (lds_cap / sizeof(C2) - K2 - (K0 + K1 + K2)) / (2 * K2)));
-  size_t shz = (size_t)(K2 + 2 * wz * K2) * sizeof(C2) + (size_t)(K0 + K1 + K2) * sizeof(CT);
-  const int ncz = (H + wz - 1) / wz;
-  ConvArgs c{K0, K1, K2, H, wz, 1.0 / S.box[0], 1.0 / S.box[1], 1.0 / S.box[2],
-             kPi * kPi / (S.kappa * S.kappa),
-             1.0 / (S.box[0] * S.box[1] * S.box[2]) / kPi, 0.5 * kCcelec};
-  if (K1 * ncz > S.part.recip_n) throw RuntimeError("reciprocal partial buffer too small");
+  if (K1 * P.ncz > S.part.recip_n) throw RuntimeError("reciprocal partial buffer too small");
   double *part = S.parts.get() + S.part.recip_off;
   if (EV) {
     launch_fill_f64(part, (size_t)S.part.recip_n * kRecipPart, s);
-    k_fftz_conv<CT, true><<<K1 * ncz, 256, shz, s>>>(cgrid, c, plan_of(S.fz), twz, pre, part);
+    k_fftz_conv<CT, true><<<K1 * P.ncz, 256, P.shz, s>>>(cgrid, P.c, plan_of(S.fz), twz, pre, part);
   } else {
-    k_fftz_conv<CT, false><<<K1 * ncz, 256, shz, s>>>(cgrid, c, plan_of(S.fz), twz, pre, part);
+    k_fftz_conv<CT, false><<<K1 * P.ncz, 256, P.shz, s>>>(cgrid, P.c, plan_of(S.fz), twz, pre,
+                                                          part);
   }
   APO_LAUNCH_CHECK();
-  // y inverse
-  k_ffty<CT><<<K2 * ncy, 256, shy, s>>>(cgrid, H, K1, wy, plan_of(S.fy), twy, 1);
+  k_ffty<CT><<<K2 * P.ncy, 256, P.shy, s>>>(cgrid, H, K1, P.wy, plan_of(S.fy), twy, 1);
   APO_LAUNCH_CHECK();
-  // x inverse
-  k_fftx_inv<CT><<<(npair + rb - 1) / rb, 256, shx, s>>>(cgrid, nrows, plan_of(S.fx), rb, twx, phi);
+  k_fftx_inv<CT><<<(P.npair + P.rb - 1) / P.rb, 256, P.shx, s>>>(cgrid, P.nrows, plan_of(S.fx),
+                                                                  P.rb, twx, phi);
   APO_LAUNCH_CHECK();
   S.toc(2, s, t0);
   S.tic(3, s, t0);
   k_gather_sorted<CT><<<(S.ssort * 16 + 255) / 256, 256, 0, s>>>(
-      S.posq.get(), S.meta.get(), S.loc2glo.get(), g, qunscale, phi,
+      S.posq.get(), S.meta.get(), S.loc2glo.get(), P.g, qunscale, phi,
       reinterpret_cast<unsigned long long *>(force), stride);
   APO_LAUNCH_CHECK();
   S.toc(3, s, t0);
@@ -648,6 +712,153 @@ void recip_launch(NbSystem &S, const float4 *x, int64_t *force, int stride, bool
   else
     recip_t<float>(S, x, force, stride, E || V, S.cgrid.get(), S.phi.get(), S.fx.tw.get(),
                    S.fy.tw.get(), S.fz.tw.get(), S.prefac.get(), s);
+}
+
+// ---- fused launches over several systems (mixed precision, no energy) ------
+struct SpreadArgs {
+  const int4 *posq;
+  const float4 *bb;
+  const int *meta;
+  BrickGeom g;
+  float *grid;
+};
+struct FftxArgs {
+  const float *grid;   // forward: real grid in
+  const float2 *cin;   // inverse: complex half rows in
+  int nrows;
+  FftPlan p;
+  int rb;
+  const float2 *tw;
+  float2 *cout;        // forward: complex half rows out
+  float *phi;          // inverse: potential out
+};
+struct FftyArgs {
+  float2 *data;
+  int H, K1, W;
+  FftPlan p;
+  const float2 *tw;
+};
+struct FftzArgs {
+  float2 *data;
+  ConvArgs c;
+  FftPlan p;
+  const float2 *tw;
+  const float *prefac;
+};
+struct GatherArgs {
+  const int4 *posq;
+  const int *meta, *loc2glo;
+  GridGeom g;
+  const float *phi;
+  unsigned long long *force;
+  int stride;
+};
+
+__global__ __launch_bounds__(256) void kb_spread(const SpreadArgs *__restrict__ tab,
+                                                 const int *__restrict__ bstart, int nsys) {
+  const int s = batch_system(bstart, nsys);
+  const SpreadArgs &a = tab[s];
+  spread_body<float>(blockIdx.x - bstart[s], a.posq, a.bb, a.meta, a.g, 1.0, a.grid);
+}
+__global__ __launch_bounds__(256) void kb_fftx_fwd(const FftxArgs *__restrict__ tab,
+                                                   const int *__restrict__ bstart, int nsys) {
+  const int s = batch_system(bstart, nsys);
+  const FftxArgs &a = tab[s];
+  fftx_fwd_body<float>(blockIdx.x - bstart[s], a.grid, a.nrows, a.p, a.rb, a.tw, a.cout);
+}
+__global__ __launch_bounds__(256) void kb_fftx_inv(const FftxArgs *__restrict__ tab,
+                                                   const int *__restrict__ bstart, int nsys) {
+  const int s = batch_system(bstart, nsys);
+  const FftxArgs &a = tab[s];
+  fftx_inv_body<float>(blockIdx.x - bstart[s], a.cin, a.nrows, a.p, a.rb, a.tw, a.phi);
+}
+__global__ __launch_bounds__(256) void kb_ffty(const FftyArgs *__restrict__ tab,
+                                               const int *__restrict__ bstart, int nsys, int inv) {
+  const int s = batch_system(bstart, nsys);
+  const FftyArgs &a = tab[s];
+  ffty_body<float>(blockIdx.x - bstart[s], a.data, a.H, a.K1, a.W, a.p, a.tw, inv);
+}
+__global__ __launch_bounds__(256) void kb_fftz_conv(const FftzArgs *__restrict__ tab,
+                                                    const int *__restrict__ bstart, int nsys) {
+  const int s = batch_system(bstart, nsys);
+  const FftzArgs &a = tab[s];
+  fftz_conv_body<float, false>(blockIdx.x - bstart[s], a.data, a.c, a.p, a.tw, a.prefac, nullptr);
+}
+__global__ __launch_bounds__(256) void kb_gather(const GatherArgs *__restrict__ tab,
+                                                 const int *__restrict__ bstart, int nsys) {
+  const int s = batch_system(bstart, nsys);
+  const GatherArgs &a = tab[s];
+  gather_body<float>(blockIdx.x - bstart[s], a.posq, a.meta, a.loc2glo, a.g, 1.0, a.phi, a.force,
+                     a.stride);
+}
+
+struct PmeBatch {
+  BatchLaunch<SpreadArgs> spread;
+  BatchLaunch<FftxArgs> fx_fwd, fx_inv;
+  BatchLaunch<FftyArgs> fy;
+  BatchLaunch<FftzArgs> fz;
+  BatchLaunch<GatherArgs> gather;
+};
+
+PmeBatch *pme_batch_create() { return new PmeBatch(); }
+void pme_batch_destroy(PmeBatch *b) { delete b; }
+void pme_batch_clear(PmeBatch &b) {
+  b.spread.clear();
+  b.fx_fwd.clear();
+  b.fx_inv.clear();
+  b.fy.clear();
+  b.fz.clear();
+  b.gather.clear();
+}
+
+void pme_batch_add(PmeBatch &b, NbSystem &S, int64_t *force, int stride) {
+  if (S.precision != 0) throw InvalidArgument("fused batches run the mixed-precision path");
+  const PmeShapes<float> P = pme_shapes<float>(S);
+  if (S.K[1] * P.ncz > S.part.recip_n) throw RuntimeError("reciprocal partial buffer too small");
+  float *rgrid = reinterpret_cast<float *>(S.qgrid.get());
+  const int H = S.K[0] / 2 + 1;
+  b.spread.add(SpreadArgs{S.posq.get(), S.cell_bb.get(), S.meta.get(), P.bg, rgrid}, P.nbricks);
+  FftxArgs xa{rgrid, S.cgrid.get(), P.nrows, plan_of(S.fx), P.rb, S.fx.tw.get(), S.cgrid.get(),
+              S.phi.get()};
+  b.fx_fwd.add(xa, (P.npair + P.rb - 1) / P.rb, P.shx);
+  b.fx_inv.add(xa, (P.npair + P.rb - 1) / P.rb, P.shx);
+  b.fy.add(FftyArgs{S.cgrid.get(), H, S.K[1], P.wy, plan_of(S.fy), S.fy.tw.get()},
+           S.K[2] * P.ncy, P.shy);
+  b.fz.add(FftzArgs{S.cgrid.get(), P.c, plan_of(S.fz), S.fz.tw.get(), S.prefac.get()},
+           S.K[1] * P.ncz, P.shz);
+  b.gather.add(GatherArgs{S.posq.get(), S.meta.get(), S.loc2glo.get(), P.g, S.phi.get(),
+                          reinterpret_cast<unsigned long long *>(force), stride},
+               (S.ssort * 16 + 255) / 256);
+}
+
+void pme_batch_upload(PmeBatch &b, hipStream_t s) {
+  b.spread.upload(s);
+  b.fx_fwd.upload(s);
+  b.fx_inv.upload(s);
+  b.fy.upload(s);
+  b.fz.upload(s);
+  b.gather.upload(s);
+}
+
+// spread, the five FFT passes with the convolution, gather: one launch each
+// for all systems (cell boxes must be current)
+void pme_batch_launch(PmeBatch &b, hipStream_t s) {
+  if (!b.spread.nsys()) return;
+  kb_spread<<<b.spread.blocks(), 256, 0, s>>>(b.spread.d_args.get(), b.spread.d_start.get(),
+                                              b.spread.nsys());
+  kb_fftx_fwd<<<b.fx_fwd.blocks(), 256, b.fx_fwd.lds, s>>>(b.fx_fwd.d_args.get(),
+                                                           b.fx_fwd.d_start.get(), b.fx_fwd.nsys());
+  kb_ffty<<<b.fy.blocks(), 256, b.fy.lds, s>>>(b.fy.d_args.get(), b.fy.d_start.get(), b.fy.nsys(),
+                                               0);
+  kb_fftz_conv<<<b.fz.blocks(), 256, b.fz.lds, s>>>(b.fz.d_args.get(), b.fz.d_start.get(),
+                                                    b.fz.nsys());
+  kb_ffty<<<b.fy.blocks(), 256, b.fy.lds, s>>>(b.fy.d_args.get(), b.fy.d_start.get(), b.fy.nsys(),
+                                               1);
+  kb_fftx_inv<<<b.fx_inv.blocks(), 256, b.fx_inv.lds, s>>>(b.fx_inv.d_args.get(),
+                                                           b.fx_inv.d_start.get(), b.fx_inv.nsys());
+  kb_gather<<<b.gather.blocks(), 256, 0, s>>>(b.gather.d_args.get(), b.gather.d_start.get(),
+                                              b.gather.nsys());
+  APO_LAUNCH_CHECK();
 }
 
 }  // namespace apo

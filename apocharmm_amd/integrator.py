@@ -203,6 +203,36 @@ def propagate_batch(integrators, numSteps):
         ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
 
 
+class LangevinBatch:
+    """Persistent batch over several integrators (apo_lt_batch_*): one graph
+    launch per batch step, systems chained into `lanes` parallel branches."""
+
+    def __init__(self, integrators, lanes=0):
+        self.members = list(integrators)
+        arr = (ctypes.c_void_p * len(self.members))(*[lt._handle.value for lt in self.members])
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().apo_lt_batch_create(arr, len(self.members), int(lanes),
+                                                  ctypes.byref(h)))
+        self._handle = h
+
+    def propagate(self, numSteps):
+        torch = _torch()
+        _lib.check(_lib.lib().apo_lt_batch_propagate(
+            self._handle, int(numSteps), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+    def close(self):
+        if self._handle:
+            _torch().cuda.synchronize()
+            _lib.check(_lib.lib().apo_lt_batch_destroy(self._handle))
+            self._handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def maxwell_velocities(masses, temperature, seed):
     """Gaussian velocities with variance kB T / m per component (A/AKMA) and
     zero mass-weighted mean, as CharmmContext::assignVelocitiesAtTemperature

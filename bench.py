@@ -91,8 +91,8 @@ def bench_batched(a):
     import torch
     from apocharmm_amd import systems
     from apocharmm_amd.force_manager import ForceManager
-    from apocharmm_amd.integrator import (LangevinThermostatIntegrator, maxwell_velocities,
-                                          propagate_batch)
+    from apocharmm_amd.integrator import (LangevinBatch, LangevinThermostatIntegrator,
+                                          maxwell_velocities)
 
     torch.cuda.set_device(0)
     sizes = systems.batched_sizes(a.nsys)
@@ -117,7 +117,6 @@ def bench_batched(a):
     # warm both paths (graph capture, first rebuilds)
     for fm, lt in runs:
         lt.propagate(a.warmup)
-    propagate_batch([lt for _, lt in runs], a.warmup)
     torch.cuda.synchronize()
     t_seq = 0.0
     for fm, lt in runs:
@@ -125,10 +124,14 @@ def bench_batched(a):
         lt.propagate(a.steps)
         torch.cuda.synchronize()
         t_seq += time.perf_counter() - t0
+    batch = LangevinBatch([lt for _, lt in runs], lanes=a.lanes)
+    batch.propagate(a.warmup)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    propagate_batch([lt for _, lt in runs], a.steps)
+    batch.propagate(a.steps)
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
+    batch.close()
     n = len(runs)
     agg = lambda t: n * a.steps * DT_PS / 1000.0 / (t / 86400.0)
     for fm, lt in runs:
@@ -154,7 +157,8 @@ def bench_batched(a):
                    "dt_fs": 2.0, "friction_per_ps": FRICTION, "temperature_K": TEMP,
                    "list_rebuild_every": REBUILD_EVERY, "precision": a.precision,
                    "gpu_max_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
-                   "parallelism": "single GPU, one stream per system"},
+                   "lanes": a.lanes,
+                   "parallelism": "single GPU, one graph launch per batch step"},
         "sequential": {"value": round(agg(t_seq), 3), "unit": "ns/day",
                        "ms_per_step_all_systems": round(t_seq / a.steps * 1e3, 4)},
         "speedup_vs_sequential": round(t_seq / t_batch, 3),
@@ -174,8 +178,10 @@ def main():
     ap.add_argument("--step", choices=("md", "force"), default="md")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nsys", type=int, default=64, help="systems in the c4 batch")
-    ap.add_argument("--hw-queues", type=int, default=16,
-                    help="GPU_MAX_HW_QUEUES for the c4 batch (streams that run concurrently)")
+    ap.add_argument("--hw-queues", type=int, default=4,
+                    help="GPU_MAX_HW_QUEUES for the c4 run (HIP's default is 4)")
+    ap.add_argument("--lanes", type=int, default=4,
+                    help="parallel chains of systems in the c4 batch graph")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build runs one system per process on a "

@@ -303,6 +303,19 @@ APO_API apo_status apo_lt_propagate(apo_lt_integrator *lt, int32_t nsteps,
 APO_API apo_status apo_lt_propagate_batch(apo_lt_integrator *const *lts, int32_t n,
                                           int32_t nsteps, void *stream);
 
+/* A persistent batch over n integrators (no counterpart in the reference):
+ * apo_lt_batch_propagate advances all of them by nsteps with one graph launch
+ * per batch step, the systems' steps chained into `lanes` parallel branches
+ * (0 selects 4, the default number of hardware queues).  Each trajectory is
+ * bitwise the one apo_lt_propagate gives the system alone.  While the batch
+ * exists its systems run their force terms on one stream each; destroy the
+ * batch before the integrators.  apo_lt_propagate_batch is a one-off batch. */
+typedef struct apo_lt_batch apo_lt_batch;
+APO_API apo_status apo_lt_batch_create(apo_lt_integrator *const *lts, int32_t n, int32_t lanes,
+                                       apo_lt_batch **out);
+APO_API apo_status apo_lt_batch_propagate(apo_lt_batch *b, int32_t nsteps, void *stream);
+APO_API apo_status apo_lt_batch_destroy(apo_lt_batch *b);
+
 /* Copies the state to the host (synchronises): any output may be NULL.
  *  xyz, vel, dprev, grad: double[3*natom] AoS; vel is the on-step velocity of
  *    the last step, dprev the last half-step displacement (getCoordsDelta),

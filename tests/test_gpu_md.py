@@ -206,3 +206,38 @@ def test_reset_list_rejects_non_finite_coordinates():
     with pytest.raises(_lib.ApoError, match="non-finite"):
         fm.resetNeighborList(t)
     fm.close()
+
+
+def test_md_persistent_batch_lanes_bitwise_equals_alone():
+    """apo_lt_batch_*: two propagate calls (graphs reused) with systems
+    chained into 2 lanes give each system its lone trajectory bitwise."""
+    from apocharmm_amd.integrator import LangevinBatch, maxwell_velocities
+    sizes = [520, 610, 700]
+    specs = [systems.batched_system(i + 7, w) for i, w in enumerate(sizes)]
+
+    def fresh():
+        out = []
+        for i, sp in enumerate(specs):
+            v = maxwell_velocities(sp["masses"], TEMP, 4300 + i)
+            out.append(_make(sp, np.asarray(sp["xyz"], np.float64), v, seed=SEED + 11 * i,
+                             nbfreq=7))
+        return out
+
+    alone = []
+    for fm, lt in fresh():
+        lt.propagate(9)
+        lt.propagate(13)
+        alone.append(lt.getState())
+        lt.close()
+        fm.close()
+    runs = fresh()
+    b = LangevinBatch([lt for _, lt in runs], lanes=2)
+    b.propagate(9)
+    b.propagate(13)
+    b.close()
+    for (fm, lt), ref in zip(runs, alone):
+        st = lt.getState()
+        for k in ("xyz", "vel", "dprev", "grad"):
+            assert np.array_equal(st[k], ref[k]), k
+        lt.close()
+        fm.close()
