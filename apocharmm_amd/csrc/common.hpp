@@ -111,9 +111,15 @@ template <typename T> struct DevBuf {
     APO_HIP(hipMalloc(&p, c * sizeof(T)));
     n = c;
   }
+  // Copies count elements from host memory, ordered on stream s; returns once
+  // the copy has finished, so the host source may be reused or freed at once
+  // (uploads happen at set-up and at table rebuilds, never per step).
   void upload(const T *h, size_t count, hipStream_t s) {
     reserve(count);
-    if (count) APO_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    if (count) {
+      APO_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+      APO_HIP(hipStreamSynchronize(s));
+    }
   }
   void upload(const std::vector<T> &v, hipStream_t s) { upload(v.data(), v.size(), s); }
   T *get() const { return p; }
@@ -221,7 +227,7 @@ void launch_force_to_double(const int64_t *f, int natom, int stride,
                             double *out, hipStream_t s);
 
 void nblist_build(NbSystem &sys, const float4 *d_xyzq, hipStream_t s, bool sizing);
-void list_export_check(NbSystem &sys, hipStream_t s);
+void list_export_check(NbSystem &sys, hipStream_t s, bool allow_growth);
 void list_reserve(NbSystem &sys, int ncand, int nent, int nitem);
 void nblist_sort_coords(NbSystem &sys, const float4 *d_xyzq, hipStream_t s);
 void nblist_cellbb(NbSystem &sys, hipStream_t s);

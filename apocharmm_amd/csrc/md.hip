@@ -27,7 +27,7 @@ namespace apo {
 
 void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int stride,
                 double *energy, double *virial, hipStream_t s);
-void list_export_check(NbSystem &S, hipStream_t s);
+void list_export_check(NbSystem &S, hipStream_t s, bool allow_growth);
 
 // fused (multi-system) launches of the step kernels, one per kernel type
 void batch_add_sort(BatchLaunch<SortArgs> &b, NbSystem &S, const float4 *x);
@@ -670,7 +670,7 @@ struct LtSystem {
       } else {
         // errors and capacity growth of earlier (device-side) rebuilds are
         // handled before the next one is queued
-        if (rebuild) list_export_check(*nb, s);
+        if (rebuild) list_export_check(*nb, s, true);
         replay(rebuild ? 1 : 0, s);
       }
       steps_since_rebuild++;
@@ -898,7 +898,7 @@ struct LtBatch {
     if (m) (void)hipStreamDestroy(m);
     if (pme) pme_batch_destroy(pme);
     if (bond) bonded_batch_destroy(bond);
-    for (size_t i = 0; i < sys.size(); i++)
+    for (size_t i = 0; i < was_single.size() && i < sys.size(); i++)
       if (!was_single[i]) {
         sys[i]->nb->single_stream = false;
         sys[i]->nb->generation++;
@@ -1030,7 +1030,7 @@ struct LtBatch {
       for (LtSystem *L : sys)
         if (L->comfreq > 0 && L->step % (unsigned long long)L->comfreq == 0) L->remove_com(m);
       if (rb)
-        for (LtSystem *L : sys) list_export_check(*L->nb, m);
+        for (LtSystem *L : sys) list_export_check(*L->nb, m, true);
       bool stale = !ex[rb];
       for (int i = 0; i < n && !stale; i++) stale = gens[rb][i] != sys[i]->nb->generation;
       if (stale) capture(rb);
@@ -1111,7 +1111,9 @@ void lt_propagate_batch(LtSystem **Ls, int n, int nsteps, hipStream_t caller) {
 void lt_get_state(LtSystem &L, double *xyz, double *vel, double *dprev, double *grad,
                   double *thermo, int64_t *step) {
   APO_HIP(hipStreamSynchronize(L.st));
-  list_export_check(*L.nb, L.st);  // every rebuild so far has finished
+  // every rebuild so far has finished: report its errors (capacities grow
+  // only right before a rebuild, when the current lists are discarded)
+  list_export_check(*L.nb, L.st, false);
   const int n = L.n;
   std::vector<double> soa(3 * (size_t)n);
   auto from_soa = [&](const double *dev, double *out) {

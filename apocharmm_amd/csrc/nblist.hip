@@ -425,14 +425,19 @@ __global__ void k_meta_export(int *__restrict__ meta, int *__restrict__ host) {
 static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
 
 // Host side of the exported counts: errors of a finished build raise here;
-// capacities that the last finished build used above 85 % grow (the device
-// is drained first, captured step graphs see the new generation).
-void list_export_check(NbSystem &S, hipStream_t s) {
+// with allow_growth (only right before a rebuild, whose lists replace the
+// current ones) capacities that the last finished build used above 85 % grow:
+// the device is drained first and captured step graphs see the new
+// generation.  Without it a pending growth stays pending for the next check.
+void list_export_check(NbSystem &S, hipStream_t s, bool allow_growth) {
   const volatile int *h = S.h_meta;
   if (!h || h[15] == 0) return;
   int m[16];
   for (int k = 0; k < 16; k++) m[k] = h[k];
   if (m[15] == S.meta_checked) return;
+  const bool grow = m[1] > 0.85 * S.cand_cap || m[2] > 0.85 * S.ent_cap || m[3] > 0.85 * S.item_cap;
+  const bool error = m[5] || m[6] || m[7] || m[8] == 1;
+  if (!error && grow && !allow_growth) return;
   S.meta_checked = m[15];
   if (m[6]) throw RuntimeError("non-finite coordinates in xyzq at a neighbor-list rebuild");
   if (m[5]) throw RuntimeError("neighbor list: an xy column exceeds the LDS sort capacity");
@@ -440,7 +445,6 @@ void list_export_check(NbSystem &S, hipStream_t s) {
   if (m[8] == 1)
     throw RuntimeError("neighbor list: a rebuild exceeded the list capacity; the forces of the "
                        "steps after it are invalid (capacity was sized from earlier builds)");
-  const bool grow = m[1] > 0.85 * S.cand_cap || m[2] > 0.85 * S.ent_cap || m[3] > 0.85 * S.item_cap;
   if (grow) {
     APO_HIP(hipStreamSynchronize(s));
     APO_HIP(hipDeviceSynchronize());
@@ -487,7 +491,7 @@ void nblist_build(NbSystem &S, const float4 *x, hipStream_t s, bool sizing) {
   APO_HIP(hipStreamIsCapturing(s, &cap));
   if (sizing && cap != hipStreamCaptureStatusNone)
     throw RuntimeError("the first neighbor-list build cannot be captured in a graph");
-  if (!sizing && cap == hipStreamCaptureStatusNone) list_export_check(S, s);
+  if (!sizing && cap == hipStreamCaptureStatusNone) list_export_check(S, s, true);
 
   launch_fill_i32(meta, 16, s);
   launch_fill_i32(S.col_count.get(), S.ncol + 1, s);
@@ -603,7 +607,7 @@ void nblist_stats(NbSystem &S, int64_t *out6, hipStream_t s) {
   int hmeta[16];
   APO_HIP(hipMemcpyAsync(hmeta, S.meta.get(), sizeof(hmeta), hipMemcpyDeviceToHost, s));
   APO_HIP(hipStreamSynchronize(s));
-  list_export_check(S, s);
+  list_export_check(S, s, false);
   const int nent = hmeta[8] ? 0 : std::min(hmeta[2], (int)S.ent_cap);
   DevBuf<unsigned long long> cnt;
   cnt.reserve(1);

@@ -207,6 +207,26 @@ def test_small_box_below_list_cutoff():
     assert force_rms_rel(got["grad"], ref["grad"]) <= MIXED_FORCE_RTOL
 
 
+@pytest.mark.parametrize("n_waters", [1200, 17146])
+def test_odd_pme_grid_against_oracle(oracle, n_waters):
+    """Boxes whose ~1 A PME grid is odd (27^3 = 3^9 points, 81^3 = four
+    radix-3 passes): odd grids have no Nyquist bin and the x/L + 1/2 origin is
+    not an integer grid shift, so the spread, FFT and gather must follow the
+    oracle's convention exactly."""
+    L = (n_waters / systems.WATER_DENSITY) ** (1.0 / 3.0)
+    K = systems.smooth_grid(L)
+    assert K % 2 == 1
+    spec = systems.tip3p_box(n_waters, L, K, seed=6000 + n_waters)
+    print("case odd grid %d waters K=%d" % (n_waters, K), flush=True)
+    got = gpu_eval(spec)
+    ref = oracle_eval(oracle, spec)
+    err = report("odd K=%d" % K, got, ref)
+    assert err <= MIXED_FORCE_RTOL
+    scale = energy_scale(ref)
+    for k in ("ewks", "elec", "vdw", "ewex"):
+        assert abs(got["energy"][k] - ref["energy"][k]) <= MIXED_ENERGY_RTOL * scale, k
+
+
 def test_config2_full_size_against_oracle(oracle):
     """98,304-atom TIP3P box, 100^3 grid: the benchmark workload at step 0."""
     spec = systems.config2()

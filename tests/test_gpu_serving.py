@@ -1,0 +1,32 @@
+"""Continuous-batching server: every job's result in the mix is bitwise its
+lone result (same propagate(chunk) sequence, fused kernels run each system's
+arithmetic unchanged), and the latency bookkeeping is consistent."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.skip(reason="Server.run is disabled: a host segmentation fault inside "
+                         "apo_lt_batch_propagate after batch membership changes is unresolved")
+def test_serving_results_equal_lone_runs():
+    from apocharmm_amd import serving
+    jobs = serving.make_jobs(6, rate=50.0, atoms=(1500, 4000), steps=(100, 400), chunk=50)
+    for j in jobs:
+        j.prepare()
+    srv = serving.Server(max_slots=4, chunk=50, lanes=2, keep_state=True)
+    srv.run(jobs)
+    assert all(j.done == j.nsteps for j in jobs)
+    alone = serving.make_jobs(6, rate=50.0, atoms=(1500, 4000), steps=(100, 400), chunk=50)
+    for j in alone:
+        j.prepare()
+    serving.run_plain(alone, chunk=50, keep_state=True)
+    for a, b in zip(jobs, alone):
+        assert a.nsteps == b.nsteps
+        for k in ("xyz", "vel", "dprev"):
+            assert np.array_equal(a.result[k], b.result[k]), (a.id, k)
+    s = serving.summarize(jobs)
+    assert 0 <= s["mean_queue_wait_s"] <= s["p99_latency_s"]
+    assert s["p50_latency_s"] <= s["p99_latency_s"]
+    assert s["aggregate_ns_per_day"] > 0
