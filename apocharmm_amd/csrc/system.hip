@@ -527,16 +527,20 @@ hipEvent_t NbSystem::prof_event() {
 
 void NbSystem::tic(int, hipStream_t s, hipEvent_t &a) {
   if (!prof_on) return;
-  if (ev_used + 2 > 4096) profile_flush();
   a = prof_event();
   APO_HIP(hipEventRecord(a, s));
+  prof_open++;
 }
 
+// Events are recycled only when no timer is open: an enclosing tic's event
+// must not be re-recorded before its toc.
 void NbSystem::toc(int slot, hipStream_t s, hipEvent_t a) {
   if (!prof_on) return;
   hipEvent_t b = prof_event();
   APO_HIP(hipEventRecord(b, s));
   prof_recs.push_back({slot, a, b});
+  if (prof_open > 0) prof_open--;
+  if (prof_open == 0 && ev_used > 4096) profile_flush();
 }
 
 void NbSystem::profile_flush() {
@@ -549,6 +553,7 @@ void NbSystem::profile_flush() {
   }
   prof_recs.clear();
   ev_used = 0;
+  prof_open = 0;
 }
 
 NbSystem::~NbSystem() {

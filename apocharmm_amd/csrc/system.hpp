@@ -114,6 +114,7 @@ struct NbSystem {
   bool prof_on = false;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
+  int prof_open = 0;  // tics not yet closed by their toc
   std::vector<TimerRec> prof_recs;
   double prof_ms[APO_PROFILE_SLOTS] = {0};
   long long prof_cnt[APO_PROFILE_SLOTS] = {0};

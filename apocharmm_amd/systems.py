@@ -147,15 +147,32 @@ def batched_system(i, n_waters):
 
 def _chain(n, rng, b=1.53, theta=math.radians(111.0), clash=3.0, L=None):
     """Self-avoiding random walk: fixed bond length and angle, uniform
-    torsions, rejection when a non-neighbour atom comes within `clash` A."""
+    torsions, rejection when a non-neighbour atom (more than three bonds away)
+    comes within `clash` A.  With a box edge L the check uses minimum-image
+    distances, so the chain does not overlap its own periodic images."""
     pos = np.zeros((n, 3))
     pos[1] = [b, 0, 0]
     pos[2] = pos[1] + b * np.array([-math.cos(theta), math.sin(theta), 0.0])
-    cell = clash
+    if L is None:
+        ncell, cell = None, clash
+    else:
+        ncell = max(3, int(L // clash))
+        cell = L / ncell
     grid = {}
 
     def key(p):
-        return tuple(np.floor(p / cell).astype(int))
+        if ncell is None:
+            return tuple(np.floor(p / cell).astype(int))
+        return tuple(np.floor(np.mod(p, L) / cell).astype(int) % ncell)
+
+    def near(kk):
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    c = (kk[0] + dx, kk[1] + dy, kk[2] + dz)
+                    if ncell is not None:
+                        c = (c[0] % ncell, c[1] % ncell, c[2] % ncell)
+                    yield c
 
     for k in range(3):
         grid.setdefault(key(pos[k]), []).append(k)
@@ -173,12 +190,14 @@ def _chain(n, rng, b=1.53, theta=math.radians(111.0), clash=3.0, L=None):
         p = c + b * (d[0] * bc + d[1] * m + d[2] * nrm)
         kk = key(p)
         ok = True
-        for dx in (-1, 0, 1):
-            for dy in (-1, 0, 1):
-                for dz in (-1, 0, 1):
-                    for j in grid.get((kk[0] + dx, kk[1] + dy, kk[2] + dz), ()):
-                        if j < k - 3 and np.sum((pos[j] - p) ** 2) < clash * clash:
-                            ok = False
+        for cc in set(near(kk)):
+            for j in grid.get(cc, ()):
+                if j < k - 3:
+                    dv = pos[j] - p
+                    if L is not None:
+                        dv -= L * np.round(dv / L)
+                    if np.sum(dv * dv) < clash * clash:
+                        ok = False
         tries += 1
         if not ok:
             if tries > 100:
@@ -201,7 +220,7 @@ def solvated_chain(n_chain=2489, n_waters=7023, L=62.23, fft_grid=64, seeds=(300
     n=3 k=0.2, q ~ N(0,0.3) shifted to zero, eps ~ U(0.02,0.2), Rmin/2 ~
     U(1,2)) in TIP3P water on lattice sites > 2.6 A from the chain."""
     rng = np.random.default_rng(seeds[0])
-    chain = _chain(n_chain, rng)
+    chain = _chain(n_chain, rng, L=L)
     chain += 0.5 * L - chain.mean(0)
     rq = np.random.default_rng(seeds[1])
     q = rq.normal(0.0, 0.3, n_chain)

@@ -36,6 +36,7 @@ SEED = 2003
 REBUILD_EVERY = 20
 FLOP_PER_PAIR = 61
 PEAK_FP32_TFLOPS = 157.3
+KBOLTZ = 1.987191e-3  # kcal/mol/K
 
 
 def count_pairs(spec, rc):
@@ -80,6 +81,128 @@ def cpu_baseline(spec, threads, md):
     return {"value": DT_PS * 86.4 / t_eval, "unit": "ns/day", "cores": threads, "kind": "port",
             "sample": "%d FP64 %s of the same system (oracle, cell list + PME, %d OpenMP "
                       "threads), %.3f s each; ns/day = dt*86.4/t_step" % (reps, what, threads, t_eval)}
+
+
+def copy_calibration():
+    """A plain copy of known size: 256 MiB read and 256 MiB written per launch,
+    three launches.  Its FETCH_SIZE / WRITE_SIZE readings scale the PMC
+    counters of the other kernels to bytes."""
+    import torch
+    n = 64 << 20
+    src = torch.ones(n, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read_per_launch": 4 * n, "copy_bytes_written_per_launch": 4 * n,
+                      "launches": 3}), flush=True)
+
+
+def pair_traffic(workload):
+    """HBM bytes per pair-kernel launch from the committed rocprofv3 PMC summary
+    of this workload (FETCH_SIZE and WRITE_SIZE in KiB, each scaled by the
+    reading of a plain copy of known size), or None if there is none."""
+    path = os.path.join(ROOT, "profiles", "k_pair_%s_pmc.md" % workload)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
+def bench_nve(a):
+    """Energy conservation: Langevin equilibration (friction --equil-friction,
+    --equil-steps steps), then NVE (friction 0) for --steps steps at 2 fs with
+    the same list and SETTLE settings.  H = U(x_n) + on-step KE(v_n) is
+    sampled every --sample-every steps; the drift is the least-squares slope
+    of H over simulated time, per degree of freedom."""
+    import numpy as np
+    import torch
+    from apocharmm_amd import systems
+    from apocharmm_amd.force_manager import ForceManager
+    from apocharmm_amd.integrator import LangevinThermostatIntegrator, maxwell_velocities
+
+    torch.cuda.set_device(0)
+    spec = {"c1": systems.config1, "c2": systems.config2, "c3": systems.config3}[a.config]()
+    n = len(spec["charges"])
+    fm = ForceManager(spec).initialize()
+    vseed = {"c1": 1002, "c2": 2002, "c3": 3005}[a.config]
+    eq = LangevinThermostatIntegrator(DT_PS, friction=a.equil_friction, temperature=TEMP, seed=SEED)
+    eq.setNonbondedListUpdateFrequency(REBUILD_EVERY)
+    eq.setCharmmContext(fm, np.asarray(spec["xyz"], np.float64),
+                        maxwell_velocities(spec["masses"], TEMP, vseed))
+    eq.initialize()
+    eq.propagate(a.equil_steps)
+    st = eq.getState()
+    t_eq = float(eq.getTemperature()[1])
+    eq.close()
+    lt = LangevinThermostatIntegrator(DT_PS, friction=0.0, temperature=TEMP, seed=SEED)
+    lt.setNonbondedListUpdateFrequency(REBUILD_EVERY)
+    lt.setCharmmContext(fm, st["xyz"], st["vel"])
+    lt.initialize()
+    H, U, K, steps = [], [], [], []
+    k = max(1, a.sample_every)
+    done = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    while True:
+        fm.calcForce(lt.getXYZQ(), calcEnergy=True)
+        u = fm.getPotentialEnergy()
+        lt.propagate(1)
+        ke = float(lt.getKineticEnergy()[1])
+        H.append(u + ke)
+        U.append(u)
+        K.append(ke)
+        steps.append(done)
+        done += 1
+        if done >= a.steps:
+            break
+        m = min(k - 1, a.steps - done)
+        if m > 0:
+            lt.propagate(m)
+            done += m
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    H, U, K = np.array(H), np.array(U), np.array(K)
+    t_ns = np.array(steps) * DT_PS / 1000.0
+    slope = float(np.polyfit(t_ns, H, 1)[0])
+    ndof = 3 * n - 3 - 3 * len(spec.get("waters", []))
+    kT = KBOLTZ * TEMP
+    out = {
+        "metric": "NVE total-energy drift, %s (%d steps at 2 fs after %d Langevin steps)" % (
+            a.config, a.steps, a.equil_steps),
+        "value": round(slope / ndof, 8),
+        "unit": "kcal/mol/ns per degree of freedom",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.equil_steps,
+        "ms_per_step": round(wall / a.steps * 1e3, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+int64",
+        "data": "synthetic (seeded system, Maxwell velocities seed %d)" % vseed,
+        "config": {"workload": "%s_nve_%d_atoms" % (a.config, n), "natom": n, "ndof": ndof,
+                   "dt_fs": 2.0, "list_rebuild_every": REBUILD_EVERY,
+                   "equilibration": {"steps": a.equil_steps, "friction_per_ps": a.equil_friction,
+                                     "T_end_K": round(t_eq, 2)},
+                   "sample_every": k, "parallelism": "single GPU"},
+        "nve": {"drift_kcal_per_mol_per_ns": round(slope, 4),
+                "drift_kT_per_ns_per_dof": round(slope / ndof / kT, 8),
+                "H_first": round(float(H[0]), 4), "H_last": round(float(H[-1]), 4),
+                "H_std": round(float(H.std()), 4), "U_std": round(float(U.std()), 4),
+                "KE_mean": round(float(K.mean()), 3),
+                "T_mean_K": round(float(2 * K.mean() / (ndof * KBOLTZ)), 2),
+                "samples": len(H), "simulated_ps": round(a.steps * DT_PS, 3),
+                "ns_per_day_incl_sampling": round(a.steps * DT_PS / 1000.0 / (wall / 86400.0), 2)},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    lt.close()
+    fm.close()
+    print(json.dumps(out), flush=True)
 
 
 def bench_batched(a):
@@ -226,8 +349,8 @@ def bench_serving(a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--config", default="c2")
     ap.add_argument("--precision", default="mixed")
     ap.add_argument("--step", choices=("md", "force"), default="md")
@@ -244,11 +367,24 @@ def main():
     ap.add_argument("--chunk", type=int, default=100, help="c5: steps per batch call")
     ap.add_argument("--slots", type=int, default=64, help="c5: most jobs on the GPU at once")
     ap.add_argument("--max-atoms", type=int, default=100000, help="c5: largest job size")
+    ap.add_argument("--nve", action="store_true",
+                    help="energy-conservation run: Langevin equilibration, then --steps NVE steps")
+    ap.add_argument("--equil-steps", type=int, default=10000)
+    ap.add_argument("--equil-friction", type=float, default=5.0)
+    ap.add_argument("--sample-every", type=int, default=500)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 256 MiB device-to-device copy (PMC byte calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build runs one system per process on a "
               "single MI355X; --gpus %d is not supported" % a.gpus, file=sys.stderr)
         sys.exit(2)
+    if a.copy_calibration:
+        copy_calibration()
+        return
+    if a.nve:
+        bench_nve(a)
+        return
     if a.config == "c4":
         # hardware queues are fixed when the HIP runtime starts in this process
         os.environ["GPU_MAX_HW_QUEUES"] = str(min(max(a.hw_queues, 1), 32))
@@ -321,6 +457,11 @@ def main():
         thermo = {"T_K": round(float(th[2]), 2), "T_avg_K": round(float(th[4]), 2),
                   "steps_total": int(lt.getCurrentStep())}
     what = "NVT Langevin + SETTLE step" if md else "force evaluation only (no integrator)"
+    workload = "%s_tip3p_%d_atoms_pme%d" % (a.config, n, int(spec["fft_grid"][0]))
+    if not md:
+        workload += "_force"
+    if a.precision != "mixed":
+        workload += "_" + a.precision
     out = {
         "metric": "ns/day @98k-atom PME water (%s, 2 fs)" % ("NVT Langevin MD" if md else
                                                               "force path"),
@@ -335,7 +476,7 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32+int64" if a.precision == "mixed" else "fp64+int64",
         "data": "synthetic (seeded TIP3P lattice, random orientations, Maxwell velocities)",
-        "config": {"workload": "%s_tip3p_%d_atoms_pme%d" % (a.config, n, int(spec["fft_grid"][0])),
+        "config": {"workload": workload,
                    "natom": n, "box_A": float(spec["box"][0]),
                    "pme_grid": [int(v) for v in spec["fft_grid"]],
                    "cutoffs_A": [float(spec["ctonnb"]), float(spec["ctofnb"]), float(spec["cutnb"])],
@@ -350,7 +491,8 @@ def main():
         "roofline": {"bound": "mfma", "engine": "fp32 VALU (no dense contraction on this path)",
                      "kernel": "k_pair (direct-space pair kernel)", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_TFLOPS, 5), "traffic": None,
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
+                     "traffic": pair_traffic(workload),
                      "work": "%d pairs inside r_off x %d flop" % (pairs, FLOP_PER_PAIR)},
     }
     if not a.no_cpu_baseline:
