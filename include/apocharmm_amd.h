@@ -288,15 +288,18 @@ APO_API apo_status apo_lt_initialize(apo_lt_integrator *lt, void *stream);
  * centering and list rebuild every nbfreq steps, centre-of-mass removal every
  * comfreq steps, then per step force, Langevin leapfrog update, SETTLE,
  * velocities, kinetic energy and the float xyzq.  Enqueued after the work
- * already on `stream`; the host waits only at list rebuilds. */
+ * already on `stream`.  The host does not wait: list capacities come from the
+ * first build, and a later build's counts are checked before the next rebuild
+ * (growing the buffers after draining the device, or failing with the
+ * build's error). */
 APO_API apo_status apo_lt_propagate(apo_lt_integrator *lt, int32_t nsteps,
                                     void *stream);
 
 /* Advances n independent integrators (each bound to its own force system) by
- * nsteps together: step k of every system is queued before step k+1 of any,
- * each system on its own stream, so their kernels overlap on the GPU and no
- * host synchronisation happens between list rebuilds.  Each system's
- * trajectory is bitwise the one apo_lt_propagate gives it alone.  No
+ * nsteps together through a one-off apo_lt_batch (below): one graph launch per
+ * batch step runs every system's step in fused launches, with no host
+ * synchronisation.  Each system's trajectory is bitwise the one
+ * apo_lt_propagate gives it alone.  No
  * counterpart in the reference, which runs one CharmmContext per process; it
  * serves many-system workloads (replica exchange, FEP windows, separate
  * jobs). */
