@@ -201,15 +201,11 @@ struct SortArgs {
   const float4 *xyzq;
   const int *vdwtype, *loc2glo, *meta;
   double3 inv_len;
+  float3 unit;
   int4 *posq;
   int *type_s;
   int64_t *fsort;
   int ss;
-};
-struct CellbbArgs {
-  const int4 *posq;
-  const int *loc2glo, *meta;
-  float3 unit;
   float4 *bb;
 };
 struct FinalizeArgs {
@@ -230,7 +226,6 @@ void nblist_build(NbSystem &sys, const float4 *d_xyzq, hipStream_t s, bool sizin
 void list_export_check(NbSystem &sys, hipStream_t s, bool allow_growth);
 void list_reserve(NbSystem &sys, int ncand, int nent, int nitem);
 void nblist_sort_coords(NbSystem &sys, const float4 *d_xyzq, hipStream_t s);
-void nblist_cellbb(NbSystem &sys, hipStream_t s);
 void nblist_stats(NbSystem &sys, int64_t *out6, hipStream_t s);
 
 void direct_launch(NbSystem &sys, bool energy, bool virial, hipStream_t s);

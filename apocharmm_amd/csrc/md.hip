@@ -32,8 +32,6 @@ void list_export_check(NbSystem &S, hipStream_t s, bool allow_growth);
 // fused (multi-system) launches of the step kernels, one per kernel type
 void batch_add_sort(BatchLaunch<SortArgs> &b, NbSystem &S, const float4 *x);
 void batch_sort(BatchLaunch<SortArgs> &b, hipStream_t s);
-void batch_add_cellbb(BatchLaunch<CellbbArgs> &b, NbSystem &S);
-void batch_cellbb(BatchLaunch<CellbbArgs> &b, hipStream_t s);
 void batch_add_pair(BatchLaunch<PairArgs> &b, NbSystem &S, int blocks);
 void batch_pair(BatchLaunch<PairArgs> &b, bool tab_lds, hipStream_t s);
 bool pair_tab_lds(const NbSystem &S);
@@ -879,7 +877,6 @@ struct LtBatch {
   BatchLaunch<FillArgs> fill;
   BatchLaunch<SortArgs> sort;
   BatchLaunch<PairArgs> pair;
-  BatchLaunch<CellbbArgs> cellbb;
   BatchLaunch<FinalizeArgs> fin;
   BatchLaunch<LtStepArgs> lstep;
   BatchLaunch<LtFinArgs> lfin;
@@ -911,7 +908,6 @@ struct LtBatch {
     fill.clear();
     sort.clear();
     pair.clear();
-    cellbb.clear();
     fin.clear();
     lstep.clear();
     lfin.clear();
@@ -930,7 +926,6 @@ struct LtBatch {
       const int pb = std::max(8, (int)std::lround(kPairBlocksPerCu * ncu * (L->n / ntot)));
       batch_add_pair(pair, S, pb);
       tab_lds = tab_lds && pair_tab_lds(S);
-      batch_add_cellbb(cellbb, S);
       pme_batch_add(*pme, S, g, L->gstride);
       bonded_batch_add(*bond, S, x, g, L->gstride);
       batch_add_finalize(fin, S, g, L->gstride);
@@ -942,7 +937,6 @@ struct LtBatch {
     fill.upload(m);
     sort.upload(m);
     pair.upload(m);
-    cellbb.upload(m);
     fin.upload(m);
     lstep.upload(m);
     lfin.upload(m);
@@ -953,12 +947,11 @@ struct LtBatch {
     for (size_t i = 0; i < sys.size(); i++) tab_gens[i] = sys[i]->nb->generation;
   }
 
-  // force path of every system + integrator update: 16 launches in all
+  // force path of every system + integrator update: 15 launches in all
   void fused_step(hipStream_t s) {
     batch_fill_i64(fill, s);
     batch_sort(sort, s);
     batch_pair(pair, tab_lds, s);
-    batch_cellbb(cellbb, s);
     pme_batch_launch(*pme, s);
     bonded_batch_launch(*bond, s);
     batch_finalize(fin, s);

@@ -149,17 +149,23 @@ __global__ __launch_bounds__(256) void k_colsort(const unsigned long long *__res
 
 // Per-step gather of xyzq into cell order as 32-bit fixed-point coordinates
 // (setSortedCoords, src/CudaPMEDirectForce.cu:1389-1400), fused with the
-// clear of the sorted force buffer.
+// clear of the sorted force buffer and the cell bounding boxes (centre and
+// half extent in A of the fixed-point coordinates; the list build and the
+// spread read them).  A block of 256 slots covers 8 whole cells, so each
+// cell's box is a reduction over one half wave.
 __device__ __forceinline__ void sort_coords_body(int bid, const float4 *__restrict__ xyzq,
                                                  const int *__restrict__ vdwtype,
                                                  const int *__restrict__ loc2glo,
                                                  const int *__restrict__ meta, double3 inv_len,
-                                                 float qscale, int4 *__restrict__ posq,
+                                                 float qscale, float3 unit,
+                                                 int4 *__restrict__ posq,
                                                  int *__restrict__ type_s,
-                                                 int64_t *__restrict__ fsort, int ss) {
-  int loc = bid * blockDim.x + threadIdx.x;
-  if (loc >= meta[0] * kCell) return;
-  int g = loc2glo[loc];
+                                                 int64_t *__restrict__ fsort, int ss,
+                                                 float4 *__restrict__ bb) {
+  const int loc = bid * 256 + threadIdx.x;
+  const int ncell = meta[0];
+  const bool in = loc < ncell * kCell;
+  const int g = in ? loc2glo[loc] : -1;
   int4 o = make_int4(0, 0, 0, 0);
   int ty = 0;
   if (g >= 0) {
@@ -170,44 +176,16 @@ __device__ __forceinline__ void sort_coords_body(int bid, const float4 *__restri
     o.w = __float_as_int(p.w * qscale);
     ty = vdwtype[g];
   }
-  posq[loc] = o;
-  type_s[loc] = ty;
-  fsort[loc] = 0;
-  fsort[ss + loc] = 0;
-  fsort[2 * (size_t)ss + loc] = 0;
-}
-__global__ void k_sort_coords(const float4 *__restrict__ xyzq, const int *__restrict__ vdwtype,
-                              const int *__restrict__ loc2glo, const int *__restrict__ meta,
-                              double3 inv_len, float qscale, int4 *__restrict__ posq,
-                              int *__restrict__ type_s, int64_t *__restrict__ fsort, int ss) {
-  sort_coords_body(blockIdx.x, xyzq, vdwtype, loc2glo, meta, inv_len, qscale, posq, type_s, fsort,
-                   ss);
-}
-__global__ __launch_bounds__(256) void kb_sort_coords(const SortArgs *__restrict__ tab,
-                                                      const int *__restrict__ bstart, int nsys) {
-  const int s = batch_system(bstart, nsys);
-  const SortArgs a = tab[s];
-  sort_coords_body(blockIdx.x - bstart[s], a.xyzq, a.vdwtype, a.loc2glo, a.meta, a.inv_len, 1.0f,
-                   a.posq, a.type_s, a.fsort, a.ss);
-}
-
-// 5. cell bounding boxes from the fixed-point coordinates (two cells per wave).
-__device__ __forceinline__ void cellbb_body(int bid, const int4 *__restrict__ posq,
-                                            const int *__restrict__ loc2glo,
-                                            const int *__restrict__ meta, float3 unit,
-                                            float4 *__restrict__ bb) {
-  const int lane = threadIdx.x;
-  const int cell = bid * 2 + (lane >> 5);
-  const bool cv = cell < meta[0];
-  const int loc = cell * kCell + (lane & 31);
-  bool valid = cv && loc2glo[loc] >= 0;
-  float x = 0, y = 0, z = 0;
-  if (valid) {
-    int4 p = posq[loc];
-    x = (float)(uint32_t)p.x * unit.x;
-    y = (float)(uint32_t)p.y * unit.y;
-    z = (float)(uint32_t)p.z * unit.z;
+  if (in) {
+    posq[loc] = o;
+    type_s[loc] = ty;
+    fsort[loc] = 0;
+    fsort[ss + loc] = 0;
+    fsort[2 * (size_t)ss + loc] = 0;
   }
+  const bool valid = g >= 0;
+  const float x = (float)(uint32_t)o.x * unit.x, y = (float)(uint32_t)o.y * unit.y,
+              z = (float)(uint32_t)o.z * unit.z;
   float mnx = valid ? x : 3e38f, mny = valid ? y : 3e38f, mnz = valid ? z : 3e38f;
   float mxx = valid ? x : -3e38f, mxy = valid ? y : -3e38f, mxz = valid ? z : -3e38f;
   for (int off = 16; off > 0; off >>= 1) {
@@ -218,22 +196,30 @@ __device__ __forceinline__ void cellbb_body(int bid, const int4 *__restrict__ po
     mxy = fmaxf(mxy, __shfl_xor(mxy, off));
     mxz = fmaxf(mxz, __shfl_xor(mxz, off));
   }
-  if (cv && (lane & 31) == 0) {
+  const int cell = loc >> 5;
+  if (cell < ncell && (loc & 31) == 0) {
     bb[2 * cell] = make_float4(0.5f * (mnx + mxx), 0.5f * (mny + mxy), 0.5f * (mnz + mxz), 0.f);
     bb[2 * cell + 1] = make_float4(0.5f * (mxx - mnx), 0.5f * (mxy - mny), 0.5f * (mxz - mnz), 0.f);
   }
 }
-__global__ __launch_bounds__(64) void k_cellbb(const int4 *__restrict__ posq,
-                                               const int *__restrict__ loc2glo,
-                                               const int *__restrict__ meta, float3 unit,
-                                               float4 *__restrict__ bb) {
-  cellbb_body(blockIdx.x, posq, loc2glo, meta, unit, bb);
+__global__ __launch_bounds__(256) void k_sort_coords(const float4 *__restrict__ xyzq,
+                                                     const int *__restrict__ vdwtype,
+                                                     const int *__restrict__ loc2glo,
+                                                     const int *__restrict__ meta, double3 inv_len,
+                                                     float qscale, float3 unit,
+                                                     int4 *__restrict__ posq,
+                                                     int *__restrict__ type_s,
+                                                     int64_t *__restrict__ fsort, int ss,
+                                                     float4 *__restrict__ bb) {
+  sort_coords_body(blockIdx.x, xyzq, vdwtype, loc2glo, meta, inv_len, qscale, unit, posq, type_s,
+                   fsort, ss, bb);
 }
-__global__ __launch_bounds__(64) void kb_cellbb(const CellbbArgs *__restrict__ tab,
-                                                const int *__restrict__ bstart, int nsys) {
+__global__ __launch_bounds__(256) void kb_sort_coords(const SortArgs *__restrict__ tab,
+                                                      const int *__restrict__ bstart, int nsys) {
   const int s = batch_system(bstart, nsys);
-  const CellbbArgs a = tab[s];
-  cellbb_body(blockIdx.x - bstart[s], a.posq, a.loc2glo, a.meta, a.unit, a.bb);
+  const SortArgs a = tab[s];
+  sort_coords_body(blockIdx.x - bstart[s], a.xyzq, a.vdwtype, a.loc2glo, a.meta, a.inv_len, 1.0f,
+                   a.unit, a.posq, a.type_s, a.fsort, a.ss, a.bb);
 }
 
 __device__ __forceinline__ float bb_gap2(float4 ci, float4 hi, float4 cj, float4 hj, float3 L) {
@@ -511,10 +497,7 @@ void nblist_build(NbSystem &S, const float4 *x, hipStream_t s, bool sizing) {
   k_colsort<<<S.ncol, 256, 0, s>>>(kb, S.col_count.get(), S.col_start.get(), S.col_cellstart.get(),
                                    S.loc2glo.get(), S.glo2loc.get(), meta);
   APO_LAUNCH_CHECK();
-  nblist_sort_coords(S, x, s);
-  k_cellbb<<<nblk(S.ncell_cap, 2), 64, 0, s>>>(S.posq.get(), S.loc2glo.get(), meta, unit,
-                                                S.cell_bb.get());
-  APO_LAUNCH_CHECK();
+  nblist_sort_coords(S, x, s);  // also the cell boxes
   k_tiles<<<S.ncell_cap, 64, 0, s>>>(S.cell_bb.get(), meta, Lf, rl2, S.icell_cnt.get(), nullptr,
                                      nullptr, nullptr, 0, 0);
   APO_LAUNCH_CHECK();
@@ -576,20 +559,15 @@ void nblist_build(NbSystem &S, const float4 *x, hipStream_t s, bool sizing) {
   S.list_valid = true;
 }
 
-void nblist_cellbb(NbSystem &S, hipStream_t s) {
-  const float3 unit = make_float3((float)(S.box[0] / 4294967296.0), (float)(S.box[1] / 4294967296.0),
-                                  (float)(S.box[2] / 4294967296.0));
-  k_cellbb<<<nblk(S.ncell_cap, 2), 64, 0, s>>>(S.posq.get(), S.loc2glo.get(), S.meta.get(), unit,
-                                                S.cell_bb.get());
-  APO_LAUNCH_CHECK();
-}
-
 void nblist_sort_coords(NbSystem &S, const float4 *x, hipStream_t s) {
   const double3 inv = make_double3(1.0 / S.box[0], 1.0 / S.box[1], 1.0 / S.box[2]);
   const float qscale = 1.0f;  // raw charge; ccelec is applied per tile in the pair kernel
+  const float3 unit = make_float3((float)(S.box[0] / 4294967296.0), (float)(S.box[1] / 4294967296.0),
+                                  (float)(S.box[2] / 4294967296.0));
   k_sort_coords<<<nblk(S.ssort, 256), 256, 0, s>>>(x, S.vdwtype.get(), S.loc2glo.get(),
-                                                    S.meta.get(), inv, qscale, S.posq.get(),
-                                                    S.type_s.get(), S.fsort.get(), S.ssort);
+                                                    S.meta.get(), inv, qscale, unit, S.posq.get(),
+                                                    S.type_s.get(), S.fsort.get(), S.ssort,
+                                                    S.cell_bb.get());
   APO_LAUNCH_CHECK();
 }
 
@@ -629,25 +607,15 @@ void nblist_stats(NbSystem &S, int64_t *out6, hipStream_t s) {
 // ---- fused launches over several systems ------------------------------------
 void batch_add_sort(BatchLaunch<SortArgs> &b, NbSystem &S, const float4 *x) {
   SortArgs a{x, S.vdwtype.get(), S.loc2glo.get(), S.meta.get(),
-             make_double3(1.0 / S.box[0], 1.0 / S.box[1], 1.0 / S.box[2]), S.posq.get(),
-             S.type_s.get(), S.fsort.get(), S.ssort};
+             make_double3(1.0 / S.box[0], 1.0 / S.box[1], 1.0 / S.box[2]),
+             make_float3((float)(S.box[0] / 4294967296.0), (float)(S.box[1] / 4294967296.0),
+                         (float)(S.box[2] / 4294967296.0)),
+             S.posq.get(), S.type_s.get(), S.fsort.get(), S.ssort, S.cell_bb.get()};
   b.add(a, nblk(S.ssort, 256));
 }
 void batch_sort(BatchLaunch<SortArgs> &b, hipStream_t s) {
   if (!b.blocks()) return;
   kb_sort_coords<<<b.blocks(), 256, 0, s>>>(b.d_args.get(), b.d_start.get(), b.nsys());
-  APO_LAUNCH_CHECK();
-}
-void batch_add_cellbb(BatchLaunch<CellbbArgs> &b, NbSystem &S) {
-  CellbbArgs a{S.posq.get(), S.loc2glo.get(), S.meta.get(),
-               make_float3((float)(S.box[0] / 4294967296.0), (float)(S.box[1] / 4294967296.0),
-                           (float)(S.box[2] / 4294967296.0)),
-               S.cell_bb.get()};
-  b.add(a, nblk(S.ncell_cap, 2));
-}
-void batch_cellbb(BatchLaunch<CellbbArgs> &b, hipStream_t s) {
-  if (!b.blocks()) return;
-  kb_cellbb<<<b.blocks(), 64, 0, s>>>(b.d_args.get(), b.d_start.get(), b.nsys());
   APO_LAUNCH_CHECK();
 }
 

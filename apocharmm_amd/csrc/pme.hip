@@ -186,6 +186,12 @@ struct GridGeom {
 // whose (per-step) bounding box reaches the brick; the brick is then stored
 // once as CT.  The integer sums are exact, so the grid is bitwise identical to
 // any order of accumulation, and no zero fill or HBM atomic is needed.
+//
+// Work split: each lane prepares one candidate atom (grid base index and
+// fractional offsets, FP64); the wave then walks the atoms that reach the
+// brick one at a time, lane l adding the stencil point (l & 3, (l >> 2) & 3,
+// l >> 4) of that atom, so the 64 LDS atomics of one instruction hit 64
+// distinct points.
 constexpr int kBx = 8, kBy = 8, kBz = 16;
 
 struct BrickGeom {
@@ -194,13 +200,23 @@ struct BrickGeom {
   float L0, L1, L2;
 };
 
-// Cyclic-interval overlap on a circle of length K.
-__device__ __forceinline__ bool cyc_overlap(float a, float la, float b, float lb, float K) {
+// Cyclic-interval overlap on a circle of length K (iK = 1/K).  The reduced
+// offsets may be off by one rounding of K, which the callers' slack of two
+// grid points covers.
+__device__ __forceinline__ bool cyc_overlap(float a, float la, float b, float lb, float K,
+                                            float iK) {
   float d1 = b - a;
-  d1 -= K * floorf(d1 / K);
+  d1 -= K * floorf(d1 * iK);
   float d2 = a - b;
-  d2 -= K * floorf(d2 / K);
+  d2 -= K * floorf(d2 * iK);
   return d1 < la || d2 < lb;
+}
+
+// round-to-nearest-even of v as an integer, for |v| < 2^51: adding 1.5 * 2^52
+// leaves the rounded integer in the low mantissa bits
+__device__ __forceinline__ long long rint_i64(double v) {
+  const double magic = 6755399441055744.0;
+  return __double_as_longlong(v + magic) - __double_as_longlong(magic);
 }
 
 // fr = K frac(u + 1/2) from the 32-bit fixed-point box coordinate u.
@@ -213,6 +229,31 @@ __device__ __forceinline__ void frac_fixed(int ufix, int K, int &ix, double &w) 
   w = fr - (double)ix;
 }
 
+// 6 x the order-4 B-spline weight of stencil point t at offset w:
+// (1-w)^3, 4 - 6w^2 + 3w^3, 1 + 3w + 3w^2 - 3w^3, w^3 for t = 0..3 (the
+// closed form of the recursion in bspline4).
+struct Theta6 {
+  double c0, c1, c2, c3;
+};
+__device__ __forceinline__ Theta6 theta6_coef(int t) {
+  switch (t) {
+    case 0: return Theta6{1.0, -3.0, 3.0, -1.0};
+    case 1: return Theta6{4.0, 0.0, -6.0, 3.0};
+    case 2: return Theta6{1.0, 3.0, 3.0, -3.0};
+    default: return Theta6{0.0, 0.0, 0.0, 1.0};
+  }
+}
+__device__ __forceinline__ double theta6(const Theta6 &c, double w) {
+  return fma(fma(fma(c.c3, w, c.c2), w, c.c1), w, c.c0);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int k) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), k);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 template <typename CT>
 __device__ __forceinline__ void spread_body(int bid, const int4 *__restrict__ posq,
                                             const float4 *__restrict__ bb,
@@ -221,7 +262,7 @@ __device__ __forceinline__ void spread_body(int bid, const int4 *__restrict__ po
   __shared__ unsigned long long brick[kBx * kBy * kBz];
   __shared__ int cand[256];
   __shared__ int ncand;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int bxi = bid % g.nbx, byi = (bid / g.nbx) % g.nby, bzi = bid / (g.nbx * g.nby);
   const int x0 = bxi * kBx, y0 = byi * kBy, z0 = bzi * kBz;
   for (int k = tid; k < kBx * kBy * kBz; k += 256) brick[k] = 0ull;
@@ -229,6 +270,11 @@ __device__ __forceinline__ void spread_body(int bid, const int4 *__restrict__ po
   // atoms whose stencil base lies in [p0 - 3, p0 + B) reach the brick
   const float bx0 = (float)(x0 - 3), by0 = (float)(y0 - 3), bz0 = (float)(z0 - 3);
   const float K0 = (float)g.K0, K1 = (float)g.K1, K2 = (float)g.K2;
+  const float iK0 = 1.f / K0, iK1 = 1.f / K1, iK2 = 1.f / K2;
+  const float gx = K0 / g.L0, gy = K1 / g.L1, gz = K2 / g.L2;  // grid points per A
+  // this lane's stencil point and its weight polynomials
+  const int sa = lane & 3, sb = (lane >> 2) & 3, sc = lane >> 4;
+  const Theta6 cx = theta6_coef(sa), cy = theta6_coef(sb), cz = theta6_coef(sc);
   for (int c0 = 0; c0 < ncell; c0 += 256) {
     if (tid == 0) ncand = 0;
     __syncthreads();
@@ -237,55 +283,54 @@ __device__ __forceinline__ void spread_body(int bid, const int4 *__restrict__ po
       const float4 cc = bb[2 * c], hh = bb[2 * c + 1];
       // cell extent in grid units (x/L + 1/2 origin), +2 for the floor and
       // single-precision slack
-      float ax = K0 * ((cc.x - hh.x) / g.L0 + 0.5f), lax = 2.f * hh.x * K0 / g.L0 + 2.f;
-      float ay = K1 * ((cc.y - hh.y) / g.L1 + 0.5f), lay = 2.f * hh.y * K1 / g.L1 + 2.f;
-      float az = K2 * ((cc.z - hh.z) / g.L2 + 0.5f), laz = 2.f * hh.z * K2 / g.L2 + 2.f;
-      if (cyc_overlap(ax - 1.f, lax, bx0, (float)(kBx + 3), K0) &&
-          cyc_overlap(ay - 1.f, lay, by0, (float)(kBy + 3), K1) &&
-          cyc_overlap(az - 1.f, laz, bz0, (float)(kBz + 3), K2)) {
+      float ax = fmaf(cc.x - hh.x, gx, 0.5f * K0), lax = fmaf(2.f * hh.x, gx, 2.f);
+      float ay = fmaf(cc.y - hh.y, gy, 0.5f * K1), lay = fmaf(2.f * hh.y, gy, 2.f);
+      float az = fmaf(cc.z - hh.z, gz, 0.5f * K2), laz = fmaf(2.f * hh.z, gz, 2.f);
+      if (cyc_overlap(ax - 1.f, lax, bx0, (float)(kBx + 3), K0, iK0) &&
+          cyc_overlap(ay - 1.f, lay, by0, (float)(kBy + 3), K1, iK1) &&
+          cyc_overlap(az - 1.f, laz, bz0, (float)(kBz + 3), K2, iK2)) {
         int slot = atomicAdd(&ncand, 1);
         cand[slot] = c;
       }
     }
     __syncthreads();
-    const int nc = ncand;
-    for (int w = tid; w < nc * kCell; w += 256) {
-      const int loc = cand[w >> 5] * kCell + (w & 31);
-      const int4 p = posq[loc];
-      const double q = (double)__int_as_float(p.w) * qunscale;
-      if (q == 0.0) continue;
-      int ix, iy, iz;
-      double wx, wy, wz;
-      frac_fixed(p.x, g.K0, ix, wx);
-      frac_fixed(p.y, g.K1, iy, wy);
-      frac_fixed(p.z, g.K2, iz, wz);
-      // offsets of the stencil base relative to the brick origin, in [-3, K-4]
-      int dx = ix - x0, dy = iy - y0, dz = iz - z0;
-      dx -= g.K0 * (int)floorf((float)(dx + 3) / K0);
-      dy -= g.K1 * (int)floorf((float)(dy + 3) / K1);
-      dz -= g.K2 * (int)floorf((float)(dz + 3) / K2);
-      if (dx >= kBx || dy >= kBy || dz >= kBz) continue;
-      double tx[4], ty[4], tz[4], d[4];
-      bspline4<double>(wx, tx, d);
-      bspline4<double>(wy, ty, d);
-      bspline4<double>(wz, tz, d);
-      const double qs = q * kGridScale;
-#pragma unroll
-      for (int cz = 0; cz < 4; cz++) {
-        const int pz = dz + cz;
-        if (pz < 0 || pz >= kBz) continue;
-#pragma unroll
-        for (int cy = 0; cy < 4; cy++) {
-          const int py = dy + cy;
-          if (py < 0 || py >= kBy) continue;
-          const double qyz = qs * ty[cy] * tz[cz];
-#pragma unroll
-          for (int cx = 0; cx < 4; cx++) {
-            const int px = dx + cx;
-            if (px < 0 || px >= kBx) continue;
-            atomicAdd(&brick[(pz * kBy + py) * kBx + px],
-                      (unsigned long long)(long long)__builtin_rint(qyz * tx[cx]));
-          }
+    const int na = ncand * kCell;
+    for (int base = wave * 64; base < na; base += 256) {
+      // lane prepares atom base + lane
+      const int w = base + lane;
+      int dx = kBx, dy = kBy, dz = kBz;
+      double wx = 0.0, wy = 0.0, wz = 0.0, qs = 0.0;
+      if (w < na) {
+        const int4 p = posq[cand[w >> 5] * kCell + (w & 31)];
+        const double q = (double)__int_as_float(p.w) * qunscale;
+        if (q != 0.0) {
+          int ix, iy, iz;
+          frac_fixed(p.x, g.K0, ix, wx);
+          frac_fixed(p.y, g.K1, iy, wy);
+          frac_fixed(p.z, g.K2, iz, wz);
+          // offsets of the stencil base relative to the brick origin, in [-3, K-4]
+          dx = ix - x0;
+          dy = iy - y0;
+          dz = iz - z0;
+          dx -= g.K0 * (int)floorf((float)(dx + 3) / K0);
+          dy -= g.K1 * (int)floorf((float)(dy + 3) / K1);
+          dz -= g.K2 * (int)floorf((float)(dz + 3) / K2);
+          qs = q * (kGridScale / 216.0);
+        }
+      }
+      unsigned long long hit = __ballot(dx < kBx && dy < kBy && dz < kBz);
+      while (hit) {
+        const int k = __builtin_ctzll(hit);
+        hit &= hit - 1ull;
+        const int px = __builtin_amdgcn_readlane(dx, k) + sa;
+        const int py = __builtin_amdgcn_readlane(dy, k) + sb;
+        const int pz = __builtin_amdgcn_readlane(dz, k) + sc;
+        const double awx = readlane_f64(wx, k), awy = readlane_f64(wy, k),
+                     awz = readlane_f64(wz, k), aqs = readlane_f64(qs, k);
+        if (px >= 0 && px < kBx && py >= 0 && py < kBy && pz >= 0 && pz < kBz) {
+          const double v = aqs * theta6(cx, awx) * theta6(cy, awy) * theta6(cz, awz);
+          const long long iv = rint_i64(v);
+          if (iv) atomicAdd(&brick[(pz * kBy + py) * kBx + px], (unsigned long long)iv);
         }
       }
     }

@@ -490,7 +490,6 @@ void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int s
         APO_HIP(hipEventRecord(S.ev_sorted, s));
         APO_HIP(hipStreamWaitEvent(sp, S.ev_sorted, 0));
       }
-      nblist_cellbb(S, sp);  // only the spread reads the cell boxes
       recip_launch(S, x4, force, stride, E, V, sp);
       if (fork) APO_HIP(hipEventRecord(S.ev_pme, sp));
     }
