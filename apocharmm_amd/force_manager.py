@@ -98,8 +98,11 @@ class ForceManager:
         eps, rh = s["eps"], s["rmin_half"]
         t.ntypes = len(eps)
         _, t.vdwtype = i32(s["vdwtype"], 1)
-        t.vdwtab = f32(lj_table(eps, rh), 2)
-        t.vdwtab14 = f32(lj_table(s.get("eps14", eps), s.get("rmin_half14", rh)), 2)
+        # explicit (c6, c12) tables (NBFIX applied) take precedence over the
+        # combination rule
+        t.vdwtab = f32(s["vdwtab"] if "vdwtab" in s else lj_table(eps, rh), 2)
+        t.vdwtab14 = f32(s["vdwtab14"] if "vdwtab14" in s else
+                         lj_table(s.get("eps14", eps), s.get("rmin_half14", rh)), 2)
         t.nbond, t.bonds = i32(s["bonds"], 2)
         t.bond_par = f32(s["bond_par"], 2)
         t.nureyb, t.ureyb = i32(s["ureyb"], 2)

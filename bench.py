@@ -40,17 +40,54 @@ KBOLTZ = 1.987191e-3  # kcal/mol/K
 
 
 def count_pairs(spec, rc):
-    """Non-excluded minimum-image pairs with r < rc (host, periodic KD-tree)."""
+    """Non-excluded minimum-image pairs with r < rc (host, periodic KD-tree;
+    the 1-2/1-3/1-4 exclusions from the product's own derivation,
+    apo_topology_exclusions)."""
+    import ctypes
     import numpy as np
     from scipy.spatial import cKDTree
+    from apocharmm_amd import _lib
     L = float(spec["box"][0])
     x = np.mod(spec["xyz"], L)
     tree = cKDTree(x, boxsize=L)
-    n_ordered = tree.count_neighbors(tree, rc)
-    pairs = (n_ordered - len(x)) // 2
-    # topological exclusions of rigid waters (O-H1, O-H2, H1-H2) all lie inside rc
-    n_excl = 3 * len(spec.get("waters", []))
+    n = len(x)
+    pairs = (tree.count_neighbors(tree, rc) - n) // 2
+    b = np.ascontiguousarray(np.asarray(spec.get("bonds", np.zeros((0, 2))), np.int32).reshape(-1, 2))
+    counts = (ctypes.c_int32 * 3)()
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    lib = _lib.lib()
+    _lib.check(lib.apo_topology_exclusions(n, len(b), vp(b), counts, None, None, None, None))
+    ptr = np.zeros(n + 1, np.int32)
+    idx = np.zeros(max(counts[0], 1), np.int32)
+    in14 = np.zeros(max(2 * counts[1], 2), np.int32)
+    ex14 = np.zeros(max(2 * counts[2], 2), np.int32)
+    _lib.check(lib.apo_topology_exclusions(n, len(b), vp(b), counts, vp(ptr), vp(idx), vp(in14),
+                                           vp(ex14)))
+    i = np.repeat(np.arange(n), np.diff(ptr))
+    j = idx[: counts[0]]
+    keep = i < j
+    d = x[i[keep]] - x[j[keep]]
+    d -= L * np.round(d / L)
+    n_excl = int((np.sum(d * d, 1) < rc * rc).sum())
     return int(pairs - n_excl)
+
+
+VSEED = {"c1": 1002, "c2": 2002, "c3": 3005, "jac": 5003}
+CHARMM_DIR = os.path.join(ROOT, "tests", "golden", "charmm")
+
+
+def config_spec(name):
+    """c1-c3: the synthetic TIP3P / solvated-chain systems; jac: the JAC DHFR
+    benchmark system (23,558 atoms, CHARMM22 + TIP3P, box 62.23 A, PME 64^3)
+    read from its PSF/CRD and parameter files."""
+    from apocharmm_amd import systems
+    if name == "jac":
+        from apocharmm_amd import charmm_io
+        f = lambda n: os.path.join(CHARMM_DIR, n + ".gz")
+        return charmm_io.charmm_system(f("jac_5dhfr.psf"),
+                                       [f("par_all22_prot.prm"), f("toppar_water_ions.str")],
+                                       f("jac_5dhfr.crd"), (62.23,) * 3, 64, name="jac_5dhfr")
+    return {"c1": systems.config1, "c2": systems.config2, "c3": systems.config3}[name]()
 
 
 def cpu_baseline(spec, threads, md):
@@ -125,10 +162,10 @@ def bench_nve(a):
     from apocharmm_amd.integrator import LangevinThermostatIntegrator, maxwell_velocities
 
     torch.cuda.set_device(0)
-    spec = {"c1": systems.config1, "c2": systems.config2, "c3": systems.config3}[a.config]()
+    spec = config_spec(a.config)
     n = len(spec["charges"])
     fm = ForceManager(spec).initialize()
-    vseed = {"c1": 1002, "c2": 2002, "c3": 3005}[a.config]
+    vseed = VSEED[a.config]
     eq = LangevinThermostatIntegrator(DT_PS, friction=a.equil_friction, temperature=TEMP, seed=SEED)
     eq.setNonbondedListUpdateFrequency(REBUILD_EVERY)
     eq.setCharmmContext(fm, np.asarray(spec["xyz"], np.float64),
@@ -185,7 +222,7 @@ def bench_nve(a):
         "dtype": "fp32+int64",
         "data": "synthetic (seeded system, Maxwell velocities seed %d)" % vseed,
         "config": {"workload": "%s_nve_%d_atoms" % (a.config, n), "natom": n, "ndof": ndof,
-                   "dt_fs": 2.0, "list_rebuild_every": REBUILD_EVERY,
+                   "dt_fs": DT_PS * 1000.0, "list_rebuild_every": REBUILD_EVERY,
                    "equilibration": {"steps": a.equil_steps, "friction_per_ps": a.equil_friction,
                                      "T_end_K": round(t_eq, 2)},
                    "sample_every": k, "parallelism": "single GPU"},
@@ -351,7 +388,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--config", default="c2")
+    ap.add_argument("--config", default="c2", help="c1 | c2 | c3 | c4 | c5 | jac")
+    ap.add_argument("--dt-fs", type=float, default=2.0, help="integrator time step (fs)")
     ap.add_argument("--precision", default="mixed")
     ap.add_argument("--step", choices=("md", "force"), default="md")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -375,6 +413,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device-to-device copy (PMC byte calibration)")
     a = ap.parse_args()
+    global DT_PS
+    DT_PS = a.dt_fs / 1000.0
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build runs one system per process on a "
               "single MI355X; --gpus %d is not supported" % a.gpus, file=sys.stderr)
@@ -404,13 +444,13 @@ def main():
     from apocharmm_amd.force_manager import ForceManager
     from apocharmm_amd.integrator import LangevinThermostatIntegrator, maxwell_velocities
 
-    spec = {"c1": systems.config1, "c2": systems.config2, "c3": systems.config3}[a.config]()
+    spec = config_spec(a.config)
     n = len(spec["charges"])
     torch.cuda.set_device(0)
     fm = ForceManager(spec, a.precision).initialize()
     md = a.step == "md"
     if md:
-        vseed = {"c1": 1002, "c2": 2002, "c3": 3005}[a.config]
+        vseed = VSEED[a.config]
         vel = maxwell_velocities(spec["masses"], TEMP, vseed)
         lt = LangevinThermostatIntegrator(DT_PS, friction=FRICTION, temperature=TEMP, seed=SEED)
         lt.setNonbondedListUpdateFrequency(REBUILD_EVERY)
@@ -457,7 +497,8 @@ def main():
         thermo = {"T_K": round(float(th[2]), 2), "T_avg_K": round(float(th[4]), 2),
                   "steps_total": int(lt.getCurrentStep())}
     what = "NVT Langevin + SETTLE step" if md else "force evaluation only (no integrator)"
-    workload = "%s_tip3p_%d_atoms_pme%d" % (a.config, n, int(spec["fft_grid"][0]))
+    workload = "%s_%d_atoms_pme%d" % ("jac_dhfr" if a.config == "jac" else a.config + "_tip3p", n,
+                                       int(spec["fft_grid"][0]))
     if not md:
         workload += "_force"
     if a.precision != "mixed":
@@ -480,7 +521,8 @@ def main():
                    "natom": n, "box_A": float(spec["box"][0]),
                    "pme_grid": [int(v) for v in spec["fft_grid"]],
                    "cutoffs_A": [float(spec["ctonnb"]), float(spec["ctofnb"]), float(spec["cutnb"])],
-                   "kappa": float(spec["kappa"]), "dt_fs": 2.0, "list_rebuild_every": REBUILD_EVERY,
+                   "kappa": float(spec["kappa"]), "dt_fs": DT_PS * 1000.0,
+                   "list_rebuild_every": REBUILD_EVERY,
                    "step": what, "friction_per_ps": FRICTION if md else None,
                    "temperature_K": TEMP if md else None, "precision": a.precision,
                    "parallelism": "single GPU"},

@@ -263,9 +263,15 @@ def prepare(spec, table_dtype=np.float32):
     for key in ("bond_par", "ub_par", "angle_par", "dihe_par", "imp_par"):
         if key in d and d[key] is not None:
             d[key] = np.asarray(d[key], np.float64).astype(np.float32).astype(np.float64)
+    # explicit (c6, c12) tables (NBFIX applied, float) take precedence over
+    # the combination rule
+    tab = (np.asarray(spec["vdwtab"], np.float64) if "vdwtab" in spec
+           else lj_table(spec["eps"], spec["rmin_half"], table_dtype))
+    tab14 = (np.asarray(spec["vdwtab14"], np.float64) if "vdwtab14" in spec
+             else lj_table(eps14, rh14, table_dtype))
     d.update(
-        vdwtab=lj_table(spec["eps"], spec["rmin_half"], table_dtype),
-        vdwtab14=lj_table(eps14, rh14, table_dtype),
+        vdwtab=tab,
+        vdwtab14=tab14,
         excl_ptr=ptr, excl_idx=idx, in14=in14, ex14=ex14,
         roff=float(spec["ctofnb"]), ron=float(spec["ctonnb"]),
         order=int(spec.get("order", 4)), e14fac=float(spec.get("e14fac", 1.0)),
