@@ -30,6 +30,7 @@ void batch_propagate(LtBatch &B, int nsteps, hipStream_t caller);
 void lt_get_state(LtSystem &L, double *xyz, double *vel, double *dprev, double *grad,
                   double *thermo, int64_t *step);
 const float *lt_xyzq(LtSystem &L);
+void lt_list_info(LtSystem &L, double *out4);
 }  // namespace apo
 
 struct apo_nb_system {
@@ -276,6 +277,13 @@ APO_API apo_status apo_lt_get_state(apo_lt_integrator *lt, double *xyz, double *
                                     double *dprev, double *grad, double *thermo6,
                                     int64_t *step) {
   return guard([&] { apo::lt_get_state(get(lt), xyz, vel, dprev, grad, thermo6, step); });
+}
+
+APO_API apo_status apo_lt_list_info(apo_lt_integrator *lt, double *out4) {
+  return guard([&] {
+    if (!out4) throw apo::InvalidArgument("null output");
+    apo::lt_list_info(get(lt), out4);
+  });
 }
 
 APO_API apo_status apo_lt_xyzq(apo_lt_integrator *lt, const float **d_xyzq) {

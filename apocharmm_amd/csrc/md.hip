@@ -173,6 +173,7 @@ struct LtState {
   int nwater;
   const int *singles;        // atoms in no water
   int nsingle;
+  const float4 *xref;        // coordinates of the last list build
 };
 
 struct LtCoef {
@@ -236,19 +237,36 @@ __device__ inline void lt_finish(const LtState &S, const LtCoef &C, int i, doubl
   store_xyzq(S, i, xn);
 }
 
+// Squared displacement since the last list build and over this step.
+__device__ inline void disp_track(const LtState &S, int i, double3 xr, double3 xn, float &d2,
+                                  float &s2) {
+  const float4 r = S.xref[i];
+  const float dx = (float)xn.x - r.x, dy = (float)xn.y - r.y, dz = (float)xn.z - r.z;
+  d2 = fmaxf(d2, dx * dx + dy * dy + dz * dz);
+  const double3 st = xn - xr;
+  s2 = fmaxf(s2, (float)dot3(st, st));
+}
+
 __device__ inline double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
   return v;
 }
 
 // One thread per water (first nwater threads) or free atom (the rest).
 __device__ __forceinline__ void lt_step_body(int bid, const LtState &S, const LtCoef &C,
                                              const unsigned long long *__restrict__ stepc,
-                                             double2 *__restrict__ part) {
+                                             double2 *__restrict__ part,
+                                             float2 *__restrict__ dpart) {
   const int u = bid * kStepBlock + threadIdx.x;
   const unsigned long long step = stepc[0];
   double ke0 = 0.0, ke1 = 0.0;
+  float d2 = 0.f, s2 = 0.f;
   if (u < S.nwater) {
     const int ia = S.waters[3 * u], ib = S.waters[3 * u + 1], ic = S.waters[3 * u + 2];
     const double3 ra = ld3(S.x, S.n, ia), rb = ld3(S.x, S.n, ib), rc = ld3(S.x, S.n, ic);
@@ -260,59 +278,99 @@ __device__ __forceinline__ void lt_step_body(int bid, const LtState &S, const Lt
     lt_finish(S, C, ia, ra, na, pa, ke0, ke1);
     lt_finish(S, C, ib, rb, nb, pb, ke0, ke1);
     lt_finish(S, C, ic, rc, nc, pc, ke0, ke1);
+    disp_track(S, ia, ra, na, d2, s2);
+    disp_track(S, ib, rb, nb, d2, s2);
+    disp_track(S, ic, rc, nc, d2, s2);
   } else if (u < S.nwater + S.nsingle) {
     const int i = S.singles[u - S.nwater];
     const double3 r = ld3(S.x, S.n, i), p = ld3(S.dp, S.n, i);
     const double3 xn = lt_position(S, C, step, i, r, p);
     lt_finish(S, C, i, r, xn, p, ke0, ke1);
+    disp_track(S, i, r, xn, d2, s2);
   }
   ke0 = wave_sum(ke0);
   ke1 = wave_sum(ke1);
-  if (threadIdx.x == 0) part[bid] = make_double2(ke0, ke1);
+  d2 = wave_max(d2);
+  s2 = wave_max(s2);
+  if (threadIdx.x == 0) {
+    part[bid] = make_double2(ke0, ke1);
+    dpart[bid] = make_float2(d2, s2);
+  }
 }
 __global__ __launch_bounds__(kStepBlock) void k_lt_step(LtState S, LtCoef C,
                                                         const unsigned long long *__restrict__ stepc,
-                                                        double2 *__restrict__ part) {
-  lt_step_body(blockIdx.x, S, C, stepc, part);
+                                                        double2 *__restrict__ part,
+                                                        float2 *__restrict__ dpart) {
+  lt_step_body(blockIdx.x, S, C, stepc, part, dpart);
 }
 struct LtStepArgs {
   LtState S;
   LtCoef C;
   const unsigned long long *stepc;
   double2 *part;
+  float2 *dpart;
 };
 __global__ __launch_bounds__(kStepBlock) void kb_lt_step(const LtStepArgs *__restrict__ tab,
                                                          const int *__restrict__ bstart, int nsys) {
   const int s = batch_system(bstart, nsys);
   const LtStepArgs &a = tab[s];
-  lt_step_body(blockIdx.x - bstart[s], a.S, a.C, a.stepc, a.part);
+  lt_step_body(blockIdx.x - bstart[s], a.S, a.C, a.stepc, a.part, a.dpart);
 }
 
 
 // Fixed-order kinetic energy sum, temperatures and their running means
 // (src/CudaLangevinThermostatIntegrator.cu:684-701), step counter + 1.
 // thermo: [0] ke (three-estimate mean) [1] ke (on-step) [2] T [3] T on-step
-//         [4] <T> [5] <T on-step>; counters: [0] step [1] averaging window.
+//         [4] <T> [5] <T on-step> [6] max displacement since the last list
+//         build (A) [7] max displacement of one step since that build (A);
+// counters: [0] step [1] averaging window [2] number of the list build that
+// [6], [7] refer to (the build's meta[16]).  [6], [7], the step count and the
+// build number also go to the host's pinned hdisp[0..3] (step count last).
 __device__ __forceinline__ void lt_finalize_body(const double2 *__restrict__ part, int npart,
                                                  double ndof, double *__restrict__ thermo,
-                                                 unsigned long long *__restrict__ counters) {
+                                                 unsigned long long *__restrict__ counters,
+                                                 const float2 *__restrict__ dpart,
+                                                 const int *__restrict__ meta,
+                                                 double *__restrict__ hdisp) {
   __shared__ double s0[256], s1[256];
+  __shared__ float m0[256], m1[256];
   double a = 0.0, b = 0.0;
+  float d2 = 0.f, s2 = 0.f;
   for (int k = threadIdx.x; k < npart; k += 256) {
     a += part[k].x;
     b += part[k].y;
+    d2 = fmaxf(d2, dpart[k].x);
+    s2 = fmaxf(s2, dpart[k].y);
   }
   s0[threadIdx.x] = a;
   s1[threadIdx.x] = b;
+  m0[threadIdx.x] = d2;
+  m1[threadIdx.x] = s2;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
       s0[threadIdx.x] += s0[threadIdx.x + w];
       s1[threadIdx.x] += s1[threadIdx.x + w];
+      m0[threadIdx.x] = fmaxf(m0[threadIdx.x], m0[threadIdx.x + w]);
+      m1[threadIdx.x] = fmaxf(m1[threadIdx.x], m1[threadIdx.x + w]);
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
+    const unsigned long long seq = (unsigned long long)meta[16];
+    if (counters[2] != seq) {
+      counters[2] = seq;
+      thermo[6] = 0.0;
+      thermo[7] = 0.0;
+    }
+    thermo[6] = fmax(thermo[6], sqrt((double)m0[0]));
+    thermo[7] = fmax(thermo[7], sqrt((double)m1[0]));
+    hdisp[0] = thermo[6];
+    hdisp[1] = thermo[7];
+    hdisp[3] = (double)seq;
+    __threadfence_system();
+    hdisp[2] = (double)(counters[0] + 1);
+    __threadfence_system();
     const double t0 = s0[0] / (0.5 * ndof * kBoltz), t1 = s1[0] / (0.5 * ndof * kBoltz);
     const double cnt = (double)(counters[1] + 1);
     thermo[0] = s0[0];
@@ -327,8 +385,11 @@ __device__ __forceinline__ void lt_finalize_body(const double2 *__restrict__ par
 }
 __global__ __launch_bounds__(256) void k_lt_finalize(const double2 *__restrict__ part, int npart,
                                                      double ndof, double *__restrict__ thermo,
-                                                     unsigned long long *__restrict__ counters) {
-  lt_finalize_body(part, npart, ndof, thermo, counters);
+                                                     unsigned long long *__restrict__ counters,
+                                                     const float2 *__restrict__ dpart,
+                                                     const int *__restrict__ meta,
+                                                     double *__restrict__ hdisp) {
+  lt_finalize_body(part, npart, ndof, thermo, counters, dpart, meta, hdisp);
 }
 struct LtFinArgs {
   const double2 *part;
@@ -336,11 +397,14 @@ struct LtFinArgs {
   double ndof;
   double *thermo;
   unsigned long long *counters;
+  const float2 *dpart;
+  const int *meta;
+  double *hdisp;
 };
 // one block per system
 __global__ __launch_bounds__(256) void kb_lt_finalize(const LtFinArgs *__restrict__ tab) {
   const LtFinArgs &a = tab[blockIdx.x];
-  lt_finalize_body(a.part, a.npart, a.ndof, a.thermo, a.counters);
+  lt_finalize_body(a.part, a.npart, a.ndof, a.thermo, a.counters, a.dpart, a.meta, a.hdisp);
 }
 
 
@@ -515,8 +579,20 @@ struct LtSystem {
   DevBuf<double2> ke_part;
   DevBuf<double4> com_part;
   int com_blocks = 0;
-  DevBuf<double> thermo;                 // 6 values, see k_lt_finalize
-  DevBuf<unsigned long long> counters;   // [0] step [1] averaging window
+  DevBuf<double> thermo;                 // 8 values, see k_lt_finalize
+  DevBuf<unsigned long long> counters;   // [0] step [1] averaging window [2] build number
+  DevBuf<float2> dpart;                  // per-block displacement maxima (squared)
+  double *h_disp = nullptr;              // pinned: see k_lt_finalize
+  // list rebuild policy: 0 = every nbfreq steps (the reference's cadence);
+  // 1 = also whenever an atom may have moved more than half the list buffer
+  // since the last build (at most nbfreq steps apart)
+  int list_policy = 0;
+  double half_buffer = 0.0;              // (cutnb - ctofnb) / 2
+  unsigned long long builds_triggered = 0;
+  // step completion events of policy 1 (the host looks at most kLag steps
+  // behind the GPU)
+  static constexpr int kLag = 4;
+  hipEvent_t step_ev[kLag] = {};
   unsigned long long step = 0;           // host mirror of counters[0]
   long steps_since_rebuild = 0;
   bool initialized = false;
@@ -530,6 +606,9 @@ struct LtSystem {
   ~LtSystem() {
     for (hipGraphExec_t g : gexec)
       if (g) (void)hipGraphExecDestroy(g);
+    for (hipEvent_t e : step_ev)
+      if (e) (void)hipEventDestroy(e);
+    if (h_disp) (void)hipHostFree(h_disp);
     if (ev_in) (void)hipEventDestroy(ev_in);
     if (ev_out) (void)hipEventDestroy(ev_out);
     if (st) (void)hipStreamDestroy(st);
@@ -550,6 +629,7 @@ struct LtSystem {
     s.nwater = nwater;
     s.singles = singles.get();
     s.nsingle = nsingle;
+    s.xref = nb->xref.get();
     return s;
   }
 
@@ -581,9 +661,10 @@ struct LtSystem {
   void update(hipStream_t s) {
     hipEvent_t t0 = nullptr;
     nb->tic(8, s, t0);
-    k_lt_step<<<nunit_blocks, kStepBlock, 0, s>>>(state(), C, counters.get(), ke_part.get());
+    k_lt_step<<<nunit_blocks, kStepBlock, 0, s>>>(state(), C, counters.get(), ke_part.get(),
+                                                  dpart.get());
     k_lt_finalize<<<1, 256, 0, s>>>(ke_part.get(), nunit_blocks, ndof, thermo.get(),
-                                    counters.get());
+                                    counters.get(), dpart.get(), nb->meta.get(), h_disp);
     APO_LAUNCH_CHECK();
     nb->toc(8, s, t0);
   }
@@ -622,6 +703,8 @@ struct LtSystem {
         ex = nullptr;
       }
       hipGraph_t g = nullptr;
+      // a build inside the graph counts when the graph is launched
+      const unsigned long long bq = nb->builds_queued;
       APO_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       try {
         if (which) rebuild_body(s);
@@ -629,8 +712,10 @@ struct LtSystem {
       } catch (...) {
         (void)hipStreamEndCapture(s, &g);
         if (g) (void)hipGraphDestroy(g);
+        nb->builds_queued = bq;
         throw;
       }
+      nb->builds_queued = bq;
       APO_HIP(hipStreamEndCapture(s, &g));
       hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
@@ -638,6 +723,7 @@ struct LtSystem {
       graph_gen[which] = nb->generation;
     }
     APO_HIP(hipGraphLaunch(ex, s));
+    if (which) nb->builds_queued++;
   }
 
   // CudaIntegrator::propagate (src/CudaIntegrator.cu:140-230) with the
@@ -655,9 +741,39 @@ struct LtSystem {
     steps_since_rebuild = 0;
   }
 
+  // Policy 1: whether the next step must rebuild the list.  The latest
+  // displacements the GPU exported (at most kLag steps old: the host waits
+  // for the step kLag back) plus the largest one-step displacement times the
+  // steps not yet seen must stay below half the list buffer, with a 10 %
+  // margin for the float coordinates of the check.
+  bool displacement_trigger(hipStream_t s) {
+    (void)s;
+    const unsigned long long k = step;  // the step about to be queued
+    if (k >= (unsigned long long)kLag) APO_HIP(hipEventSynchronize(step_ev[(k - kLag) % kLag]));
+    const volatile double *h = h_disp;
+    const double done = h[2];
+    const double dmax = h[0], smax = h[1], seq = h[3];
+    const double smax_est = std::max(smax, last_smax);
+    last_smax = smax_est;
+    const double unseen = (double)k - done + 1.0;
+    double pred;
+    if (seq == (double)nb->builds_queued)
+      pred = dmax + std::max(unseen, 1.0) * smax_est;
+    else  // the exported values belong to an earlier build
+      pred = (double)(steps_since_rebuild + 1) * smax_est;
+    last_pred = pred;
+    return pred > 0.9 * half_buffer;
+  }
+  double last_smax = 0.0, last_pred = 0.0;
+
   void one_step(hipStream_t s) {
     {
-      const bool rebuild = steps_since_rebuild % nbfreq == 0;
+      bool rebuild = steps_since_rebuild % nbfreq == 0;
+      if (list_policy == 1 && !rebuild && displacement_trigger(s)) {
+        rebuild = true;
+        builds_triggered++;
+        steps_since_rebuild = 0;
+      }
       const bool com = comfreq > 0 && step % (unsigned long long)comfreq == 0;
       // centre-of-mass removal only touches the half-step displacements, so
       // it may precede the force call
@@ -671,6 +787,7 @@ struct LtSystem {
         if (rebuild) list_export_check(*nb, s, true);
         replay(rebuild ? 1 : 0, s);
       }
+      if (list_policy == 1) APO_HIP(hipEventRecord(step_ev[step % kLag], s));
       steps_since_rebuild++;
       step++;
     }
@@ -822,10 +939,19 @@ LtSystem *create_lt(NbSystem *nb, const apo_lt_params &p, const double *xyz, con
     L->gatom.upload(gatom, s);
     L->ke_part.reserve(L->nunit_blocks);
     L->com_part.reserve(L->com_blocks + 1);
-    L->thermo.reserve(6);
-    launch_fill_f64(L->thermo.get(), 6, s);
-    std::vector<unsigned long long> c0 = {0ull, 0ull};
+    L->thermo.reserve(8);
+    launch_fill_f64(L->thermo.get(), 8, s);
+    std::vector<unsigned long long> c0 = {0ull, 0ull, ~0ull};
     L->counters.upload(c0, s);
+    L->dpart.reserve(L->nunit_blocks);
+    APO_HIP(hipHostMalloc(reinterpret_cast<void **>(&L->h_disp), 4 * sizeof(double)));
+    for (int k = 0; k < 4; k++) L->h_disp[k] = 0.0;
+    L->list_policy = p.list_policy;
+    if (L->list_policy != 0 && L->list_policy != 1)
+      throw InvalidArgument("list_policy must be 0 (fixed cadence) or 1 (displacement check)");
+    L->half_buffer = 0.5 * (nb->cutnb - nb->ctofnb);
+    for (hipEvent_t &e : L->step_ev)
+      APO_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     APO_HIP(hipStreamSynchronize(s));
   } catch (...) {
     delete L;
@@ -929,9 +1055,10 @@ struct LtBatch {
       pme_batch_add(*pme, S, g, L->gstride);
       bonded_batch_add(*bond, S, x, g, L->gstride);
       batch_add_finalize(fin, S, g, L->gstride);
-      lstep.add(LtStepArgs{L->state(), L->C, L->counters.get(), L->ke_part.get()}, L->nunit_blocks);
+      lstep.add(LtStepArgs{L->state(), L->C, L->counters.get(), L->ke_part.get(), L->dpart.get()},
+                L->nunit_blocks);
       lfin.add(LtFinArgs{L->ke_part.get(), L->nunit_blocks, L->ndof, L->thermo.get(),
-                         L->counters.get()},
+                         L->counters.get(), L->dpart.get(), S.meta.get(), L->h_disp},
                1);
     }
     fill.upload(m);
@@ -971,6 +1098,12 @@ struct LtBatch {
       ex[rb] = nullptr;
     }
     hipGraph_t g = nullptr;
+    // builds inside the graph count when it is launched
+    std::vector<unsigned long long> bq(sys.size());
+    for (size_t i = 0; i < sys.size(); i++) bq[i] = sys[i]->nb->builds_queued;
+    auto restore_bq = [&] {
+      for (size_t i = 0; i < sys.size(); i++) sys[i]->nb->builds_queued = bq[i];
+    };
     APO_HIP(hipStreamBeginCapture(m, hipStreamCaptureModeThreadLocal));
     try {
       if (rb) {
@@ -991,8 +1124,10 @@ struct LtBatch {
     } catch (...) {
       (void)hipStreamEndCapture(m, &g);
       if (g) (void)hipGraphDestroy(g);
+      restore_bq();
       throw;
     }
+    restore_bq();
     APO_HIP(hipStreamEndCapture(m, &g));
     hipError_t e = hipGraphInstantiate(&ex[rb], g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -1028,6 +1163,8 @@ struct LtBatch {
       for (int i = 0; i < n && !stale; i++) stale = gens[rb][i] != sys[i]->nb->generation;
       if (stale) capture(rb);
       APO_HIP(hipGraphLaunch(ex[rb], m));
+      if (rb)
+        for (LtSystem *L : sys) L->nb->builds_queued++;
       for (LtSystem *L : sys) {
         L->steps_since_rebuild++;
         L->step++;
@@ -1068,9 +1205,12 @@ LtBatch *create_batch(LtSystem **Ls, int n, int lanes) {
       APO_HIP(hipStreamCreateWithFlags(&B->lane[l], hipStreamNonBlocking));
       APO_HIP(hipEventCreateWithFlags(&B->lane_ev[l], hipEventDisableTiming));
     }
-    for (LtSystem *L : B->sys)
+    for (LtSystem *L : B->sys) {
       if (L->nb->precision != 0)
         throw InvalidArgument("a batch runs the mixed-precision force path only");
+      if (L->list_policy != 0)
+        throw InvalidArgument("a batch rebuilds lists at the fixed cadence (list_policy 0)");
+    }
     B->pme = pme_batch_create();
     B->bond = bonded_batch_create();
   } catch (...) {
@@ -1130,5 +1270,15 @@ void lt_get_state(LtSystem &L, double *xyz, double *vel, double *dprev, double *
 }
 
 const float *lt_xyzq(LtSystem &L) { return reinterpret_cast<const float *>(L.xyzq.get()); }
+
+void lt_list_info(LtSystem &L, double *out4) {
+  APO_HIP(hipStreamSynchronize(L.st));
+  double t[8];
+  APO_HIP(hipMemcpy(t, L.thermo.get(), sizeof(t), hipMemcpyDeviceToHost));
+  out4[0] = t[6];
+  out4[1] = t[7];
+  out4[2] = (double)L.builds_triggered;
+  out4[3] = L.half_buffer;
+}
 
 }  // namespace apo

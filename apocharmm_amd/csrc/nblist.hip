@@ -31,13 +31,16 @@ __device__ __forceinline__ uint32_t to_fixed(double x, double inv_len) {
 // that are neighbours in index order usually share a column, so the lanes of
 // a wave that hit the same column take their slots with one atomic (the
 // first such lane adds the group's size, the others take consecutive slots).
+// The coordinates the list is built from are kept (xref) for the integrator's
+// displacement check.
 __global__ void k_assign(const float4 *__restrict__ xyzq, int n, double3 inv_len, double3 len,
                          int ncx, int ncy, int *__restrict__ col_count, int *__restrict__ atom_col,
                          int *__restrict__ atom_slot, float *__restrict__ zkey,
-                         int *__restrict__ meta) {
+                         int *__restrict__ meta, float4 *__restrict__ xref) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float4 p = xyzq[i];
+  xref[i] = p;
   double fx = (double)p.x * inv_len.x, fy = (double)p.y * inv_len.y, fz = (double)p.z * inv_len.z;
   fx -= floor(fx);
   fy -= floor(fy);
@@ -482,7 +485,8 @@ void nblist_build(NbSystem &S, const float4 *x, hipStream_t s, bool sizing) {
   launch_fill_i32(meta, 16, s);
   launch_fill_i32(S.col_count.get(), S.ncol + 1, s);
   k_assign<<<nblk(n, 256), 256, 0, s>>>(x, n, inv, len, S.ncx, S.ncy, S.col_count.get(),
-                                         S.atom_col.get(), S.atom_slot.get(), S.zkey.get(), meta);
+                                         S.atom_col.get(), S.atom_slot.get(), S.zkey.get(), meta,
+                                         S.xref.get());
   APO_LAUNCH_CHECK();
   launch_scan(S.col_count.get(), S.col_start.get(), nullptr, S.ncol, s);
   k_col_prep<<<nblk(S.ncol, 256), 256, 0, s>>>(S.col_count.get(), S.ncol, S.col_ncell.get(), meta);
@@ -556,6 +560,7 @@ void nblist_build(NbSystem &S, const float4 *x, hipStream_t s, bool sizing) {
   APO_LAUNCH_CHECK();
   k_meta_export<<<1, 64, 0, s>>>(meta, S.h_meta);
   APO_LAUNCH_CHECK();
+  S.builds_queued++;
   S.list_valid = true;
 }
 

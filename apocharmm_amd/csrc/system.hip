@@ -390,6 +390,7 @@ NbSystem *create_system(const apo_nb_params &p, const apo_nb_topology &t) {
     S->fsort.reserve(3 * (size_t)S->ssort);
     S->loc2glo.reserve(S->ssort);
     S->glo2loc.reserve(n);
+    S->xref.reserve(n);
     launch_fill_i32(S->glo2loc.get(), n, 0);
     S->cell_bb.reserve(2 * (size_t)S->ncell_cap);
     S->meta.reserve(32);

@@ -80,6 +80,10 @@ struct NbSystem {
   // = build number) and checked lazily on the host
   int *h_meta = nullptr;
   int meta_checked = 0;
+  // list builds enqueued so far; the device numbers its builds the same way
+  // (meta[16], exported with the counts)
+  unsigned long long builds_queued = 0;
+  DevBuf<float4> xref;  // coordinates of the last list build (global order)
 
   // ---- per-step direct-space buffers -----------------------------------------
   DevBuf<int4> posq;       // fixed-point coords (x,y,z as uint32) + scaled q

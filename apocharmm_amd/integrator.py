@@ -38,6 +38,7 @@ class LangevinThermostatIntegrator:
         self._nbfreq = 20
         self._comfreq = 1000
         self._use_graph = True
+        self._list_policy = 0
         self._handle = ctypes.c_void_p()
         self._fm = None
         self._keep = []
@@ -60,6 +61,12 @@ class LangevinThermostatIntegrator:
 
     def setUseGraph(self, use):
         self._use_graph = bool(use)
+
+    def setListUpdatePolicy(self, policy):
+        """"fixed": rebuild every nbfreq steps (the reference's cadence);
+        "displacement": also when an atom may have moved more than half the
+        list buffer since the last build."""
+        self._list_policy = {"fixed": 0, "displacement": 1}[policy]
 
     def getTimeStep(self):
         return self._dt
@@ -95,6 +102,7 @@ class LangevinThermostatIntegrator:
         p.comfreq = self._comfreq
         p.nwater = len(waters) // 3
         p.use_graph = 1 if self._use_graph else 0
+        p.list_policy = self._list_policy
         p.waters = waters.ctypes.data_as(_lib._i32p)
         if water_geom is not None:
             for k in range(4):
@@ -172,6 +180,14 @@ class LangevinThermostatIntegrator:
 
     def getCurrentStep(self):
         return self._state(())["step"]
+
+    def getListInfo(self):
+        """dict: max_displacement (A, since the last list build),
+        max_step_displacement (A), triggered_rebuilds, half_buffer (A)."""
+        out = (ctypes.c_double * 4)()
+        _lib.check(_lib.lib().apo_lt_list_info(self._handle, out))
+        return dict(max_displacement=out[0], max_step_displacement=out[1],
+                    triggered_rebuilds=int(out[2]), half_buffer=out[3])
 
     def getXYZQ(self):
         """Device pointer (int) of the float xyzq the force path reads."""

@@ -268,6 +268,15 @@ typedef struct apo_lt_params {
                             (src/CudaHolonomicConstraint.cu:18-33) */
   const double *mass;    /* natom, amu */
   const float *charge;   /* natom, e; the q of the xyzq handed to calc_force */
+  int32_t list_policy;   /* 0: rebuild every nbfreq steps (CudaIntegrator's
+                            fixed cadence, src/CudaIntegrator.cu:29); 1: also
+                            rebuild when an atom may have moved more than half
+                            the list buffer (cutnb - ctofnb)/2 since the last
+                            build: the step kernel measures every atom's
+                            displacement from the build's coordinates, and
+                            the host, which reads it at most 4 steps late,
+                            adds the largest one-step displacement for each
+                            step it has not seen.  Batches need 0. */
 } apo_lt_params;
 
 /* Binds an integrator to `sys` (setCharmmContext) with host coordinates
@@ -330,6 +339,13 @@ APO_API apo_status apo_lt_batch_destroy(apo_lt_batch *b);
 APO_API apo_status apo_lt_get_state(apo_lt_integrator *lt, double *xyz, double *vel,
                                     double *dprev, double *grad, double *thermo6,
                                     int64_t *step);
+
+/* Neighbor-list health (synchronises): out4 = largest displacement of any
+ * atom since the last list build (A), largest one-step displacement since
+ * that build (A), rebuilds the displacement check added (list_policy 1), and
+ * half the list buffer (cutnb - ctofnb)/2 (A).  No counterpart in the
+ * reference, which rebuilds at a fixed cadence without a check. */
+APO_API apo_status apo_lt_list_info(apo_lt_integrator *lt, double *out4);
 
 /* Device float xyzq[4*natom] the integrator keeps for the force path
  * (CharmmContext::getXYZQ); valid until apo_lt_destroy. */

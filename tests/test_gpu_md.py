@@ -241,3 +241,71 @@ def test_md_persistent_batch_lanes_bitwise_equals_alone():
             assert np.array_equal(st[k], ref[k]), k
         lt.close()
         fm.close()
+
+
+@pytest.fixture(scope="module")
+def c1_equilibrated(c1):
+    """C1 after 400 Langevin steps at friction 20/ps (the lattice start is
+    hot: its first steps move atoms by tenths of an A)."""
+    sp, xyz, vel = c1
+    fm, lt = _make(sp, xyz, vel, friction=20.0)
+    lt.propagate(400)
+    st = lt.getState()
+    lt.close()
+    fm.close()
+    return sp, st["xyz"], st["vel"]
+
+
+def _list_vs_fresh(start, policy, cutnb, nsteps, nbfreq):
+    """Gradient at the final coordinates with the integrator's current list
+    and with a list built from those coordinates: they differ by FP32
+    rounding (the lists group the pairs differently) plus every pair inside
+    r_off that the current list lacks."""
+    from apocharmm_amd.force_manager import ForceManager
+    from apocharmm_amd.integrator import LangevinThermostatIntegrator
+    sp, xyz, vel = start
+    sp = dict(sp, cutnb=cutnb)
+    fm = ForceManager(sp).initialize()
+    lt = LangevinThermostatIntegrator(DT_PS, friction=FRICTION, temperature=TEMP, seed=SEED)
+    lt.setNonbondedListUpdateFrequency(nbfreq)
+    lt.setListUpdatePolicy(policy)
+    lt.setCharmmContext(fm, xyz, vel)
+    lt.propagate(nsteps)
+    info = lt.getListInfo()
+    ptr = lt.getXYZQ()
+    fm.calcForce(ptr)
+    torch.cuda.synchronize()
+    g_list = fm.getForces().cpu().numpy().copy()
+    fm.calcForce(ptr, reset=True)
+    torch.cuda.synchronize()
+    g_fresh = fm.getForces().cpu().numpy().copy()
+    lt.close()
+    fm.close()
+    err = float(np.abs(g_list - g_fresh).max())
+    rel = float(np.sqrt(np.mean(np.sum((g_list - g_fresh) ** 2, 1))) /
+                np.sqrt(np.mean(np.sum(g_fresh ** 2, 1))))
+    return info, err, rel
+
+
+def test_md_displacement_policy_rebuilds_before_pairs_are_missed(c1_equilibrated):
+    # list buffer 0.6 A (half buffer 0.3 A), at most one rebuild per 60 steps
+    info_f, err_f, rel_f = _list_vs_fresh(c1_equilibrated, "fixed", 12.6, 59, 60)
+    print("\n[fixed] %s max |dg| %.2e rms rel %.2e" % (info_f, err_f, rel_f), flush=True)
+    info, err, rel = _list_vs_fresh(c1_equilibrated, "displacement", 12.6, 59, 60)
+    print("[displacement] %s max |dg| %.2e rms rel %.2e" % (info, err, rel), flush=True)
+    assert info_f["max_displacement"] > info_f["half_buffer"]
+    assert info_f["triggered_rebuilds"] == 0
+    assert info["triggered_rebuilds"] > 0
+    assert info["max_displacement"] <= info["half_buffer"]
+    # FP32 rounding only (no missed pair); the fixed cadence missed pairs
+    assert rel <= 1e-6
+    assert err_f > 10 * err
+
+
+def test_md_displacement_policy_with_default_buffer_keeps_all_pairs(c1_equilibrated):
+    """The default 2 A buffer over 20-step intervals: the list still holds
+    every pair inside r_off at the end."""
+    info, err, rel = _list_vs_fresh(c1_equilibrated, "displacement", 14.0, 79, 20)
+    print("\n[default buffer] %s max |dg| %.2e rms rel %.2e" % (info, err, rel), flush=True)
+    assert info["max_displacement"] <= info["half_buffer"]
+    assert rel <= 1e-6
