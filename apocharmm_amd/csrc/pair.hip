@@ -185,11 +185,16 @@ __device__ __forceinline__ void pair_body(
           const CT rinv2 = rinv * rinv;
           const CT rinv3 = rinv * rinv2;
           const CT rinv6 = rinv3 * rinv3;
+          // force switch: in the switched region (r > r_on) the c6/c12
+          // terms are k6 c6 r^-3 (r^-3 - r_off^-3) and k12 c12 r^-6 (r^-6 -
+          // r_off^-6); selects between per-lane values and an fma with a -1/0
+          // factor avoid copying the switch constants into registers
           const bool outer = r2 > P.ron2;
-          const CT A6 = outer ? P.k6 : (CT)1, B6 = outer ? P.roffinv3 : (CT)0;
-          const CT A12 = outer ? P.k12 : (CT)1, B12 = outer ? P.roffinv6 : (CT)0;
-          const CT t6 = rinv3 - B6, t12 = rinv6 - B12;
-          const CT f_vdw = c.x * A6 * t6 * rinv3 - c.y * A12 * t12 * rinv6;
+          const CT mo = outer ? (CT)-1 : (CT)0;
+          const CT t6 = fma(mo, P.roffinv3, rinv3), t12 = fma(mo, P.roffinv6, rinv6);
+          const CT c6k = c.x * P.k6, c12k = c.y * P.k12;
+          const CT c6 = outer ? c6k : c.x, c12 = outer ? c12k : c.y;
+          const CT f_vdw = c6 * t6 * rinv3 - c12 * t12 * rinv6;
           const CT fij = (f_vdw + f_el) * rinv2;
           const CT gx = fij * dx, gy = fij * dy, gz = fij * dz;
           fjx -= gx;
@@ -199,6 +204,7 @@ __device__ __forceinline__ void pair_body(
           fi[3 * k + 1] += gy;
           fi[3 * k + 2] += gz;
           if (CALC_E) {
+            const CT A6 = outer ? P.k6 : (CT)1, A12 = outer ? P.k12 : (CT)1;
             const CT C6 = outer ? (CT)0 : P.dv6, C12 = outer ? (CT)0 : P.dv12;
             el += (double)u_el;
             ev += (double)(c.y * ((CT)1 / (CT)12) * (A12 * t12 * t12 + C12) -
