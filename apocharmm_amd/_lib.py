@@ -104,6 +104,7 @@ def lib():
         L.apo_lt_batch_create.argtypes = [ctypes.POINTER(vp), i32, i32, ctypes.POINTER(vp)]
         L.apo_lt_batch_propagate.argtypes = [vp, i32, vp]
         L.apo_lt_batch_destroy.argtypes = [vp]
+        L.apo_lt_batch_autotune.argtypes = [vp, vp, dp]
         L.apo_lt_get_state.argtypes = [vp, vp, vp, vp, vp, vp, i64p]
         L.apo_lt_xyzq.argtypes = [vp, ctypes.POINTER(vp)]
         L.apo_lt_list_info.argtypes = [vp, dp]
@@ -112,7 +113,7 @@ def lib():
                      "apo_nb_force_to_double", "apo_nb_list_stats", "apo_topology_exclusions",
                      "apo_lt_create", "apo_lt_destroy", "apo_lt_initialize", "apo_lt_propagate",
                      "apo_lt_propagate_batch", "apo_lt_batch_create", "apo_lt_batch_propagate",
-                     "apo_lt_batch_destroy",
+                     "apo_lt_batch_destroy", "apo_lt_batch_autotune",
                      "apo_lt_get_state", "apo_lt_xyzq", "apo_lt_list_info"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L

@@ -41,9 +41,10 @@ struct PmeBatch;
 PmeBatch *pme_batch_create();
 void pme_batch_destroy(PmeBatch *b);
 void pme_batch_clear(PmeBatch &b);
-void pme_batch_add(PmeBatch &b, NbSystem &S, int64_t *force, int stride);
+void pme_batch_add(PmeBatch &b, NbSystem &S, int64_t *force, int stride, int fft_rows,
+                   int fft_cols);
 void pme_batch_upload(PmeBatch &b, hipStream_t s);
-void pme_batch_launch(PmeBatch &b, hipStream_t s);
+void pme_batch_launch(PmeBatch &b, hipStream_t s, bool gather);
 struct BondedBatch;
 BondedBatch *bonded_batch_create();
 void bonded_batch_destroy(BondedBatch *b);
@@ -1010,6 +1011,10 @@ struct LtBatch {
   BondedBatch *bond = nullptr;
   bool tab_lds = true;
   std::vector<unsigned long long> tab_gens;
+  // launch shapes (the autotuner's choices): pair-kernel blocks per CU
+  // shared out by atom count, FFT row pairs / columns per block
+  double pair_bpc = kPairBlocksPerCu;
+  int fft_rows = kFftRowsPerBlock, fft_cols = kFftColsPerBlock;
 
   ~LtBatch() {
     for (hipGraphExec_t g : ex)
@@ -1049,10 +1054,10 @@ struct LtBatch {
       int64_t *g = L->grad.get();
       fill.add(FillArgs{g, 3 * (size_t)L->gstride}, fill_blocks(3 * (size_t)L->gstride));
       batch_add_sort(sort, S, x);
-      const int pb = std::max(8, (int)std::lround(kPairBlocksPerCu * ncu * (L->n / ntot)));
+      const int pb = std::max(8, (int)std::lround(pair_bpc * ncu * (L->n / ntot)));
       batch_add_pair(pair, S, pb);
       tab_lds = tab_lds && pair_tab_lds(S);
-      pme_batch_add(*pme, S, g, L->gstride);
+      pme_batch_add(*pme, S, g, L->gstride, fft_rows, fft_cols);
       bonded_batch_add(*bond, S, x, g, L->gstride);
       batch_add_finalize(fin, S, g, L->gstride);
       lstep.add(LtStepArgs{L->state(), L->C, L->counters.get(), L->ke_part.get(), L->dpart.get()},
@@ -1079,13 +1084,99 @@ struct LtBatch {
     batch_fill_i64(fill, s);
     batch_sort(sort, s);
     batch_pair(pair, tab_lds, s);
-    pme_batch_launch(*pme, s);
+    pme_batch_launch(*pme, s, true);
     bonded_batch_launch(*bond, s);
     batch_finalize(fin, s);
     kb_lt_step<<<lstep.blocks(), kStepBlock, 0, s>>>(lstep.d_args.get(), lstep.d_start.get(),
                                                      lstep.nsys());
     kb_lt_finalize<<<lfin.nsys(), 256, 0, s>>>(lfin.d_args.get());
     APO_LAUNCH_CHECK();
+  }
+
+  // Autotuner of the launch shapes.  It times the fused pair kernel for
+  // eight block budgets and the fused spread + FFT launches for several
+  // FFT blockings on the systems' current coordinates and lists, and keeps
+  // the fastest.  Those launches write only per-step scratch (the sorted
+  // force buffer, the grids), which the next step rewrites from scratch, and
+  // each row, column and item is computed the same way in any blocking, so
+  // trajectories do not change.  report[8]: chosen blocks per CU, FFT rows,
+  // FFT columns, their pair and PME times (ms), the default shape's pair and
+  // PME times, candidates timed.
+  void autotune(hipStream_t caller, double *report) {
+    APO_HIP(hipEventRecord(m_ev, caller));
+    APO_HIP(hipStreamWaitEvent(m, m_ev, 0));
+    for (LtSystem *L : sys)
+      if (!L->initialized) {
+        APO_HIP(hipStreamWaitEvent(L->st, m_ev, 0));
+        L->initialize(L->st);
+        APO_HIP(hipEventRecord(L->ev_out, L->st));
+        APO_HIP(hipStreamWaitEvent(m, L->ev_out, 0));
+      }
+    APO_HIP(hipStreamSynchronize(m));
+    for (LtSystem *L : sys) APO_HIP(hipStreamSynchronize(L->st));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    APO_HIP(hipEventCreate(&e0));
+    APO_HIP(hipEventCreate(&e1));
+    int ntimed = 0;
+    auto timed = [&](auto &&launch) {
+      launch();  // warm
+      APO_HIP(hipEventRecord(e0, m));
+      for (int r = 0; r < 3; r++) launch();
+      APO_HIP(hipEventRecord(e1, m));
+      APO_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      APO_HIP(hipEventElapsedTime(&ms, e0, e1));
+      ntimed++;
+      return ms / 3.0;
+    };
+    try {
+      const double bpc_cand[] = {8, 16, 24, 32, 48, 64, 96, 128};
+      double best_bpc = kPairBlocksPerCu, best_pair = 1e30, def_pair = 0;
+      for (double bpc : bpc_cand) {
+        pair_bpc = bpc;
+        build_tables();
+        const double t = timed([&] { batch_pair(pair, tab_lds, m); });
+        if (bpc == kPairBlocksPerCu) def_pair = t;
+        if (t < best_pair) best_pair = t, best_bpc = bpc;
+      }
+      pair_bpc = best_bpc;
+      const int fft_cand[][2] = {{16, 16}, {8, 8}, {4, 4}, {16, 8}, {8, 16}, {4, 8}};
+      int best_r = kFftRowsPerBlock, best_c = kFftColsPerBlock;
+      double best_pme = 1e30, def_pme = 0;
+      for (const auto &rc : fft_cand) {
+        fft_rows = rc[0];
+        fft_cols = rc[1];
+        build_tables();
+        const double t = timed([&] { pme_batch_launch(*pme, m, false); });
+        if (rc[0] == kFftRowsPerBlock && rc[1] == kFftColsPerBlock) def_pme = t;
+        if (t < best_pme) best_pme = t, best_r = rc[0], best_c = rc[1];
+      }
+      fft_rows = best_r;
+      fft_cols = best_c;
+      build_tables();
+      if (report) {
+        report[0] = pair_bpc;
+        report[1] = fft_rows;
+        report[2] = fft_cols;
+        report[3] = best_pair;
+        report[4] = best_pme;
+        report[5] = def_pair;
+        report[6] = def_pme;
+        report[7] = ntimed;
+      }
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // graphs captured with the old shapes are recaptured at the next step
+    for (hipGraphExec_t &g : ex)
+      if (g) {
+        APO_HIP(hipGraphExecDestroy(g));
+        g = nullptr;
+      }
   }
 
   void capture(int rb) {
@@ -1221,6 +1312,10 @@ LtBatch *create_batch(LtSystem **Ls, int n, int lanes) {
 }
 
 void destroy_batch(LtBatch *B) { delete B; }
+
+void batch_autotune(LtBatch &B, hipStream_t caller, double *report8) {
+  B.autotune(caller, report8);
+}
 
 void batch_propagate(LtBatch &B, int nsteps, hipStream_t caller) {
   if (nsteps < 0) throw InvalidArgument("nsteps must be >= 0");

@@ -675,7 +675,11 @@ template <typename CT> struct PmeShapes {
   ConvArgs c;
 };
 
-template <typename CT> static PmeShapes<CT> pme_shapes(const NbSystem &S) {
+// rows: x-pass row pairs per block; cols: y/z-pass columns per block (both
+// capped by 64 KiB of LDS)
+template <typename CT>
+static PmeShapes<CT> pme_shapes(const NbSystem &S, int rows = kFftRowsPerBlock,
+                                int cols = kFftColsPerBlock) {
   using C2 = typename Cx<CT>::T;
   PmeShapes<CT> P;
   const int K0 = S.K[0], K1 = S.K[1], K2 = S.K[2], H = K0 / 2 + 1;
@@ -686,10 +690,10 @@ template <typename CT> static PmeShapes<CT> pme_shapes(const NbSystem &S) {
   const size_t lds_cap = 64 * 1024;
   P.nrows = K1 * K2;
   P.rb = (int)std::max<size_t>(
-      1, std::min<size_t>(kFftRowsPerBlock, (lds_cap / sizeof(C2) - K0) / (2 * K0)));
+      1, std::min<size_t>((size_t)std::max(rows, 1), (lds_cap / sizeof(C2) - K0) / (2 * K0)));
   P.shx = (size_t)(K0 + 2 * P.rb * K0) * sizeof(C2);
   P.npair = (P.nrows + 1) / 2;
-  const int W = kFftColsPerBlock;
+  const int W = std::max(cols, 1);
   P.wy = (int)std::max<size_t>(1, std::min<size_t>(W, (lds_cap / sizeof(C2) - K1) / (2 * K1)));
   P.shy = (size_t)(K1 + 2 * P.wy * K1) * sizeof(C2);
   P.ncy = (H + P.wy - 1) / P.wy;
@@ -856,9 +860,10 @@ void pme_batch_clear(PmeBatch &b) {
   b.gather.clear();
 }
 
-void pme_batch_add(PmeBatch &b, NbSystem &S, int64_t *force, int stride) {
+void pme_batch_add(PmeBatch &b, NbSystem &S, int64_t *force, int stride, int fft_rows,
+                   int fft_cols) {
   if (S.precision != 0) throw InvalidArgument("fused batches run the mixed-precision path");
-  const PmeShapes<float> P = pme_shapes<float>(S);
+  const PmeShapes<float> P = pme_shapes<float>(S, fft_rows, fft_cols);
   if (S.K[1] * P.ncz > S.part.recip_n) throw RuntimeError("reciprocal partial buffer too small");
   float *rgrid = reinterpret_cast<float *>(S.qgrid.get());
   const int H = S.K[0] / 2 + 1;
@@ -886,8 +891,9 @@ void pme_batch_upload(PmeBatch &b, hipStream_t s) {
 }
 
 // spread, the five FFT passes with the convolution, gather: one launch each
-// for all systems (cell boxes must be current)
-void pme_batch_launch(PmeBatch &b, hipStream_t s) {
+// for all systems (cell boxes must be current).  Without the gather the
+// launches write only the batch's grids (the autotuner times them so).
+void pme_batch_launch(PmeBatch &b, hipStream_t s, bool gather) {
   if (!b.spread.nsys()) return;
   kb_spread<<<b.spread.blocks(), 256, 0, s>>>(b.spread.d_args.get(), b.spread.d_start.get(),
                                               b.spread.nsys());
@@ -901,8 +907,9 @@ void pme_batch_launch(PmeBatch &b, hipStream_t s) {
                                                1);
   kb_fftx_inv<<<b.fx_inv.blocks(), 256, b.fx_inv.lds, s>>>(b.fx_inv.d_args.get(),
                                                            b.fx_inv.d_start.get(), b.fx_inv.nsys());
-  kb_gather<<<b.gather.blocks(), 256, 0, s>>>(b.gather.d_args.get(), b.gather.d_start.get(),
-                                              b.gather.nsys());
+  if (gather)
+    kb_gather<<<b.gather.blocks(), 256, 0, s>>>(b.gather.d_args.get(), b.gather.d_start.get(),
+                                                b.gather.nsys());
   APO_LAUNCH_CHECK();
 }
 

@@ -286,6 +286,7 @@ def bench_batched(a):
         torch.cuda.synchronize()
         t_seq += time.perf_counter() - t0
     batch = LangevinBatch([lt for _, lt in runs], lanes=a.lanes)
+    tuned = None if a.no_autotune else batch.autotune()
     batch.propagate(a.warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -318,7 +319,7 @@ def bench_batched(a):
                    "dt_fs": 2.0, "friction_per_ps": FRICTION, "temperature_K": TEMP,
                    "list_rebuild_every": REBUILD_EVERY, "precision": a.precision,
                    "gpu_max_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
-                   "lanes": a.lanes,
+                   "lanes": a.lanes, "autotune": tuned,
                    "parallelism": "single GPU, one graph launch per batch step"},
         "sequential": {"value": round(agg(t_seq), 3), "unit": "ns/day",
                        "ms_per_step_all_systems": round(t_seq / a.steps * 1e3, 4)},
@@ -405,6 +406,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=100, help="c5: steps per batch call")
     ap.add_argument("--slots", type=int, default=64, help="c5: most jobs on the GPU at once")
     ap.add_argument("--max-atoms", type=int, default=100000, help="c5: largest job size")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="c4: keep the default launch shapes (no timing pass)")
     ap.add_argument("--nve", action="store_true",
                     help="energy-conservation run: Langevin equilibration, then --steps NVE steps")
     ap.add_argument("--equil-steps", type=int, default=10000)

@@ -309,3 +309,42 @@ def test_md_displacement_policy_with_default_buffer_keeps_all_pairs(c1_equilibra
     print("\n[default buffer] %s max |dg| %.2e rms rel %.2e" % (info, err, rel), flush=True)
     assert info["max_displacement"] <= info["half_buffer"]
     assert rel <= 1e-6
+
+
+def test_md_batch_autotune_keeps_trajectories():
+    """apo_lt_batch_autotune between propagate calls: the chosen launch shapes
+    change no bit of any system's trajectory."""
+    from apocharmm_amd.integrator import LangevinBatch, maxwell_velocities
+    sizes = [540, 760, 980]
+    specs = [systems.batched_system(i + 20, w) for i, w in enumerate(sizes)]
+
+    def fresh():
+        out = []
+        for i, sp in enumerate(specs):
+            v = maxwell_velocities(sp["masses"], TEMP, 4400 + i)
+            out.append(_make(sp, np.asarray(sp["xyz"], np.float64), v, seed=SEED + 5 * i,
+                             nbfreq=10))
+        return out
+
+    alone = []
+    for fm, lt in fresh():
+        lt.propagate(15)
+        lt.propagate(15)
+        alone.append(lt.getState())
+        lt.close()
+        fm.close()
+    runs = fresh()
+    b = LangevinBatch([lt for _, lt in runs], lanes=2)
+    rep = b.autotune()
+    print("\n[autotune] %s" % rep, flush=True)
+    assert rep["launches_timed"] == 14 and rep["pair_ms"] <= rep["default_pair_ms"]
+    b.propagate(15)
+    b.autotune()
+    b.propagate(15)
+    b.close()
+    for (fm, lt), ref in zip(runs, alone):
+        st = lt.getState()
+        for k in ("xyz", "vel", "dprev", "grad"):
+            assert np.array_equal(st[k], ref[k]), k
+        lt.close()
+        fm.close()
