@@ -32,14 +32,27 @@ constexpr int kMaxCellTiles = 2048;
 #define APO_PAIR_BLOCKS_PER_CU 32
 #endif
 constexpr int kPairBlocksPerCu = APO_PAIR_BLOCKS_PER_CU;
-// FFT passes: real-row pairs per block (x) and columns per block (y, z).
+// PME spread + x FFT: slab of whole x rows, by x bz rows (y, z) per block.
+#ifndef APO_SLAB_BY
+#define APO_SLAB_BY 4
+#endif
+#ifndef APO_SLAB_BZ
+#define APO_SLAB_BZ 4
+#endif
+constexpr int kSlabBy = APO_SLAB_BY;
+constexpr int kSlabBz = APO_SLAB_BZ;
+// FFT passes: real-row pairs per block (x inverse) and columns per block (y, z).
 #ifndef APO_FFT_ROWS_PER_BLOCK
 #define APO_FFT_ROWS_PER_BLOCK 16
 #endif
 #ifndef APO_FFT_COLS_PER_BLOCK
 #define APO_FFT_COLS_PER_BLOCK 16
 #endif
+#ifndef APO_FFT_THREADS
+#define APO_FFT_THREADS 256
+#endif
 constexpr int kFftRowsPerBlock = APO_FFT_ROWS_PER_BLOCK;
+constexpr int kFftThreads = APO_FFT_THREADS;  // threads per block of the y, z and x-inverse passes
 constexpr int kFftColsPerBlock = APO_FFT_COLS_PER_BLOCK;
 // Largest xy column population the LDS column sort handles.
 constexpr int kMaxColumn = 2048;

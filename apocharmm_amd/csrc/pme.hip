@@ -11,8 +11,9 @@
 //   * the charge grid is int64 fixed point (2^-40): order-independent sums;
 //   * the 3-D R2C/C2R FFT is five LDS passes of a hand-written mixed-radix
 //     (4, 2, 3, 5, 7, 11, 13) Stockham FFT: x (two real rows packed per complex
-//     transform), y, [z forward -> k-space convolution + energy/virial -> z
-//     inverse] fused in one kernel, y inverse, x inverse;
+//     transform, run by the spread kernel on the rows it owns), y, [z forward
+//     -> k-space convolution + energy/virial -> z inverse] fused in one kernel,
+//     y inverse, x inverse;
 //   * gather uses 16 lanes per atom and writes int64 2^-40 gradients.
 #include <algorithm>
 #include <cmath>
@@ -180,25 +181,38 @@ struct GridGeom {
   double inv0, inv1, inv2;
 };
 
-// Brick spread (no global atomics).  One block owns a brick of the grid
-// (kBx x kBy x kBz points) and accumulates, in LDS int64 2^-40 fixed point,
-// every stencil contribution that lands in it, from the atoms of the cells
-// whose (per-step) bounding box reaches the brick; the brick is then stored
-// once as CT.  The integer sums are exact, so the grid is bitwise identical to
-// any order of accumulation, and no zero fill or HBM atomic is needed.
+// Spread fused with the forward x FFT (no global atomics, no real grid in
+// HBM).  One block owns a slab of whole x rows, by x bz rows in (y, z), and
+// accumulates in LDS int64 2^-40 fixed point every stencil contribution that
+// lands in it, from the atoms of the cells whose (per-step) bounding box
+// reaches the slab.  The integer sums are exact, so the charges are bitwise
+// those of any order of accumulation, and no zero fill or HBM atomic is
+// needed.  The block then converts its rows to CT, packs rows (y, y + 1) of one
+// z plane into one complex transform, runs the x FFT in LDS and stores the
+// half-complex rows the y pass reads.
 //
 // Work split: each lane prepares one candidate atom (grid base index and
 // fractional offsets, FP64); the wave then walks the atoms that reach the
-// brick one at a time, lane l adding the stencil point (l & 3, (l >> 2) & 3,
+// slab one at a time, lane l adding the stencil point (l & 3, (l >> 2) & 3,
 // l >> 4) of that atom, so the 64 LDS atomics of one instruction hit 64
 // distinct points.
-constexpr int kBx = 8, kBy = 8, kBz = 16;
-
 struct BrickGeom {
   int K0, K1, K2;
-  int nbx, nby, nbz;
+  int by, bz;    // rows per slab in y (even) and z
+  int nby, nbz;  // slabs per axis
   float L0, L1, L2;
 };
+
+// LDS of one slab block, all of it dynamic so that the int64 rows start at
+// offset 0 (64-bit LDS atomics need 8-byte alignment, and the dynamic region
+// would otherwise begin right after any static arrays): int64 rows, twiddles,
+// two complex FFT buffers, then the candidate cells and their count.
+constexpr int kSlabCand = 1024;  // cells tested per round (a multiple of 256)
+template <typename C2> static size_t slab_lds(int K0, int by, int bz) {
+  const size_t rows = (size_t)by * bz;
+  return rows * K0 * sizeof(unsigned long long) + (size_t)K0 * sizeof(C2) +
+         rows * K0 * sizeof(C2) + (kSlabCand + 4) * sizeof(int);
+}
 
 // Cyclic-interval overlap on a circle of length K (iK = 1/K).  The reduced
 // offsets may be off by one rounding of K, which the callers' slack of two
@@ -255,104 +269,153 @@ __device__ __forceinline__ double readlane_f64(double v, int k) {
 }
 
 template <typename CT>
-__device__ __forceinline__ void spread_body(int bid, const int4 *__restrict__ posq,
-                                            const float4 *__restrict__ bb,
-                                            const int *__restrict__ meta, const BrickGeom &g,
-                                            double qunscale, CT *__restrict__ grid) {
-  __shared__ unsigned long long brick[kBx * kBy * kBz];
-  __shared__ int cand[256];
-  __shared__ int ncand;
+__device__ __forceinline__ void spread_fftx_body(int bid, const int4 *__restrict__ posq,
+                                                 const float4 *__restrict__ bb,
+                                                 const int *__restrict__ meta, const BrickGeom &g,
+                                                 double qunscale, const FftPlan &p,
+                                                 const typename Cx<CT>::T *__restrict__ twg,
+                                                 typename Cx<CT>::T *__restrict__ out) {
+  using C2 = typename Cx<CT>::T;
+  extern __shared__ char smem[];  // no static LDS in this kernel: smem is at offset 0
+  const int n = g.K0, by = g.by, bz = g.bz, nrow = by * bz;
+  unsigned long long *slab = reinterpret_cast<unsigned long long *>(smem);
+  C2 *tw = reinterpret_cast<C2 *>(slab + (size_t)nrow * n);
+  C2 *a = tw + n;
+  C2 *b = a + (nrow / 2) * n;
+  int *cand = reinterpret_cast<int *>(b + (nrow / 2) * n);
+  int &ncand = cand[kSlabCand];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int bxi = bid % g.nbx, byi = (bid / g.nbx) % g.nby, bzi = bid / (g.nbx * g.nby);
-  const int x0 = bxi * kBx, y0 = byi * kBy, z0 = bzi * kBz;
-  for (int k = tid; k < kBx * kBy * kBz; k += 256) brick[k] = 0ull;
+  const int byi = bid % g.nby, bzi = bid / g.nby;
+  const int y0 = byi * by, z0 = bzi * bz;
+  for (int k = tid; k < nrow * n; k += 256) slab[k] = 0ull;
+  for (int e = tid; e < n; e += 256) tw[e] = twg[e];
   const int ncell = meta[0];
-  // atoms whose stencil base lies in [p0 - 3, p0 + B) reach the brick
-  const float bx0 = (float)(x0 - 3), by0 = (float)(y0 - 3), bz0 = (float)(z0 - 3);
-  const float K0 = (float)g.K0, K1 = (float)g.K1, K2 = (float)g.K2;
-  const float iK0 = 1.f / K0, iK1 = 1.f / K1, iK2 = 1.f / K2;
-  const float gx = K0 / g.L0, gy = K1 / g.L1, gz = K2 / g.L2;  // grid points per A
+  // atoms whose stencil base lies in [p0 - 3, p0 + B) reach the slab in y and z
+  const float by0 = (float)(y0 - 3), bz0 = (float)(z0 - 3);
+  const float K1 = (float)g.K1, K2 = (float)g.K2;
+  const float iK1 = 1.f / K1, iK2 = 1.f / K2;
+  const float gy = K1 / g.L1, gz = K2 / g.L2;  // grid points per A
   // this lane's stencil point and its weight polynomials
   const int sa = lane & 3, sb = (lane >> 2) & 3, sc = lane >> 4;
   const Theta6 cx = theta6_coef(sa), cy = theta6_coef(sb), cz = theta6_coef(sc);
-  for (int c0 = 0; c0 < ncell; c0 += 256) {
+  // cells are tested kSlabCand at a time, four per thread with their loads in
+  // flight together; the atoms of the cells that pass are walked by the four
+  // waves, each prefetching its next atom while it adds the current hits
+  for (int c0 = 0; c0 < ncell; c0 += kSlabCand) {
     if (tid == 0) ncand = 0;
     __syncthreads();
-    const int c = c0 + tid;
-    if (c < ncell) {
-      const float4 cc = bb[2 * c], hh = bb[2 * c + 1];
-      // cell extent in grid units (x/L + 1/2 origin), +2 for the floor and
-      // single-precision slack
-      float ax = fmaf(cc.x - hh.x, gx, 0.5f * K0), lax = fmaf(2.f * hh.x, gx, 2.f);
-      float ay = fmaf(cc.y - hh.y, gy, 0.5f * K1), lay = fmaf(2.f * hh.y, gy, 2.f);
-      float az = fmaf(cc.z - hh.z, gz, 0.5f * K2), laz = fmaf(2.f * hh.z, gz, 2.f);
-      if (cyc_overlap(ax - 1.f, lax, bx0, (float)(kBx + 3), K0, iK0) &&
-          cyc_overlap(ay - 1.f, lay, by0, (float)(kBy + 3), K1, iK1) &&
-          cyc_overlap(az - 1.f, laz, bz0, (float)(kBz + 3), K2, iK2)) {
-        int slot = atomicAdd(&ncand, 1);
-        cand[slot] = c;
+    float4 cc[kSlabCand / 256], hh[kSlabCand / 256];
+#pragma unroll
+    for (int u = 0; u < kSlabCand / 256; u++) {
+      const int c = c0 + u * 256 + tid;
+      if (c < ncell) {
+        cc[u] = bb[2 * c];
+        hh[u] = bb[2 * c + 1];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kSlabCand / 256; u++) {
+      const int c = c0 + u * 256 + tid;
+      if (c < ncell) {
+        // cell extent in grid units (x/L + 1/2 origin), +2 for the floor and
+        // single-precision slack
+        float ay = fmaf(cc[u].y - hh[u].y, gy, 0.5f * K1), lay = fmaf(2.f * hh[u].y, gy, 2.f);
+        float az = fmaf(cc[u].z - hh[u].z, gz, 0.5f * K2), laz = fmaf(2.f * hh[u].z, gz, 2.f);
+        if (cyc_overlap(ay - 1.f, lay, by0, (float)(by + 3), K1, iK1) &&
+            cyc_overlap(az - 1.f, laz, bz0, (float)(bz + 3), K2, iK2)) {
+          int slot = atomicAdd(&ncand, 1);
+          cand[slot] = c;
+        }
       }
     }
     __syncthreads();
     const int na = ncand * kCell;
+    int4 nxt = make_int4(0, 0, 0, 0);
+    if (wave * 64 + lane < na) {
+      const int w = wave * 64 + lane;
+      nxt = posq[cand[w >> 5] * kCell + (w & 31)];
+    }
     for (int base = wave * 64; base < na; base += 256) {
       // lane prepares atom base + lane
       const int w = base + lane;
-      int dx = kBx, dy = kBy, dz = kBz;
+      const int4 q4 = nxt;
+      if (w + 256 < na) nxt = posq[cand[(w + 256) >> 5] * kCell + ((w + 256) & 31)];
+      int ix = 0, dy = by, dz = bz;
       double wx = 0.0, wy = 0.0, wz = 0.0, qs = 0.0;
       if (w < na) {
-        const int4 p = posq[cand[w >> 5] * kCell + (w & 31)];
-        const double q = (double)__int_as_float(p.w) * qunscale;
+        const double q = (double)__int_as_float(q4.w) * qunscale;
         if (q != 0.0) {
-          int ix, iy, iz;
-          frac_fixed(p.x, g.K0, ix, wx);
-          frac_fixed(p.y, g.K1, iy, wy);
-          frac_fixed(p.z, g.K2, iz, wz);
-          // offsets of the stencil base relative to the brick origin, in [-3, K-4]
-          dx = ix - x0;
+          int iy, iz;
+          frac_fixed(q4.x, g.K0, ix, wx);
+          frac_fixed(q4.y, g.K1, iy, wy);
+          frac_fixed(q4.z, g.K2, iz, wz);
+          // offsets of the stencil base relative to the slab origin, in [-3, K-4]
           dy = iy - y0;
           dz = iz - z0;
-          dx -= g.K0 * (int)floorf((float)(dx + 3) / K0);
           dy -= g.K1 * (int)floorf((float)(dy + 3) / K1);
           dz -= g.K2 * (int)floorf((float)(dz + 3) / K2);
           qs = q * (kGridScale / 216.0);
         }
       }
-      unsigned long long hit = __ballot(dx < kBx && dy < kBy && dz < kBz);
+      unsigned long long hit = __ballot(dy < by && dz < bz);
       while (hit) {
         const int k = __builtin_ctzll(hit);
         hit &= hit - 1ull;
-        const int px = __builtin_amdgcn_readlane(dx, k) + sa;
+        int px = __builtin_amdgcn_readlane(ix, k) + sa;
         const int py = __builtin_amdgcn_readlane(dy, k) + sb;
         const int pz = __builtin_amdgcn_readlane(dz, k) + sc;
         const double awx = readlane_f64(wx, k), awy = readlane_f64(wy, k),
                      awz = readlane_f64(wz, k), aqs = readlane_f64(qs, k);
-        if (px >= 0 && px < kBx && py >= 0 && py < kBy && pz >= 0 && pz < kBz) {
+        if (px >= n) px -= n;
+        if (py >= 0 && py < by && pz >= 0 && pz < bz) {
           const double v = aqs * theta6(cx, awx) * theta6(cy, awy) * theta6(cz, awz);
           const long long iv = rint_i64(v);
-          if (iv) atomicAdd(&brick[(pz * kBy + py) * kBx + px], (unsigned long long)iv);
+          if (iv) atomicAdd(&slab[(pz * by + py) * n + px], (unsigned long long)iv);
         }
       }
     }
     __syncthreads();
   }
-  // store the owned points (edge bricks may overhang the grid)
-  for (int k = tid; k < kBx * kBy * kBz; k += 256) {
-    const int px = k % kBx, py = (k / kBx) % kBy, pz = k / (kBx * kBy);
-    const int gx = x0 + px, gy = y0 + py, gz = z0 + pz;
-    if (gx < g.K0 && gy < g.K1 && gz < g.K2)
-      grid[((size_t)gz * g.K1 + gy) * g.K0 + gx] =
-          (CT)((double)(long long)brick[k] * (1.0 / kGridScale));
+  // rows (y, y + 1) of one z plane -> one complex row; rows past the grid
+  // (edge slabs) hold only wrapped duplicates and enter as zero
+  const int hb = by >> 1, ncx = nrow >> 1, H = n / 2 + 1;
+  for (int e = tid; e < ncx * n; e += 256) {
+    const int r = e / n, x = e - r * n;
+    const int pz = r / hb, py = 2 * (r - pz * hb);
+    const int y = y0 + py, z = z0 + pz;
+    const unsigned long long *row = slab + (size_t)(pz * by + py) * n;
+    C2 v;
+    v.x = (y < g.K1 && z < g.K2) ? (CT)((double)(long long)row[x] * (1.0 / kGridScale)) : (CT)0;
+    v.y = (y + 1 < g.K1 && z < g.K2) ? (CT)((double)(long long)row[n + x] * (1.0 / kGridScale))
+                                     : (CT)0;
+    a[e] = v;
+  }
+  __syncthreads();
+  C2 *res = lds_fft<C2>(a, b, p, ncx, tw, false);
+  for (int e = tid; e < ncx * H; e += 256) {
+    const int r = e / H, k = e - r * H;
+    const int pz = r / hb, py = 2 * (r - pz * hb);
+    const int y = y0 + py, z = z0 + pz;
+    const C2 Z = res[r * n + k], Zm = res[r * n + (n - k) % n];
+    C2 A, B;
+    A.x = (CT)0.5 * (Z.x + Zm.x);
+    A.y = (CT)0.5 * (Z.y - Zm.y);
+    B.x = (CT)0.5 * (Z.y + Zm.y);
+    B.y = (CT)0.5 * (Zm.x - Z.x);
+    const size_t o = ((size_t)z * g.K1 + y) * H + k;
+    if (z < g.K2 && y < g.K1) out[o] = A;
+    if (z < g.K2 && y + 1 < g.K1) out[o + H] = B;
   }
 }
 template <typename CT>
-__global__ __launch_bounds__(256) void k_spread_brick(const int4 *__restrict__ posq,
-                                                      const float4 *__restrict__ bb,
-                                                      const int *__restrict__ meta, BrickGeom g,
-                                                      double qunscale, CT *__restrict__ grid) {
-  spread_body<CT>(blockIdx.x, posq, bb, meta, g, qunscale, grid);
+__global__ __launch_bounds__(256) void k_spread_fftx(const int4 *__restrict__ posq,
+                                                     const float4 *__restrict__ bb,
+                                                     const int *__restrict__ meta, BrickGeom g,
+                                                     double qunscale, FftPlan p,
+                                                     const typename Cx<CT>::T *__restrict__ twg,
+                                                     typename Cx<CT>::T *__restrict__ out) {
+  spread_fftx_body<CT>(blockIdx.x, posq, bb, meta, g, qunscale, p, twg, out);
 }
-
 
 // Gather in cell order: 16 lanes per atom; the gradient goes straight into
 // the global-order output with int64 atomics (three per atom), so the gather
@@ -419,52 +482,6 @@ __global__ __launch_bounds__(256) void k_gather_sorted(const int4 *__restrict__ 
                                                        unsigned long long *__restrict__ force,
                                                        int stride) {
   gather_body<CT>(blockIdx.x, posq, meta, loc2glo, g, qunscale, phi, force, stride);
-}
-
-
-// x forward: int64 rows -> complex half rows, two real rows per complex FFT.
-template <typename CT>
-__device__ __forceinline__ void fftx_fwd_body(int bid, const CT *__restrict__ grid, int nrows,
-                                              const FftPlan &p, int rb,
-                                              const typename Cx<CT>::T *__restrict__ twg,
-                                              typename Cx<CT>::T *__restrict__ out) {
-  using C2 = typename Cx<CT>::T;
-  extern __shared__ char smem[];
-  const int n = p.n, H = n / 2 + 1;
-  C2 *tw = reinterpret_cast<C2 *>(smem);
-  C2 *a = tw + n;
-  C2 *b = a + rb * n;
-  for (int e = threadIdx.x; e < n; e += blockDim.x) tw[e] = twg[e];
-  const int r0 = bid * rb;  // first row pair
-  for (int e = threadIdx.x; e < rb * n; e += blockDim.x) {
-    const int r = e / n, x = e - r * n;
-    const int ra = 2 * (r0 + r), rbw = ra + 1;
-    C2 v;
-    v.x = ra < nrows ? grid[(size_t)ra * n + x] : (CT)0;
-    v.y = rbw < nrows ? grid[(size_t)rbw * n + x] : (CT)0;
-    a[e] = v;
-  }
-  __syncthreads();
-  C2 *res = lds_fft<C2>(a, b, p, rb, tw, false);
-  for (int e = threadIdx.x; e < rb * H; e += blockDim.x) {
-    const int r = e / H, k = e - r * H;
-    const int ra = 2 * (r0 + r), rbw = ra + 1;
-    const C2 Z = res[r * n + k], Zm = res[r * n + (n - k) % n];
-    C2 A, B;
-    A.x = (CT)0.5 * (Z.x + Zm.x);
-    A.y = (CT)0.5 * (Z.y - Zm.y);
-    B.x = (CT)0.5 * (Z.y + Zm.y);
-    B.y = (CT)0.5 * (Zm.x - Z.x);
-    if (ra < nrows) out[(size_t)ra * H + k] = A;
-    if (rbw < nrows) out[(size_t)rbw * H + k] = B;
-  }
-}
-template <typename CT>
-__global__ __launch_bounds__(256) void k_fftx_fwd(const CT *__restrict__ grid, int nrows,
-                                                  FftPlan p, int rb,
-                                                  const typename Cx<CT>::T *__restrict__ twg,
-                                                  typename Cx<CT>::T *__restrict__ out) {
-  fftx_fwd_body<CT>(blockIdx.x, grid, nrows, p, rb, twg, out);
 }
 
 
@@ -648,7 +665,7 @@ __device__ __forceinline__ void fftz_conv_body(int bid, typename Cx<CT>::T *__re
     __syncthreads();
     if (threadIdx.x < 7) {
       double s = 0;
-      for (int w = 0; w < 4; w++) s += red[w * 8 + threadIdx.x];
+      for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += red[w * 8 + threadIdx.x];
       // energy +1/2 ccelec sum, virial -1/2 ccelec sum
       part[bid * kRecipPart + threadIdx.x] =
           threadIdx.x == 0 ? s * c.half_ccelec : -s * c.half_ccelec;
@@ -670,13 +687,14 @@ __global__ __launch_bounds__(256) void k_fftz_conv(typename Cx<CT>::T *__restric
 template <typename CT> struct PmeShapes {
   GridGeom g;
   BrickGeom bg;
-  int nbricks, nrows, npair, rb, wy, ncy, wz, ncz;
-  size_t shx, shy, shz;
+  int nslabs, nrows, npair, rb, wy, ncy, wz, ncz;
+  size_t shs, shx, shy, shz;
   ConvArgs c;
 };
 
-// rows: x-pass row pairs per block; cols: y/z-pass columns per block (both
-// capped by 64 KiB of LDS)
+// rows: x-inverse row pairs per block; cols: y/z-pass columns per block (both
+// capped by 64 KiB of LDS).  The spread slab is kSlabBy x kSlabBz rows, halved
+// (z first, then y down to 2) until its LDS fits.
 template <typename CT>
 static PmeShapes<CT> pme_shapes(const NbSystem &S, int rows = kFftRowsPerBlock,
                                 int cols = kFftColsPerBlock) {
@@ -684,10 +702,18 @@ static PmeShapes<CT> pme_shapes(const NbSystem &S, int rows = kFftRowsPerBlock,
   PmeShapes<CT> P;
   const int K0 = S.K[0], K1 = S.K[1], K2 = S.K[2], H = K0 / 2 + 1;
   P.g = GridGeom{K0, K1, K2, 1.0 / S.box[0], 1.0 / S.box[1], 1.0 / S.box[2]};
-  P.bg = BrickGeom{K0, K1, K2, (K0 + kBx - 1) / kBx, (K1 + kBy - 1) / kBy, (K2 + kBz - 1) / kBz,
-                   (float)S.box[0], (float)S.box[1], (float)S.box[2]};
-  P.nbricks = P.bg.nbx * P.bg.nby * P.bg.nbz;
   const size_t lds_cap = 64 * 1024;
+  int by = std::max(2, kSlabBy & ~1), bz = std::max(1, kSlabBz);
+  while (slab_lds<C2>(K0, by, bz) > lds_cap - 2048 && (bz > 1 || by > 2)) {
+    if (bz > 1) bz /= 2;
+    else by /= 2;
+  }
+  if (slab_lds<C2>(K0, by, bz) > lds_cap - 2048)
+    throw InvalidArgument("PME grid x dimension too large for the spread slab");
+  P.bg = BrickGeom{K0, K1, K2, by, bz, (K1 + by - 1) / by, (K2 + bz - 1) / bz, (float)S.box[0],
+                   (float)S.box[1], (float)S.box[2]};
+  P.nslabs = P.bg.nby * P.bg.nbz;
+  P.shs = slab_lds<C2>(K0, by, bz);
   P.nrows = K1 * K2;
   P.rb = (int)std::max<size_t>(
       1, std::min<size_t>((size_t)std::max(rows, 1), (lds_cap / sizeof(C2) - K0) / (2 * K0)));
@@ -718,30 +744,26 @@ static void recip_t(NbSystem &S, const float4 *x, int64_t *force, int stride, bo
   const double qunscale = 1.0;  // sorted coordinates carry the raw charge
   hipEvent_t t0 = nullptr;
   S.tic(1, s, t0);
-  CT *rgrid = reinterpret_cast<CT *>(S.qgrid.get());
-  k_spread_brick<CT><<<P.nbricks, 256, 0, s>>>(S.posq.get(), S.cell_bb.get(), S.meta.get(), P.bg,
-                                               qunscale, rgrid);
+  k_spread_fftx<CT><<<P.nslabs, 256, P.shs, s>>>(S.posq.get(), S.cell_bb.get(), S.meta.get(),
+                                                  P.bg, qunscale, plan_of(S.fx), twx, cgrid);
   APO_LAUNCH_CHECK();
   S.toc(1, s, t0);
   S.tic(2, s, t0);
-  k_fftx_fwd<CT><<<(P.npair + P.rb - 1) / P.rb, 256, P.shx, s>>>(rgrid, P.nrows, plan_of(S.fx),
-                                                                  P.rb, twx, cgrid);
-  APO_LAUNCH_CHECK();
-  k_ffty<CT><<<K2 * P.ncy, 256, P.shy, s>>>(cgrid, H, K1, P.wy, plan_of(S.fy), twy, 0);
+  k_ffty<CT><<<K2 * P.ncy, kFftThreads, P.shy, s>>>(cgrid, H, K1, P.wy, plan_of(S.fy), twy, 0);
   APO_LAUNCH_CHECK();
   if (K1 * P.ncz > S.part.recip_n) throw RuntimeError("reciprocal partial buffer too small");
   double *part = S.parts.get() + S.part.recip_off;
   if (EV) {
     launch_fill_f64(part, (size_t)S.part.recip_n * kRecipPart, s);
-    k_fftz_conv<CT, true><<<K1 * P.ncz, 256, P.shz, s>>>(cgrid, P.c, plan_of(S.fz), twz, pre, part);
+    k_fftz_conv<CT, true><<<K1 * P.ncz, kFftThreads, P.shz, s>>>(cgrid, P.c, plan_of(S.fz), twz, pre, part);
   } else {
-    k_fftz_conv<CT, false><<<K1 * P.ncz, 256, P.shz, s>>>(cgrid, P.c, plan_of(S.fz), twz, pre,
+    k_fftz_conv<CT, false><<<K1 * P.ncz, kFftThreads, P.shz, s>>>(cgrid, P.c, plan_of(S.fz), twz, pre,
                                                           part);
   }
   APO_LAUNCH_CHECK();
-  k_ffty<CT><<<K2 * P.ncy, 256, P.shy, s>>>(cgrid, H, K1, P.wy, plan_of(S.fy), twy, 1);
+  k_ffty<CT><<<K2 * P.ncy, kFftThreads, P.shy, s>>>(cgrid, H, K1, P.wy, plan_of(S.fy), twy, 1);
   APO_LAUNCH_CHECK();
-  k_fftx_inv<CT><<<(P.npair + P.rb - 1) / P.rb, 256, P.shx, s>>>(cgrid, P.nrows, plan_of(S.fx),
+  k_fftx_inv<CT><<<(P.npair + P.rb - 1) / P.rb, kFftThreads, P.shx, s>>>(cgrid, P.nrows, plan_of(S.fx),
                                                                   P.rb, twx, phi);
   APO_LAUNCH_CHECK();
   S.toc(2, s, t0);
@@ -769,17 +791,17 @@ struct SpreadArgs {
   const float4 *bb;
   const int *meta;
   BrickGeom g;
-  float *grid;
+  FftPlan p;
+  const float2 *tw;
+  float2 *cout;  // complex half rows out
 };
 struct FftxArgs {
-  const float *grid;   // forward: real grid in
-  const float2 *cin;   // inverse: complex half rows in
+  const float2 *cin;  // complex half rows in
   int nrows;
   FftPlan p;
   int rb;
   const float2 *tw;
-  float2 *cout;        // forward: complex half rows out
-  float *phi;          // inverse: potential out
+  float *phi;         // potential out
 };
 struct FftyArgs {
   float2 *data;
@@ -803,17 +825,12 @@ struct GatherArgs {
   int stride;
 };
 
-__global__ __launch_bounds__(256) void kb_spread(const SpreadArgs *__restrict__ tab,
-                                                 const int *__restrict__ bstart, int nsys) {
+__global__ __launch_bounds__(256) void kb_spread_fftx(const SpreadArgs *__restrict__ tab,
+                                                      const int *__restrict__ bstart, int nsys) {
   const int s = batch_system(bstart, nsys);
   const SpreadArgs &a = tab[s];
-  spread_body<float>(blockIdx.x - bstart[s], a.posq, a.bb, a.meta, a.g, 1.0, a.grid);
-}
-__global__ __launch_bounds__(256) void kb_fftx_fwd(const FftxArgs *__restrict__ tab,
-                                                   const int *__restrict__ bstart, int nsys) {
-  const int s = batch_system(bstart, nsys);
-  const FftxArgs &a = tab[s];
-  fftx_fwd_body<float>(blockIdx.x - bstart[s], a.grid, a.nrows, a.p, a.rb, a.tw, a.cout);
+  spread_fftx_body<float>(blockIdx.x - bstart[s], a.posq, a.bb, a.meta, a.g, 1.0, a.p, a.tw,
+                          a.cout);
 }
 __global__ __launch_bounds__(256) void kb_fftx_inv(const FftxArgs *__restrict__ tab,
                                                    const int *__restrict__ bstart, int nsys) {
@@ -843,7 +860,7 @@ __global__ __launch_bounds__(256) void kb_gather(const GatherArgs *__restrict__ 
 
 struct PmeBatch {
   BatchLaunch<SpreadArgs> spread;
-  BatchLaunch<FftxArgs> fx_fwd, fx_inv;
+  BatchLaunch<FftxArgs> fx_inv;
   BatchLaunch<FftyArgs> fy;
   BatchLaunch<FftzArgs> fz;
   BatchLaunch<GatherArgs> gather;
@@ -853,7 +870,6 @@ PmeBatch *pme_batch_create() { return new PmeBatch(); }
 void pme_batch_destroy(PmeBatch *b) { delete b; }
 void pme_batch_clear(PmeBatch &b) {
   b.spread.clear();
-  b.fx_fwd.clear();
   b.fx_inv.clear();
   b.fy.clear();
   b.fz.clear();
@@ -865,13 +881,12 @@ void pme_batch_add(PmeBatch &b, NbSystem &S, int64_t *force, int stride, int fft
   if (S.precision != 0) throw InvalidArgument("fused batches run the mixed-precision path");
   const PmeShapes<float> P = pme_shapes<float>(S, fft_rows, fft_cols);
   if (S.K[1] * P.ncz > S.part.recip_n) throw RuntimeError("reciprocal partial buffer too small");
-  float *rgrid = reinterpret_cast<float *>(S.qgrid.get());
   const int H = S.K[0] / 2 + 1;
-  b.spread.add(SpreadArgs{S.posq.get(), S.cell_bb.get(), S.meta.get(), P.bg, rgrid}, P.nbricks);
-  FftxArgs xa{rgrid, S.cgrid.get(), P.nrows, plan_of(S.fx), P.rb, S.fx.tw.get(), S.cgrid.get(),
-              S.phi.get()};
-  b.fx_fwd.add(xa, (P.npair + P.rb - 1) / P.rb, P.shx);
-  b.fx_inv.add(xa, (P.npair + P.rb - 1) / P.rb, P.shx);
+  b.spread.add(SpreadArgs{S.posq.get(), S.cell_bb.get(), S.meta.get(), P.bg, plan_of(S.fx),
+                          S.fx.tw.get(), S.cgrid.get()},
+               P.nslabs, P.shs);
+  b.fx_inv.add(FftxArgs{S.cgrid.get(), P.nrows, plan_of(S.fx), P.rb, S.fx.tw.get(), S.phi.get()},
+               (P.npair + P.rb - 1) / P.rb, P.shx);
   b.fy.add(FftyArgs{S.cgrid.get(), H, S.K[1], P.wy, plan_of(S.fy), S.fy.tw.get()},
            S.K[2] * P.ncy, P.shy);
   b.fz.add(FftzArgs{S.cgrid.get(), P.c, plan_of(S.fz), S.fz.tw.get(), S.prefac.get()},
@@ -883,29 +898,26 @@ void pme_batch_add(PmeBatch &b, NbSystem &S, int64_t *force, int stride, int fft
 
 void pme_batch_upload(PmeBatch &b, hipStream_t s) {
   b.spread.upload(s);
-  b.fx_fwd.upload(s);
   b.fx_inv.upload(s);
   b.fy.upload(s);
   b.fz.upload(s);
   b.gather.upload(s);
 }
 
-// spread, the five FFT passes with the convolution, gather: one launch each
-// for all systems (cell boxes must be current).  Without the gather the
+// spread + x FFT, the other four FFT passes with the convolution, gather: one
+// launch each for all systems (cell boxes must be current).  Without the gather the
 // launches write only the batch's grids (the autotuner times them so).
 void pme_batch_launch(PmeBatch &b, hipStream_t s, bool gather) {
   if (!b.spread.nsys()) return;
-  kb_spread<<<b.spread.blocks(), 256, 0, s>>>(b.spread.d_args.get(), b.spread.d_start.get(),
-                                              b.spread.nsys());
-  kb_fftx_fwd<<<b.fx_fwd.blocks(), 256, b.fx_fwd.lds, s>>>(b.fx_fwd.d_args.get(),
-                                                           b.fx_fwd.d_start.get(), b.fx_fwd.nsys());
-  kb_ffty<<<b.fy.blocks(), 256, b.fy.lds, s>>>(b.fy.d_args.get(), b.fy.d_start.get(), b.fy.nsys(),
+  kb_spread_fftx<<<b.spread.blocks(), 256, b.spread.lds, s>>>(
+      b.spread.d_args.get(), b.spread.d_start.get(), b.spread.nsys());
+  kb_ffty<<<b.fy.blocks(), kFftThreads, b.fy.lds, s>>>(b.fy.d_args.get(), b.fy.d_start.get(), b.fy.nsys(),
                                                0);
-  kb_fftz_conv<<<b.fz.blocks(), 256, b.fz.lds, s>>>(b.fz.d_args.get(), b.fz.d_start.get(),
+  kb_fftz_conv<<<b.fz.blocks(), kFftThreads, b.fz.lds, s>>>(b.fz.d_args.get(), b.fz.d_start.get(),
                                                     b.fz.nsys());
-  kb_ffty<<<b.fy.blocks(), 256, b.fy.lds, s>>>(b.fy.d_args.get(), b.fy.d_start.get(), b.fy.nsys(),
+  kb_ffty<<<b.fy.blocks(), kFftThreads, b.fy.lds, s>>>(b.fy.d_args.get(), b.fy.d_start.get(), b.fy.nsys(),
                                                1);
-  kb_fftx_inv<<<b.fx_inv.blocks(), 256, b.fx_inv.lds, s>>>(b.fx_inv.d_args.get(),
+  kb_fftx_inv<<<b.fx_inv.blocks(), kFftThreads, b.fx_inv.lds, s>>>(b.fx_inv.d_args.get(),
                                                            b.fx_inv.d_start.get(), b.fx_inv.nsys());
   if (gather)
     kb_gather<<<b.gather.blocks(), 256, 0, s>>>(b.gather.d_args.get(), b.gather.d_start.get(),

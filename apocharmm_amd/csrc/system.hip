@@ -199,7 +199,6 @@ void NbSystem::setup_pme() {
   make_fft_dim(fz, K[2], 0);
   size_t ng = (size_t)K[0] * K[1] * K[2];
   size_t nc = (size_t)(K[0] / 2 + 1) * K[1] * K[2];
-  qgrid.reserve(ng);
   if (precision == 1) {
     cgrid_d.reserve(nc);
     phi_d.reserve(ng);

@@ -94,7 +94,6 @@ struct NbSystem {
 
   // ---- PME ------------------------------------------------------------------
   FftDim fx, fy, fz;
-  DevBuf<int64_t> qgrid;     // K0*K1*K2 fixed-point charge grid
   DevBuf<float2> cgrid;      // (K0/2+1)*K1*K2 complex half grid
   DevBuf<float> phi;         // K0*K1*K2 potential grid
   DevBuf<double2> cgrid_d;   // FP64 instantiation
