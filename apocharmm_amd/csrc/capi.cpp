@@ -31,7 +31,7 @@ void lt_get_state(LtSystem &L, double *xyz, double *vel, double *dprev, double *
                   double *thermo, int64_t *step);
 const float *lt_xyzq(LtSystem &L);
 void lt_list_info(LtSystem &L, double *out4);
-void batch_autotune(LtBatch &B, hipStream_t caller, double *report8);
+void batch_autotune(LtBatch &B, hipStream_t caller, double *report9);
 }  // namespace apo
 
 struct apo_nb_system {
@@ -266,10 +266,10 @@ APO_API apo_status apo_lt_batch_propagate(apo_lt_batch *b, int32_t nsteps, void 
   });
 }
 
-APO_API apo_status apo_lt_batch_autotune(apo_lt_batch *b, void *stream, double *report8) {
+APO_API apo_status apo_lt_batch_autotune(apo_lt_batch *b, void *stream, double *report9) {
   return guard([&] {
     if (!b || !b->b) throw apo::InvalidArgument("null batch handle");
-    apo::batch_autotune(*b->b, static_cast<hipStream_t>(stream), report8);
+    apo::batch_autotune(*b->b, static_cast<hipStream_t>(stream), report9);
   });
 }
 

@@ -42,7 +42,7 @@ PmeBatch *pme_batch_create();
 void pme_batch_destroy(PmeBatch *b);
 void pme_batch_clear(PmeBatch &b);
 void pme_batch_add(PmeBatch &b, NbSystem &S, int64_t *force, int stride, int fft_rows,
-                   int fft_cols);
+                   int fft_cols, int fft_threads);
 void pme_batch_upload(PmeBatch &b, hipStream_t s);
 void pme_batch_launch(PmeBatch &b, hipStream_t s, bool gather);
 struct BondedBatch;
@@ -998,6 +998,10 @@ struct LtBatch {
   std::vector<hipStream_t> lane;
   hipEvent_t fork_ev = nullptr, m_ev = nullptr;
   std::vector<hipEvent_t> lane_ev;
+  // force branches inside a step: PME and bonded on side streams, the pair
+  // kernel on m (as calc_force forks for one system)
+  hipStream_t side[2] = {nullptr, nullptr};
+  hipEvent_t sort_ev = nullptr, side_ev[2] = {nullptr, nullptr};
   hipGraphExec_t ex[2] = {nullptr, nullptr};
   std::vector<unsigned long long> gens[2];
   // fused step: one launch per kernel type for all systems
@@ -1012,15 +1016,20 @@ struct LtBatch {
   bool tab_lds = true;
   std::vector<unsigned long long> tab_gens;
   // launch shapes (the autotuner's choices): pair-kernel blocks per CU
-  // shared out by atom count, FFT row pairs / columns per block
+  // shared out by atom count, FFT threads, row pairs and columns per block
   double pair_bpc = kPairBlocksPerCu;
-  int fft_rows = kFftRowsPerBlock, fft_cols = kFftColsPerBlock;
+  int fft_threads = kFftThreads, fft_rows = kFftRowsPerBlock, fft_cols = kFftColsPerBlock;
 
   ~LtBatch() {
     for (hipGraphExec_t g : ex)
       if (g) (void)hipGraphExecDestroy(g);
     for (hipEvent_t e : lane_ev) (void)hipEventDestroy(e);
     for (hipStream_t st : lane) (void)hipStreamDestroy(st);
+    for (int i = 0; i < 2; i++) {
+      if (side_ev[i]) (void)hipEventDestroy(side_ev[i]);
+      if (side[i]) (void)hipStreamDestroy(side[i]);
+    }
+    if (sort_ev) (void)hipEventDestroy(sort_ev);
     if (fork_ev) (void)hipEventDestroy(fork_ev);
     if (m_ev) (void)hipEventDestroy(m_ev);
     if (m) (void)hipStreamDestroy(m);
@@ -1057,7 +1066,7 @@ struct LtBatch {
       const int pb = std::max(8, (int)std::lround(pair_bpc * ncu * (L->n / ntot)));
       batch_add_pair(pair, S, pb);
       tab_lds = tab_lds && pair_tab_lds(S);
-      pme_batch_add(*pme, S, g, L->gstride, fft_rows, fft_cols);
+      pme_batch_add(*pme, S, g, L->gstride, fft_rows, fft_cols, fft_threads);
       bonded_batch_add(*bond, S, x, g, L->gstride);
       batch_add_finalize(fin, S, g, L->gstride);
       lstep.add(LtStepArgs{L->state(), L->C, L->counters.get(), L->ke_part.get(), L->dpart.get()},
@@ -1079,13 +1088,23 @@ struct LtBatch {
     for (size_t i = 0; i < sys.size(); i++) tab_gens[i] = sys[i]->nb->generation;
   }
 
-  // force path of every system + integrator update: 15 launches in all
+  // force path of every system + integrator update: 14 launches in all.
+  // After the sort, the PME chain and the bonded terms fork onto side
+  // streams and the pair kernel runs on s; all three write int64 sums, so
+  // their overlap changes no bit.
   void fused_step(hipStream_t s) {
     batch_fill_i64(fill, s);
     batch_sort(sort, s);
+    APO_HIP(hipEventRecord(sort_ev, s));
+    APO_HIP(hipStreamWaitEvent(side[0], sort_ev, 0));
+    pme_batch_launch(*pme, side[0], true);
+    APO_HIP(hipEventRecord(side_ev[0], side[0]));
+    APO_HIP(hipStreamWaitEvent(side[1], sort_ev, 0));
+    bonded_batch_launch(*bond, side[1]);
+    APO_HIP(hipEventRecord(side_ev[1], side[1]));
     batch_pair(pair, tab_lds, s);
-    pme_batch_launch(*pme, s, true);
-    bonded_batch_launch(*bond, s);
+    APO_HIP(hipStreamWaitEvent(s, side_ev[0], 0));
+    APO_HIP(hipStreamWaitEvent(s, side_ev[1], 0));
     batch_finalize(fin, s);
     kb_lt_step<<<lstep.blocks(), kStepBlock, 0, s>>>(lstep.d_args.get(), lstep.d_start.get(),
                                                      lstep.nsys());
@@ -1140,29 +1159,35 @@ struct LtBatch {
         if (t < best_pair) best_pair = t, best_bpc = bpc;
       }
       pair_bpc = best_bpc;
-      const int fft_cand[][2] = {{16, 16}, {8, 8}, {4, 4}, {16, 8}, {8, 16}, {4, 8}};
-      int best_r = kFftRowsPerBlock, best_c = kFftColsPerBlock;
+      // (threads, row pairs, columns) per block of the y, z and x-inverse passes
+      const int fft_cand[][3] = {{256, 16, 16}, {256, 8, 8}, {128, 8, 8}, {64, 4, 4},
+                                 {64, 8, 8},    {64, 2, 2},  {64, 4, 8}};
+      int best_t = kFftThreads, best_r = kFftRowsPerBlock, best_c = kFftColsPerBlock;
       double best_pme = 1e30, def_pme = 0;
-      for (const auto &rc : fft_cand) {
-        fft_rows = rc[0];
-        fft_cols = rc[1];
+      for (const auto &c : fft_cand) {
+        fft_threads = c[0];
+        fft_rows = c[1];
+        fft_cols = c[2];
         build_tables();
         const double t = timed([&] { pme_batch_launch(*pme, m, false); });
-        if (rc[0] == kFftRowsPerBlock && rc[1] == kFftColsPerBlock) def_pme = t;
-        if (t < best_pme) best_pme = t, best_r = rc[0], best_c = rc[1];
+        if (c[0] == kFftThreads && c[1] == kFftRowsPerBlock && c[2] == kFftColsPerBlock)
+          def_pme = t;
+        if (t < best_pme) best_pme = t, best_t = c[0], best_r = c[1], best_c = c[2];
       }
+      fft_threads = best_t;
       fft_rows = best_r;
       fft_cols = best_c;
       build_tables();
       if (report) {
         report[0] = pair_bpc;
-        report[1] = fft_rows;
-        report[2] = fft_cols;
-        report[3] = best_pair;
-        report[4] = best_pme;
-        report[5] = def_pair;
-        report[6] = def_pme;
-        report[7] = ntimed;
+        report[1] = fft_threads;
+        report[2] = fft_rows;
+        report[3] = fft_cols;
+        report[4] = best_pair;
+        report[5] = best_pme;
+        report[6] = def_pair;
+        report[7] = def_pme;
+        report[8] = ntimed;
       }
     } catch (...) {
       (void)hipEventDestroy(e0);
@@ -1296,6 +1321,11 @@ LtBatch *create_batch(LtSystem **Ls, int n, int lanes) {
       APO_HIP(hipStreamCreateWithFlags(&B->lane[l], hipStreamNonBlocking));
       APO_HIP(hipEventCreateWithFlags(&B->lane_ev[l], hipEventDisableTiming));
     }
+    APO_HIP(hipEventCreateWithFlags(&B->sort_ev, hipEventDisableTiming));
+    for (int i = 0; i < 2; i++) {
+      APO_HIP(hipStreamCreateWithFlags(&B->side[i], hipStreamNonBlocking));
+      APO_HIP(hipEventCreateWithFlags(&B->side_ev[i], hipEventDisableTiming));
+    }
     for (LtSystem *L : B->sys) {
       if (L->nb->precision != 0)
         throw InvalidArgument("a batch runs the mixed-precision force path only");
@@ -1313,8 +1343,8 @@ LtBatch *create_batch(LtSystem **Ls, int n, int lanes) {
 
 void destroy_batch(LtBatch *B) { delete B; }
 
-void batch_autotune(LtBatch &B, hipStream_t caller, double *report8) {
-  B.autotune(caller, report8);
+void batch_autotune(LtBatch &B, hipStream_t caller, double *report9) {
+  B.autotune(caller, report9);
 }
 
 void batch_propagate(LtBatch &B, int nsteps, hipStream_t caller) {

@@ -120,8 +120,11 @@ __device__ __forceinline__ void fft_stage(const C2 *in, C2 *out, int n, int nrow
 
 // nrow unnormalised FFTs of length p.n held in LDS buffer a (b = scratch).
 // Returns the buffer holding the result.
+// Always inlined: as a called function it makes every caller allocate the
+// callee's worst-case register budget (262 VGPRs, one wave per SIMD).
 template <typename C2>
-__device__ C2 *lds_fft(C2 *a, C2 *b, const FftPlan &p, int nrow, const C2 *tw, bool inv) {
+__device__ __forceinline__ C2 *lds_fft(C2 *a, C2 *b, const FftPlan &p, int nrow, const C2 *tw,
+                                       bool inv) {
   int Ns = 1;
   for (int s = 0; s < p.nrad; s++) {
     const int R = p.rad[s];
@@ -208,9 +211,14 @@ struct BrickGeom {
 // would otherwise begin right after any static arrays): int64 rows, twiddles,
 // two complex FFT buffers, then the candidate cells and their count.
 constexpr int kSlabCand = 1024;  // cells tested per round (a multiple of 256)
+// Row stride of the int64 slab, >= K0 and = 4 (mod 32) elements: the 64 lanes
+// of one stencil (4 x points, 4 y rows, 4 z rows) then fall on distinct LDS
+// banks in each half wave; a stride that is a multiple of 16 puts all 16 rows
+// of a lane group on the same banks.
+__host__ __device__ __forceinline__ int slab_stride(int K0) { return K0 + ((36 - K0 % 32) % 32); }
 template <typename C2> static size_t slab_lds(int K0, int by, int bz) {
   const size_t rows = (size_t)by * bz;
-  return rows * K0 * sizeof(unsigned long long) + (size_t)K0 * sizeof(C2) +
+  return rows * slab_stride(K0) * sizeof(unsigned long long) + (size_t)K0 * sizeof(C2) +
          rows * K0 * sizeof(C2) + (kSlabCand + 4) * sizeof(int);
 }
 
@@ -233,14 +241,15 @@ __device__ __forceinline__ long long rint_i64(double v) {
   return __double_as_longlong(v + magic) - __double_as_longlong(magic);
 }
 
-// fr = K frac(u + 1/2) from the 32-bit fixed-point box coordinate u.
+// fr = K frac(u + 1/2) from the 32-bit fixed-point box coordinate u, in
+// integers: with v = u + 2^31 (mod 2^32) and K v < 2^43, ix = floor(K v / 2^32)
+// and w = (K v mod 2^32) / 2^32.  Every step of the FP64 form
+// K (u/2^32 + 1/2 - floor(u/2^32 + 1/2)) is exact, so the bits are the same.
 __device__ __forceinline__ void frac_fixed(int ufix, int K, int &ix, double &w) {
-  double u = (double)(uint32_t)ufix * (1.0 / 4294967296.0) + 0.5;
-  u -= floor(u);
-  double fr = (double)K * u;
-  if (!(fr >= 0.0 && fr < (double)K)) fr = 0.0;
-  ix = (int)fr;
-  w = fr - (double)ix;
+  const unsigned v = (unsigned)ufix + 0x80000000u;
+  const unsigned long long p = (unsigned long long)v * (unsigned)K;
+  ix = (int)(p >> 32);
+  w = (double)(unsigned)(p & 0xffffffffull) * (1.0 / 4294967296.0);
 }
 
 // 6 x the order-4 B-spline weight of stencil point t at offset w:
@@ -279,7 +288,8 @@ __device__ __forceinline__ void spread_fftx_body(int bid, const int4 *__restrict
   extern __shared__ char smem[];  // no static LDS in this kernel: smem is at offset 0
   const int n = g.K0, by = g.by, bz = g.bz, nrow = by * bz;
   unsigned long long *slab = reinterpret_cast<unsigned long long *>(smem);
-  C2 *tw = reinterpret_cast<C2 *>(slab + (size_t)nrow * n);
+  const int ss = slab_stride(n);
+  C2 *tw = reinterpret_cast<C2 *>(slab + (size_t)nrow * ss);
   C2 *a = tw + n;
   C2 *b = a + (nrow / 2) * n;
   int *cand = reinterpret_cast<int *>(b + (nrow / 2) * n);
@@ -287,7 +297,7 @@ __device__ __forceinline__ void spread_fftx_body(int bid, const int4 *__restrict
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int byi = bid % g.nby, bzi = bid / g.nby;
   const int y0 = byi * by, z0 = bzi * bz;
-  for (int k = tid; k < nrow * n; k += 256) slab[k] = 0ull;
+  for (int k = tid; k < nrow * ss; k += 256) slab[k] = 0ull;
   for (int e = tid; e < n; e += 256) tw[e] = twg[e];
   const int ncell = meta[0];
   // atoms whose stencil base lies in [p0 - 3, p0 + B) reach the slab in y and z
@@ -349,29 +359,40 @@ __device__ __forceinline__ void spread_fftx_body(int bid, const int4 *__restrict
           frac_fixed(q4.x, g.K0, ix, wx);
           frac_fixed(q4.y, g.K1, iy, wy);
           frac_fixed(q4.z, g.K2, iz, wz);
-          // offsets of the stencil base relative to the slab origin, in [-3, K-4]
+          // offsets of the stencil base relative to the slab origin, reduced
+          // into [-3, K-4] (dy + 3 lies in [4 - K, K + 2])
           dy = iy - y0;
           dz = iz - z0;
-          dy -= g.K1 * (int)floorf((float)(dy + 3) / K1);
-          dz -= g.K2 * (int)floorf((float)(dz + 3) / K2);
+          dy += dy + 3 < 0 ? g.K1 : (dy + 3 >= g.K1 ? -g.K1 : 0);
+          dz += dz + 3 < 0 ? g.K2 : (dz + 3 >= g.K2 ? -g.K2 : 0);
           qs = q * (kGridScale / 216.0);
         }
       }
       unsigned long long hit = __ballot(dy < by && dz < bz);
-      while (hit) {
-        const int k = __builtin_ctzll(hit);
-        hit &= hit - 1ull;
+      // two atoms per pass: their weight products are independent chains
+      auto contrib = [&](int k, int &at) -> long long {
         int px = __builtin_amdgcn_readlane(ix, k) + sa;
         const int py = __builtin_amdgcn_readlane(dy, k) + sb;
         const int pz = __builtin_amdgcn_readlane(dz, k) + sc;
         const double awx = readlane_f64(wx, k), awy = readlane_f64(wy, k),
                      awz = readlane_f64(wz, k), aqs = readlane_f64(qs, k);
         if (px >= n) px -= n;
-        if (py >= 0 && py < by && pz >= 0 && pz < bz) {
-          const double v = aqs * theta6(cx, awx) * theta6(cy, awy) * theta6(cz, awz);
-          const long long iv = rint_i64(v);
-          if (iv) atomicAdd(&slab[(pz * by + py) * n + px], (unsigned long long)iv);
-        }
+        at = -1;
+        if (py < 0 || py >= by || pz < 0 || pz >= bz) return 0;
+        at = (pz * by + py) * ss + px;
+        return rint_i64(aqs * theta6(cx, awx) * theta6(cy, awy) * theta6(cz, awz));
+      };
+      while (hit) {
+        const int k1 = __builtin_ctzll(hit);
+        hit &= hit - 1ull;
+        const bool two = hit != 0ull;
+        const int k2 = two ? __builtin_ctzll(hit) : k1;
+        if (two) hit &= hit - 1ull;
+        int at1, at2;
+        const long long iv1 = contrib(k1, at1);
+        const long long iv2 = two ? contrib(k2, at2) : 0ll;
+        if (iv1) atomicAdd(&slab[at1], (unsigned long long)iv1);
+        if (two && iv2) atomicAdd(&slab[at2], (unsigned long long)iv2);
       }
     }
     __syncthreads();
@@ -383,10 +404,10 @@ __device__ __forceinline__ void spread_fftx_body(int bid, const int4 *__restrict
     const int r = e / n, x = e - r * n;
     const int pz = r / hb, py = 2 * (r - pz * hb);
     const int y = y0 + py, z = z0 + pz;
-    const unsigned long long *row = slab + (size_t)(pz * by + py) * n;
+    const unsigned long long *row = slab + (size_t)(pz * by + py) * ss;
     C2 v;
     v.x = (y < g.K1 && z < g.K2) ? (CT)((double)(long long)row[x] * (1.0 / kGridScale)) : (CT)0;
-    v.y = (y + 1 < g.K1 && z < g.K2) ? (CT)((double)(long long)row[n + x] * (1.0 / kGridScale))
+    v.y = (y + 1 < g.K1 && z < g.K2) ? (CT)((double)(long long)row[ss + x] * (1.0 / kGridScale))
                                      : (CT)0;
     a[e] = v;
   }
@@ -688,18 +709,22 @@ template <typename CT> struct PmeShapes {
   GridGeom g;
   BrickGeom bg;
   int nslabs, nrows, npair, rb, wy, ncy, wz, ncz;
+  int ft;  // threads per block of the y, z and x-inverse passes
   size_t shs, shx, shy, shz;
   ConvArgs c;
 };
 
 // rows: x-inverse row pairs per block; cols: y/z-pass columns per block (both
-// capped by 64 KiB of LDS).  The spread slab is kSlabBy x kSlabBz rows, halved
+// capped by 64 KiB of LDS); threads: block size of those passes (64..256).  The spread slab is kSlabBy x kSlabBz rows, halved
 // (z first, then y down to 2) until its LDS fits.
 template <typename CT>
 static PmeShapes<CT> pme_shapes(const NbSystem &S, int rows = kFftRowsPerBlock,
-                                int cols = kFftColsPerBlock) {
+                                int cols = kFftColsPerBlock, int threads = kFftThreads) {
   using C2 = typename Cx<CT>::T;
   PmeShapes<CT> P;
+  if (threads < 64 || threads > 256 || threads % 64)
+    throw InvalidArgument("FFT threads per block must be 64, 128, 192 or 256");
+  P.ft = threads;
   const int K0 = S.K[0], K1 = S.K[1], K2 = S.K[2], H = K0 / 2 + 1;
   P.g = GridGeom{K0, K1, K2, 1.0 / S.box[0], 1.0 / S.box[1], 1.0 / S.box[2]};
   const size_t lds_cap = 64 * 1024;
@@ -749,21 +774,21 @@ static void recip_t(NbSystem &S, const float4 *x, int64_t *force, int stride, bo
   APO_LAUNCH_CHECK();
   S.toc(1, s, t0);
   S.tic(2, s, t0);
-  k_ffty<CT><<<K2 * P.ncy, kFftThreads, P.shy, s>>>(cgrid, H, K1, P.wy, plan_of(S.fy), twy, 0);
+  k_ffty<CT><<<K2 * P.ncy, P.ft, P.shy, s>>>(cgrid, H, K1, P.wy, plan_of(S.fy), twy, 0);
   APO_LAUNCH_CHECK();
   if (K1 * P.ncz > S.part.recip_n) throw RuntimeError("reciprocal partial buffer too small");
   double *part = S.parts.get() + S.part.recip_off;
   if (EV) {
     launch_fill_f64(part, (size_t)S.part.recip_n * kRecipPart, s);
-    k_fftz_conv<CT, true><<<K1 * P.ncz, kFftThreads, P.shz, s>>>(cgrid, P.c, plan_of(S.fz), twz, pre, part);
+    k_fftz_conv<CT, true><<<K1 * P.ncz, P.ft, P.shz, s>>>(cgrid, P.c, plan_of(S.fz), twz, pre, part);
   } else {
-    k_fftz_conv<CT, false><<<K1 * P.ncz, kFftThreads, P.shz, s>>>(cgrid, P.c, plan_of(S.fz), twz, pre,
+    k_fftz_conv<CT, false><<<K1 * P.ncz, P.ft, P.shz, s>>>(cgrid, P.c, plan_of(S.fz), twz, pre,
                                                           part);
   }
   APO_LAUNCH_CHECK();
-  k_ffty<CT><<<K2 * P.ncy, kFftThreads, P.shy, s>>>(cgrid, H, K1, P.wy, plan_of(S.fy), twy, 1);
+  k_ffty<CT><<<K2 * P.ncy, P.ft, P.shy, s>>>(cgrid, H, K1, P.wy, plan_of(S.fy), twy, 1);
   APO_LAUNCH_CHECK();
-  k_fftx_inv<CT><<<(P.npair + P.rb - 1) / P.rb, kFftThreads, P.shx, s>>>(cgrid, P.nrows, plan_of(S.fx),
+  k_fftx_inv<CT><<<(P.npair + P.rb - 1) / P.rb, P.ft, P.shx, s>>>(cgrid, P.nrows, plan_of(S.fx),
                                                                   P.rb, twx, phi);
   APO_LAUNCH_CHECK();
   S.toc(2, s, t0);
@@ -864,6 +889,7 @@ struct PmeBatch {
   BatchLaunch<FftyArgs> fy;
   BatchLaunch<FftzArgs> fz;
   BatchLaunch<GatherArgs> gather;
+  int ft = kFftThreads;  // threads per block of the y, z and x-inverse passes
 };
 
 PmeBatch *pme_batch_create() { return new PmeBatch(); }
@@ -877,9 +903,10 @@ void pme_batch_clear(PmeBatch &b) {
 }
 
 void pme_batch_add(PmeBatch &b, NbSystem &S, int64_t *force, int stride, int fft_rows,
-                   int fft_cols) {
+                   int fft_cols, int fft_threads) {
   if (S.precision != 0) throw InvalidArgument("fused batches run the mixed-precision path");
-  const PmeShapes<float> P = pme_shapes<float>(S, fft_rows, fft_cols);
+  const PmeShapes<float> P = pme_shapes<float>(S, fft_rows, fft_cols, fft_threads);
+  b.ft = P.ft;
   if (S.K[1] * P.ncz > S.part.recip_n) throw RuntimeError("reciprocal partial buffer too small");
   const int H = S.K[0] / 2 + 1;
   b.spread.add(SpreadArgs{S.posq.get(), S.cell_bb.get(), S.meta.get(), P.bg, plan_of(S.fx),
@@ -911,13 +938,13 @@ void pme_batch_launch(PmeBatch &b, hipStream_t s, bool gather) {
   if (!b.spread.nsys()) return;
   kb_spread_fftx<<<b.spread.blocks(), 256, b.spread.lds, s>>>(
       b.spread.d_args.get(), b.spread.d_start.get(), b.spread.nsys());
-  kb_ffty<<<b.fy.blocks(), kFftThreads, b.fy.lds, s>>>(b.fy.d_args.get(), b.fy.d_start.get(), b.fy.nsys(),
+  kb_ffty<<<b.fy.blocks(), b.ft, b.fy.lds, s>>>(b.fy.d_args.get(), b.fy.d_start.get(), b.fy.nsys(),
                                                0);
-  kb_fftz_conv<<<b.fz.blocks(), kFftThreads, b.fz.lds, s>>>(b.fz.d_args.get(), b.fz.d_start.get(),
+  kb_fftz_conv<<<b.fz.blocks(), b.ft, b.fz.lds, s>>>(b.fz.d_args.get(), b.fz.d_start.get(),
                                                     b.fz.nsys());
-  kb_ffty<<<b.fy.blocks(), kFftThreads, b.fy.lds, s>>>(b.fy.d_args.get(), b.fy.d_start.get(), b.fy.nsys(),
+  kb_ffty<<<b.fy.blocks(), b.ft, b.fy.lds, s>>>(b.fy.d_args.get(), b.fy.d_start.get(), b.fy.nsys(),
                                                1);
-  kb_fftx_inv<<<b.fx_inv.blocks(), kFftThreads, b.fx_inv.lds, s>>>(b.fx_inv.d_args.get(),
+  kb_fftx_inv<<<b.fx_inv.blocks(), b.ft, b.fx_inv.lds, s>>>(b.fx_inv.d_args.get(),
                                                            b.fx_inv.d_start.get(), b.fx_inv.nsys());
   if (gather)
     kb_gather<<<b.gather.blocks(), 256, 0, s>>>(b.gather.d_args.get(), b.gather.d_start.get(),

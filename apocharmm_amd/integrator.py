@@ -237,15 +237,15 @@ class LangevinBatch:
             self._handle, int(numSteps), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
 
     def autotune(self):
-        """Picks the pair-kernel block budget and the FFT blocking by timing
+        """Picks the pair-kernel block budget and the FFT block shape by timing
         (apo_lt_batch_autotune); returns the choice and the timings."""
         torch = _torch()
-        out = (ctypes.c_double * 8)()
+        out = (ctypes.c_double * 9)()
         _lib.check(_lib.lib().apo_lt_batch_autotune(
             self._handle, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream), out))
-        return dict(pair_blocks_per_cu=out[0], fft_rows=int(out[1]), fft_cols=int(out[2]),
-                    pair_ms=out[3], pme_ms=out[4], default_pair_ms=out[5],
-                    default_pme_ms=out[6], launches_timed=int(out[7]))
+        return dict(pair_blocks_per_cu=out[0], fft_threads=int(out[1]), fft_rows=int(out[2]),
+                    fft_cols=int(out[3]), pair_ms=out[4], pme_ms=out[5], default_pair_ms=out[6],
+                    default_pme_ms=out[7], launches_timed=int(out[8]))
 
     def close(self):
         if self._handle:

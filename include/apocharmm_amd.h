@@ -330,12 +330,13 @@ APO_API apo_status apo_lt_batch_destroy(apo_lt_batch *b);
 
 /* Autotuner of the batch's launch shapes (no counterpart in the reference):
  * times the fused pair kernel over eight block budgets and the fused spread +
- * FFT launches over six FFT blockings on the systems' current coordinates and
- * lists, keeps the fastest, and reports report8 = blocks per CU, FFT rows,
- * FFT columns, the chosen pair and PME times (ms), the default shape's pair
- * and PME times (ms), launches timed.  Trajectories are unchanged (the timed
- * launches write per-step scratch only).  Synchronises. */
-APO_API apo_status apo_lt_batch_autotune(apo_lt_batch *b, void *stream, double *report8);
+ * FFT launches over seven FFT shapes (threads, row pairs, columns per block)
+ * on the systems' current coordinates and lists, keeps the fastest, and
+ * reports report9 = blocks per CU, FFT threads, FFT rows, FFT columns, the
+ * chosen pair and PME times (ms), the default shape's pair and PME times (ms),
+ * launches timed.  Trajectories are unchanged (the timed launches write
+ * per-step scratch only).  Synchronises. */
+APO_API apo_status apo_lt_batch_autotune(apo_lt_batch *b, void *stream, double *report9);
 
 /* Copies the state to the host (synchronises): any output may be NULL.
  *  xyz, vel, dprev, grad: double[3*natom] AoS; vel is the on-step velocity of
