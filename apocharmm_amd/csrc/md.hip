@@ -1093,6 +1093,15 @@ struct LtBatch {
   // streams and the pair kernel runs on s; all three write int64 sums, so
   // their overlap changes no bit.
   void fused_step(hipStream_t s) {
+    force_step(s);
+    kb_lt_step<<<lstep.blocks(), kStepBlock, 0, s>>>(lstep.d_args.get(), lstep.d_start.get(),
+                                                     lstep.nsys());
+    kb_lt_finalize<<<lfin.nsys(), 256, 0, s>>>(lfin.d_args.get());
+    APO_LAUNCH_CHECK();
+  }
+  // the force part: every system's gradient from its current coordinates
+  // (rewrites per-step scratch and the gradient only)
+  void force_step(hipStream_t s) {
     batch_fill_i64(fill, s);
     batch_sort(sort, s);
     APO_HIP(hipEventRecord(sort_ev, s));
@@ -1106,21 +1115,19 @@ struct LtBatch {
     APO_HIP(hipStreamWaitEvent(s, side_ev[0], 0));
     APO_HIP(hipStreamWaitEvent(s, side_ev[1], 0));
     batch_finalize(fin, s);
-    kb_lt_step<<<lstep.blocks(), kStepBlock, 0, s>>>(lstep.d_args.get(), lstep.d_start.get(),
-                                                     lstep.nsys());
-    kb_lt_finalize<<<lfin.nsys(), 256, 0, s>>>(lfin.d_args.get());
-    APO_LAUNCH_CHECK();
   }
 
-  // Autotuner of the launch shapes.  It times the fused pair kernel for
-  // eight block budgets and the fused spread + FFT launches for several
-  // FFT blockings on the systems' current coordinates and lists, and keeps
-  // the fastest.  Those launches write only per-step scratch (the sorted
-  // force buffer, the grids), which the next step rewrites from scratch, and
-  // each row, column and item is computed the same way in any blocking, so
-  // trajectories do not change.  report[8]: chosen blocks per CU, FFT rows,
-  // FFT columns, their pair and PME times (ms), the default shape's pair and
-  // PME times, candidates timed.
+  // Autotuner of the launch shapes.  It times the fused spread + FFT
+  // launches for seven FFT block shapes, then the whole force step (fill,
+  // sort, the PME, bonded and pair branches, finalize) for eight pair-kernel
+  // block budgets, on the systems' current coordinates and lists, and keeps
+  // the fastest.  Those launches rewrite per-step scratch and the gradient,
+  // which each step recomputes before using it, and each row, column and item
+  // is computed the same way in any blocking, so trajectories do not change.
+  // report[9]:
+  // chosen blocks per CU, FFT threads, rows and columns, the force-step and
+  // PME-chain times (ms) of the choice and of the default shapes, candidates
+  // timed.
   void autotune(hipStream_t caller, double *report) {
     APO_HIP(hipEventRecord(m_ev, caller));
     APO_HIP(hipStreamWaitEvent(m, m_ev, 0));
@@ -1149,17 +1156,8 @@ struct LtBatch {
       return ms / 3.0;
     };
     try {
-      const double bpc_cand[] = {8, 16, 24, 32, 48, 64, 96, 128};
-      double best_bpc = kPairBlocksPerCu, best_pair = 1e30, def_pair = 0;
-      for (double bpc : bpc_cand) {
-        pair_bpc = bpc;
-        build_tables();
-        const double t = timed([&] { batch_pair(pair, tab_lds, m); });
-        if (bpc == kPairBlocksPerCu) def_pair = t;
-        if (t < best_pair) best_pair = t, best_bpc = bpc;
-      }
-      pair_bpc = best_bpc;
-      // (threads, row pairs, columns) per block of the y, z and x-inverse passes
+      // FFT shape first, on the PME chain alone: (threads, row pairs, columns)
+      // per block of the y, z and x-inverse passes
       const int fft_cand[][3] = {{256, 16, 16}, {256, 8, 8}, {128, 8, 8}, {64, 4, 4},
                                  {64, 8, 8},    {64, 2, 2},  {64, 4, 8}};
       int best_t = kFftThreads, best_r = kFftRowsPerBlock, best_c = kFftColsPerBlock;
@@ -1177,15 +1175,28 @@ struct LtBatch {
       fft_threads = best_t;
       fft_rows = best_r;
       fft_cols = best_c;
+      // then the pair kernel's block budget, on the whole force step: the pair
+      // kernel shares the GPU with the PME and bonded branches, and a budget
+      // that is fastest alone can starve the PME chain
+      const double bpc_cand[] = {8, 16, 24, 32, 48, 64, 96, 128};
+      double best_bpc = kPairBlocksPerCu, best_force = 1e30, def_force = 0;
+      for (double bpc : bpc_cand) {
+        pair_bpc = bpc;
+        build_tables();
+        const double t = timed([&] { force_step(m); });
+        if (bpc == kPairBlocksPerCu) def_force = t;
+        if (t < best_force) best_force = t, best_bpc = bpc;
+      }
+      pair_bpc = best_bpc;
       build_tables();
       if (report) {
         report[0] = pair_bpc;
         report[1] = fft_threads;
         report[2] = fft_rows;
         report[3] = fft_cols;
-        report[4] = best_pair;
+        report[4] = best_force;
         report[5] = best_pme;
-        report[6] = def_pair;
+        report[6] = def_force;
         report[7] = def_pme;
         report[8] = ntimed;
       }

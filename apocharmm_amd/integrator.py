@@ -244,8 +244,8 @@ class LangevinBatch:
         _lib.check(_lib.lib().apo_lt_batch_autotune(
             self._handle, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream), out))
         return dict(pair_blocks_per_cu=out[0], fft_threads=int(out[1]), fft_rows=int(out[2]),
-                    fft_cols=int(out[3]), pair_ms=out[4], pme_ms=out[5], default_pair_ms=out[6],
-                    default_pme_ms=out[7], launches_timed=int(out[8]))
+                    fft_cols=int(out[3]), force_ms=out[4], pme_ms=out[5],
+                    default_force_ms=out[6], default_pme_ms=out[7], launches_timed=int(out[8]))
 
     def close(self):
         if self._handle:

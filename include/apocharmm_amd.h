@@ -329,13 +329,16 @@ APO_API apo_status apo_lt_batch_propagate(apo_lt_batch *b, int32_t nsteps, void 
 APO_API apo_status apo_lt_batch_destroy(apo_lt_batch *b);
 
 /* Autotuner of the batch's launch shapes (no counterpart in the reference):
- * times the fused pair kernel over eight block budgets and the fused spread +
- * FFT launches over seven FFT shapes (threads, row pairs, columns per block)
- * on the systems' current coordinates and lists, keeps the fastest, and
- * reports report9 = blocks per CU, FFT threads, FFT rows, FFT columns, the
- * chosen pair and PME times (ms), the default shape's pair and PME times (ms),
- * launches timed.  Trajectories are unchanged (the timed launches write
- * per-step scratch only).  Synchronises. */
+ * times the fused spread + FFT launches over seven FFT shapes (threads, row
+ * pairs, columns per block), then the whole force step (PME, bonded and pair
+ * branches running together) over eight pair-kernel block budgets, on the
+ * systems' current coordinates and lists, keeps the fastest, and reports
+ * report9 = blocks per CU, FFT threads, FFT rows, FFT columns, the chosen
+ * force-step and PME-chain times (ms), the default shapes' force-step and
+ * PME-chain times (ms), launches timed.  Trajectories are unchanged: the timed
+ * launches rewrite per-step scratch, and the gradient, which every step
+ * recomputes before it uses it (until the next step, apo_lt_get_state's grad
+ * then holds the gradient at the current coordinates).  Synchronises. */
 APO_API apo_status apo_lt_batch_autotune(apo_lt_batch *b, void *stream, double *report9);
 
 /* Copies the state to the host (synchronises): any output may be NULL.

@@ -337,7 +337,7 @@ def test_md_batch_autotune_keeps_trajectories():
     b = LangevinBatch([lt for _, lt in runs], lanes=2)
     rep = b.autotune()
     print("\n[autotune] %s" % rep, flush=True)
-    assert rep["launches_timed"] == 15 and rep["pair_ms"] <= rep["default_pair_ms"]
+    assert rep["launches_timed"] == 15 and rep["force_ms"] <= rep["default_force_ms"]
     assert rep["pme_ms"] <= rep["default_pme_ms"]
     b.propagate(15)
     b.autotune()
