@@ -205,12 +205,6 @@ struct PartLayout {
 struct NbSystem;
 
 void launch_fill_i64(int64_t *p, size_t n, hipStream_t s);
-struct FillArgs {
-  int64_t *p;
-  size_t n;
-};
-void batch_fill_i64(BatchLaunch<FillArgs> &b, hipStream_t s);
-int fill_blocks(size_t n);
 
 // Argument structs of the fused (multi-system) launches of the step kernels.
 struct SortArgs {
@@ -223,6 +217,8 @@ struct SortArgs {
   int64_t *fsort;
   int ss;
   float4 *bb;
+  int64_t *force;  // gradient to zero (nforce entries), or null
+  size_t nforce;
 };
 struct FinalizeArgs {
   const int64_t *fsort;
@@ -241,7 +237,10 @@ void launch_force_to_double(const int64_t *f, int natom, int stride,
 void nblist_build(NbSystem &sys, const float4 *d_xyzq, hipStream_t s, bool sizing);
 void list_export_check(NbSystem &sys, hipStream_t s, bool allow_growth);
 void list_reserve(NbSystem &sys, int ncand, int nent, int nitem);
-void nblist_sort_coords(NbSystem &sys, const float4 *d_xyzq, hipStream_t s);
+// Sorts the coordinates into cell order (and the cell boxes); when force is
+// given, also zeroes its nforce entries.
+void nblist_sort_coords(NbSystem &sys, const float4 *d_xyzq, hipStream_t s,
+                        int64_t *force = nullptr, size_t nforce = 0);
 void nblist_stats(NbSystem &sys, int64_t *out6, hipStream_t s);
 
 void direct_launch(NbSystem &sys, bool energy, bool virial, hipStream_t s);

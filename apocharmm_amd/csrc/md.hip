@@ -30,7 +30,8 @@ void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int s
 void list_export_check(NbSystem &S, hipStream_t s, bool allow_growth);
 
 // fused (multi-system) launches of the step kernels, one per kernel type
-void batch_add_sort(BatchLaunch<SortArgs> &b, NbSystem &S, const float4 *x);
+void batch_add_sort(BatchLaunch<SortArgs> &b, NbSystem &S, const float4 *x, int64_t *force,
+                    size_t nforce);
 void batch_sort(BatchLaunch<SortArgs> &b, hipStream_t s);
 void batch_add_pair(BatchLaunch<PairArgs> &b, NbSystem &S, int blocks);
 void batch_pair(BatchLaunch<PairArgs> &b, bool tab_lds, hipStream_t s);
@@ -1005,7 +1006,6 @@ struct LtBatch {
   hipGraphExec_t ex[2] = {nullptr, nullptr};
   std::vector<unsigned long long> gens[2];
   // fused step: one launch per kernel type for all systems
-  BatchLaunch<FillArgs> fill;
   BatchLaunch<SortArgs> sort;
   BatchLaunch<PairArgs> pair;
   BatchLaunch<FinalizeArgs> fin;
@@ -1045,7 +1045,6 @@ struct LtBatch {
   // per-system argument tables of the fused launches (pointers and launch
   // constants move only with a system's generation)
   void build_tables() {
-    fill.clear();
     sort.clear();
     pair.clear();
     fin.clear();
@@ -1061,8 +1060,7 @@ struct LtBatch {
       NbSystem &S = *L->nb;
       const float4 *x = L->xyzq.get();
       int64_t *g = L->grad.get();
-      fill.add(FillArgs{g, 3 * (size_t)L->gstride}, fill_blocks(3 * (size_t)L->gstride));
-      batch_add_sort(sort, S, x);
+      batch_add_sort(sort, S, x, g, 3 * (size_t)L->gstride);
       const int pb = std::max(8, (int)std::lround(pair_bpc * ncu * (L->n / ntot)));
       batch_add_pair(pair, S, pb);
       tab_lds = tab_lds && pair_tab_lds(S);
@@ -1075,7 +1073,6 @@ struct LtBatch {
                          L->counters.get(), L->dpart.get(), S.meta.get(), L->h_disp},
                1);
     }
-    fill.upload(m);
     sort.upload(m);
     pair.upload(m);
     fin.upload(m);
@@ -1088,7 +1085,7 @@ struct LtBatch {
     for (size_t i = 0; i < sys.size(); i++) tab_gens[i] = sys[i]->nb->generation;
   }
 
-  // force path of every system + integrator update: 14 launches in all.
+  // force path of every system + integrator update: 13 launches in all.
   // After the sort, the PME chain and the bonded terms fork onto side
   // streams and the pair kernel runs on s; all three write int64 sums, so
   // their overlap changes no bit.
@@ -1102,8 +1099,7 @@ struct LtBatch {
   // the force part: every system's gradient from its current coordinates
   // (rewrites per-step scratch and the gradient only)
   void force_step(hipStream_t s) {
-    batch_fill_i64(fill, s);
-    batch_sort(sort, s);
+    batch_sort(sort, s);  // also zeroes every system's gradient
     APO_HIP(hipEventRecord(sort_ev, s));
     APO_HIP(hipStreamWaitEvent(side[0], sort_ev, 0));
     pme_batch_launch(*pme, side[0], true);

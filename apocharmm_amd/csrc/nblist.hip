@@ -164,8 +164,13 @@ __device__ __forceinline__ void sort_coords_body(int bid, const float4 *__restri
                                                  int4 *__restrict__ posq,
                                                  int *__restrict__ type_s,
                                                  int64_t *__restrict__ fsort, int ss,
-                                                 float4 *__restrict__ bb) {
+                                                 float4 *__restrict__ bb,
+                                                 int64_t *__restrict__ force, size_t nforce,
+                                                 int nb) {
   const int loc = bid * 256 + threadIdx.x;
+  // the step's gradient starts from zero (every term adds into it after the sort)
+  if (force)
+    for (size_t k = loc; k < nforce; k += (size_t)nb * 256) force[k] = 0;
   const int ncell = meta[0];
   const bool in = loc < ncell * kCell;
   const int g = in ? loc2glo[loc] : -1;
@@ -213,16 +218,18 @@ __global__ __launch_bounds__(256) void k_sort_coords(const float4 *__restrict__ 
                                                      int4 *__restrict__ posq,
                                                      int *__restrict__ type_s,
                                                      int64_t *__restrict__ fsort, int ss,
-                                                     float4 *__restrict__ bb) {
+                                                     float4 *__restrict__ bb,
+                                                     int64_t *__restrict__ force, size_t nforce) {
   sort_coords_body(blockIdx.x, xyzq, vdwtype, loc2glo, meta, inv_len, qscale, unit, posq, type_s,
-                   fsort, ss, bb);
+                   fsort, ss, bb, force, nforce, gridDim.x);
 }
 __global__ __launch_bounds__(256) void kb_sort_coords(const SortArgs *__restrict__ tab,
                                                       const int *__restrict__ bstart, int nsys) {
   const int s = batch_system(bstart, nsys);
   const SortArgs a = tab[s];
   sort_coords_body(blockIdx.x - bstart[s], a.xyzq, a.vdwtype, a.loc2glo, a.meta, a.inv_len, 1.0f,
-                   a.unit, a.posq, a.type_s, a.fsort, a.ss, a.bb);
+                   a.unit, a.posq, a.type_s, a.fsort, a.ss, a.bb, a.force, a.nforce,
+                   bstart[s + 1] - bstart[s]);
 }
 
 __device__ __forceinline__ float bb_gap2(float4 ci, float4 hi, float4 cj, float4 hj, float3 L) {
@@ -564,7 +571,8 @@ void nblist_build(NbSystem &S, const float4 *x, hipStream_t s, bool sizing) {
   S.list_valid = true;
 }
 
-void nblist_sort_coords(NbSystem &S, const float4 *x, hipStream_t s) {
+void nblist_sort_coords(NbSystem &S, const float4 *x, hipStream_t s, int64_t *force,
+                        size_t nforce) {
   const double3 inv = make_double3(1.0 / S.box[0], 1.0 / S.box[1], 1.0 / S.box[2]);
   const float qscale = 1.0f;  // raw charge; ccelec is applied per tile in the pair kernel
   const float3 unit = make_float3((float)(S.box[0] / 4294967296.0), (float)(S.box[1] / 4294967296.0),
@@ -572,7 +580,7 @@ void nblist_sort_coords(NbSystem &S, const float4 *x, hipStream_t s) {
   k_sort_coords<<<nblk(S.ssort, 256), 256, 0, s>>>(x, S.vdwtype.get(), S.loc2glo.get(),
                                                     S.meta.get(), inv, qscale, unit, S.posq.get(),
                                                     S.type_s.get(), S.fsort.get(), S.ssort,
-                                                    S.cell_bb.get());
+                                                    S.cell_bb.get(), force, nforce);
   APO_LAUNCH_CHECK();
 }
 
@@ -610,12 +618,14 @@ void nblist_stats(NbSystem &S, int64_t *out6, hipStream_t s) {
 }
 
 // ---- fused launches over several systems ------------------------------------
-void batch_add_sort(BatchLaunch<SortArgs> &b, NbSystem &S, const float4 *x) {
+void batch_add_sort(BatchLaunch<SortArgs> &b, NbSystem &S, const float4 *x, int64_t *force,
+                    size_t nforce) {
   SortArgs a{x, S.vdwtype.get(), S.loc2glo.get(), S.meta.get(),
              make_double3(1.0 / S.box[0], 1.0 / S.box[1], 1.0 / S.box[2]),
              make_float3((float)(S.box[0] / 4294967296.0), (float)(S.box[1] / 4294967296.0),
                          (float)(S.box[2] / 4294967296.0)),
-             S.posq.get(), S.type_s.get(), S.fsort.get(), S.ssort, S.cell_bb.get()};
+             S.posq.get(), S.type_s.get(), S.fsort.get(), S.ssort, S.cell_bb.get(), force,
+             nforce};
   b.add(a, nblk(S.ssort, 256));
 }
 void batch_sort(BatchLaunch<SortArgs> &b, hipStream_t s) {

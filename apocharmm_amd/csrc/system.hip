@@ -455,8 +455,10 @@ void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int s
   const float4 *x4 = reinterpret_cast<const float4 *>(xyzq);
   hipEvent_t t_all = nullptr, t0 = nullptr, t1 = nullptr;
   S.tic(5, s, t_all);
-  launch_fill_i64(force, 3 * (size_t)stride, s);
   const bool dir = terms & APO_TERM_DIRECT, rec = terms & APO_TERM_RECIP;
+  // the coordinate sort zeroes `force` when there is one; every term adds
+  // into it after the sort
+  if (!(dir || rec)) launch_fill_i64(force, 3 * (size_t)stride, s);
   const bool bnd = terms & (APO_TERM_PAIRS14 | APO_TERM_BONDED);
   // Three branches that meet before the sorted pair gradient is folded into
   // `force`:  bonded + 1-4 (side[1], int64 atomics into force) | pair kernel
@@ -472,7 +474,7 @@ void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int s
       S.toc(4, s, t0);
     }
     S.tic(7, s, t0);
-    nblist_sort_coords(S, x4, s);
+    nblist_sort_coords(S, x4, s, force, 3 * (size_t)stride);
     S.toc(7, s, t0);
     if (rec) {
       if (fork) {

@@ -9,23 +9,12 @@
 namespace apo {
 
 // Fills (kernels instead of hipMemsetAsync so the step stays graph-safe).
-__device__ __forceinline__ void fill_i64_body(int bid, int nb, int64_t *p, size_t n) {
-  for (size_t i = bid * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)nb * blockDim.x)
+__global__ void k_fill_i64(int64_t *p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
     p[i] = 0;
 }
-__global__ void k_fill_i64(int64_t *p, size_t n) { fill_i64_body(blockIdx.x, gridDim.x, p, n); }
-__global__ __launch_bounds__(256) void kb_fill_i64(const FillArgs *__restrict__ tab,
-                                                   const int *__restrict__ bstart, int nsys) {
-  const int s = batch_system(bstart, nsys);
-  const FillArgs a = tab[s];
-  fill_i64_body(blockIdx.x - bstart[s], bstart[s + 1] - bstart[s], a.p, a.n);
-}
-void batch_fill_i64(BatchLaunch<FillArgs> &b, hipStream_t s) {
-  if (!b.blocks()) return;
-  kb_fill_i64<<<b.blocks(), 256, 0, s>>>(b.d_args.get(), b.d_start.get(), b.nsys());
-  APO_LAUNCH_CHECK();
-}
-int fill_blocks(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 4096); }
+static int fill_blocks(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 4096); }
 __global__ void k_fill_f64(double *p, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
