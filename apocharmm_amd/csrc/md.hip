@@ -260,33 +260,63 @@ __device__ inline float wave_max(float v) {
   return v;
 }
 
-// One thread per water (first nwater threads) or free atom (the rest).
+// Three waves per block of kStepBlock units (a unit is a water, or an atom in
+// no water): wave k draws the random force and takes the unconstrained step
+// for atom k of each water (wave 0 alone for free atoms), wave 0 then applies
+// SETTLE to each water from LDS, and every wave finishes its atoms.  The
+// per-atom arithmetic does not depend on the split.
+constexpr int kStepThreads = 3 * kStepBlock;
 __device__ __forceinline__ void lt_step_body(int bid, const LtState &S, const LtCoef &C,
                                              const unsigned long long *__restrict__ stepc,
                                              double2 *__restrict__ part,
                                              float2 *__restrict__ dpart) {
-  const int u = bid * kStepBlock + threadIdx.x;
+  __shared__ double old_x[3][3][kStepBlock], new_x[3][3][kStepBlock];
+  __shared__ double red_ke[2][3];
+  __shared__ float red_d[2][3];
+  const int lane = threadIdx.x & (kStepBlock - 1), k = threadIdx.x / kStepBlock;
+  const int u = bid * kStepBlock + lane;
   const unsigned long long step = stepc[0];
+  const bool water = u < S.nwater;
+  const bool single = !water && k == 0 && u < S.nwater + S.nsingle;
+  const int i = water ? S.waters[3 * u + k] : (single ? S.singles[u - S.nwater] : -1);
+  double3 r = make_double3(0, 0, 0), p = r, xn = r;
+  if (i >= 0) {
+    r = ld3(S.x, S.n, i);
+    p = ld3(S.dp, S.n, i);
+    xn = lt_position(S, C, step, i, r, p);
+  }
+  if (water) {
+    old_x[k][0][lane] = r.x;
+    old_x[k][1][lane] = r.y;
+    old_x[k][2][lane] = r.z;
+    new_x[k][0][lane] = xn.x;
+    new_x[k][1][lane] = xn.y;
+    new_x[k][2][lane] = xn.z;
+  }
+  __syncthreads();
+  if (water && k == 0) {
+    const double3 ra = make_double3(old_x[0][0][lane], old_x[0][1][lane], old_x[0][2][lane]);
+    const double3 rb = make_double3(old_x[1][0][lane], old_x[1][1][lane], old_x[1][2][lane]);
+    const double3 rc = make_double3(old_x[2][0][lane], old_x[2][1][lane], old_x[2][2][lane]);
+    double3 na = make_double3(new_x[0][0][lane], new_x[0][1][lane], new_x[0][2][lane]);
+    double3 nb = make_double3(new_x[1][0][lane], new_x[1][1][lane], new_x[1][2][lane]);
+    double3 nc = make_double3(new_x[2][0][lane], new_x[2][1][lane], new_x[2][2][lane]);
+    settle(C.G, ra, rb, rc, na, nb, nc);
+    new_x[0][0][lane] = na.x;
+    new_x[0][1][lane] = na.y;
+    new_x[0][2][lane] = na.z;
+    new_x[1][0][lane] = nb.x;
+    new_x[1][1][lane] = nb.y;
+    new_x[1][2][lane] = nb.z;
+    new_x[2][0][lane] = nc.x;
+    new_x[2][1][lane] = nc.y;
+    new_x[2][2][lane] = nc.z;
+  }
+  __syncthreads();
   double ke0 = 0.0, ke1 = 0.0;
   float d2 = 0.f, s2 = 0.f;
-  if (u < S.nwater) {
-    const int ia = S.waters[3 * u], ib = S.waters[3 * u + 1], ic = S.waters[3 * u + 2];
-    const double3 ra = ld3(S.x, S.n, ia), rb = ld3(S.x, S.n, ib), rc = ld3(S.x, S.n, ic);
-    const double3 pa = ld3(S.dp, S.n, ia), pb = ld3(S.dp, S.n, ib), pc = ld3(S.dp, S.n, ic);
-    double3 na = lt_position(S, C, step, ia, ra, pa);
-    double3 nb = lt_position(S, C, step, ib, rb, pb);
-    double3 nc = lt_position(S, C, step, ic, rc, pc);
-    settle(C.G, ra, rb, rc, na, nb, nc);
-    lt_finish(S, C, ia, ra, na, pa, ke0, ke1);
-    lt_finish(S, C, ib, rb, nb, pb, ke0, ke1);
-    lt_finish(S, C, ic, rc, nc, pc, ke0, ke1);
-    disp_track(S, ia, ra, na, d2, s2);
-    disp_track(S, ib, rb, nb, d2, s2);
-    disp_track(S, ic, rc, nc, d2, s2);
-  } else if (u < S.nwater + S.nsingle) {
-    const int i = S.singles[u - S.nwater];
-    const double3 r = ld3(S.x, S.n, i), p = ld3(S.dp, S.n, i);
-    const double3 xn = lt_position(S, C, step, i, r, p);
+  if (i >= 0) {
+    if (water) xn = make_double3(new_x[k][0][lane], new_x[k][1][lane], new_x[k][2][lane]);
     lt_finish(S, C, i, r, xn, p, ke0, ke1);
     disp_track(S, i, r, xn, d2, s2);
   }
@@ -294,15 +324,24 @@ __device__ __forceinline__ void lt_step_body(int bid, const LtState &S, const Lt
   ke1 = wave_sum(ke1);
   d2 = wave_max(d2);
   s2 = wave_max(s2);
+  if (lane == 0) {
+    red_ke[0][k] = ke0;
+    red_ke[1][k] = ke1;
+    red_d[0][k] = d2;
+    red_d[1][k] = s2;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    part[bid] = make_double2(ke0, ke1);
-    dpart[bid] = make_float2(d2, s2);
+    part[bid] = make_double2(red_ke[0][0] + red_ke[0][1] + red_ke[0][2],
+                             red_ke[1][0] + red_ke[1][1] + red_ke[1][2]);
+    dpart[bid] = make_float2(fmaxf(fmaxf(red_d[0][0], red_d[0][1]), red_d[0][2]),
+                             fmaxf(fmaxf(red_d[1][0], red_d[1][1]), red_d[1][2]));
   }
 }
-__global__ __launch_bounds__(kStepBlock) void k_lt_step(LtState S, LtCoef C,
-                                                        const unsigned long long *__restrict__ stepc,
-                                                        double2 *__restrict__ part,
-                                                        float2 *__restrict__ dpart) {
+__global__ __launch_bounds__(kStepThreads) void k_lt_step(LtState S, LtCoef C,
+                                                          const unsigned long long *__restrict__ stepc,
+                                                          double2 *__restrict__ part,
+                                                          float2 *__restrict__ dpart) {
   lt_step_body(blockIdx.x, S, C, stepc, part, dpart);
 }
 struct LtStepArgs {
@@ -312,7 +351,7 @@ struct LtStepArgs {
   double2 *part;
   float2 *dpart;
 };
-__global__ __launch_bounds__(kStepBlock) void kb_lt_step(const LtStepArgs *__restrict__ tab,
+__global__ __launch_bounds__(kStepThreads) void kb_lt_step(const LtStepArgs *__restrict__ tab,
                                                          const int *__restrict__ bstart, int nsys) {
   const int s = batch_system(bstart, nsys);
   const LtStepArgs &a = tab[s];
@@ -663,7 +702,7 @@ struct LtSystem {
   void update(hipStream_t s) {
     hipEvent_t t0 = nullptr;
     nb->tic(8, s, t0);
-    k_lt_step<<<nunit_blocks, kStepBlock, 0, s>>>(state(), C, counters.get(), ke_part.get(),
+    k_lt_step<<<nunit_blocks, kStepThreads, 0, s>>>(state(), C, counters.get(), ke_part.get(),
                                                   dpart.get());
     k_lt_finalize<<<1, 256, 0, s>>>(ke_part.get(), nunit_blocks, ndof, thermo.get(),
                                     counters.get(), dpart.get(), nb->meta.get(), h_disp);
@@ -1091,7 +1130,7 @@ struct LtBatch {
   // their overlap changes no bit.
   void fused_step(hipStream_t s) {
     force_step(s);
-    kb_lt_step<<<lstep.blocks(), kStepBlock, 0, s>>>(lstep.d_args.get(), lstep.d_start.get(),
+    kb_lt_step<<<lstep.blocks(), kStepThreads, 0, s>>>(lstep.d_args.get(), lstep.d_start.get(),
                                                      lstep.nsys());
     kb_lt_finalize<<<lfin.nsys(), 256, 0, s>>>(lfin.d_args.get());
     APO_LAUNCH_CHECK();
