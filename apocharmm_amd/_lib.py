@@ -69,7 +69,7 @@ class LtParams(ctypes.Structure):
                 ("nwater", ctypes.c_int32), ("use_graph", ctypes.c_int32),
                 ("waters", _i32p), ("water_geom", ctypes.c_double * 4),
                 ("mass", ctypes.POINTER(ctypes.c_double)), ("charge", _f32p),
-                ("list_policy", ctypes.c_int32)]
+                ("list_policy", ctypes.c_int32), ("prune_every", ctypes.c_int32)]
 
 
 _LIB = None

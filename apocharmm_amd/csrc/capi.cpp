@@ -129,6 +129,8 @@ APO_API apo_status apo_nb_calc_force(apo_nb_system *sys, const float *d_xyzq, in
                                      int64_t *d_force, int32_t stride, double *d_energy,
                                      double *d_virial, void *stream) {
   return guard([&] {
+    constexpr int32_t known = APO_CALC_ENERGY | APO_CALC_VIRIAL | APO_RESET_LIST | APO_TERM_ALL;
+    if (flags & ~known) throw apo::InvalidArgument("unknown calc_force flag bits");
     apo::calc_force(get(sys), d_xyzq, flags, d_force, stride, d_energy, d_virial,
                     reinterpret_cast<hipStream_t>(stream));
   });

@@ -242,8 +242,16 @@ void list_reserve(NbSystem &sys, int ncand, int nent, int nitem);
 void nblist_sort_coords(NbSystem &sys, const float4 *d_xyzq, hipStream_t s,
                         int64_t *force = nullptr, size_t nforce = 0);
 void nblist_stats(NbSystem &sys, int64_t *out6, hipStream_t s);
+// Prunes the list at radius sqrt(prune_r2) on the current sorted coordinates
+// into ent_p / items_p.
+void nblist_prune(NbSystem &sys, hipStream_t s);
+// calc_force flags for internal callers (no public flag uses these bits): the
+// pair kernel reads the pruned list (which a build in the same call, or
+// kCalcPrune, refreshes after the coordinate sort).
+constexpr int kCalcUsePruned = 1 << 27;
+constexpr int kCalcPrune = 1 << 28;
 
-void direct_launch(NbSystem &sys, bool energy, bool virial, hipStream_t s);
+void direct_launch(NbSystem &sys, bool energy, bool virial, hipStream_t s, bool pruned = false);
 void direct_finalize(NbSystem &sys, int64_t *d_force, int stride,
                      hipStream_t s);
 void pairs14_launch(NbSystem &sys, const float4 *d_xyzq, int64_t *d_force,

@@ -629,6 +629,10 @@ struct LtSystem {
   // since the last build (at most nbfreq steps apart)
   int list_policy = 0;
   double half_buffer = 0.0;              // (cutnb - ctofnb) / 2
+  // list pruning (policy 0 only): every prune_every steps between builds the
+  // pair kernel's list is pruned at r_off + (cutnb - r_off) prune_every /
+  // nbfreq; 0 = the build's list every step
+  int prune_every = 0;
   unsigned long long builds_triggered = 0;
   // step completion events of policy 1 (the host looks at most kLag steps
   // behind the GPU)
@@ -640,9 +644,10 @@ struct LtSystem {
   hipStream_t st = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
   // captured step bodies: [0] plain step, [1] step with image centering and
-  // list rebuild; each is recaptured when the force system's generation moves
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
-  unsigned long long graph_gen[2] = {~0ull, ~0ull};
+  // list rebuild, [2] step with a list prune; each is recaptured when the
+  // force system's generation moves
+  hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
+  unsigned long long graph_gen[3] = {~0ull, ~0ull, ~0ull};
 
   ~LtSystem() {
     for (hipGraphExec_t g : gexec)
@@ -674,9 +679,11 @@ struct LtSystem {
     return s;
   }
 
-  void force(hipStream_t s, bool reset) {
-    calc_force(*nb, reinterpret_cast<const float *>(xyzq.get()), reset ? APO_RESET_LIST : 0,
-               grad.get(), gstride, nullptr, nullptr, s);
+  void force(hipStream_t s, bool reset, bool prune = false) {
+    int flags = reset ? APO_RESET_LIST : 0;
+    if (prune_every > 0) flags |= kCalcUsePruned | (prune ? kCalcPrune : 0);
+    calc_force(*nb, reinterpret_cast<const float *>(xyzq.get()), flags, grad.get(), gstride,
+               nullptr, nullptr, s);
   }
 
   void center(hipStream_t s) {
@@ -710,9 +717,9 @@ struct LtSystem {
     nb->toc(8, s, t0);
   }
 
-  // force + integrator update + kinetic energy
-  void step_body(hipStream_t s) {
-    force(s, false);
+  // force + integrator update + kinetic energy (after a list prune)
+  void step_body(hipStream_t s, bool prune = false) {
+    force(s, false, prune);
     update(s);
   }
   // the same after image centering and a list rebuild
@@ -726,6 +733,7 @@ struct LtSystem {
     k_lt_init_constrain<<<nunit_blocks, kStepBlock, 0, s>>>(state(), C);
     APO_LAUNCH_CHECK();
     force(s, true);
+    if (prune_every > 0) nb->pruned_build = nb->builds_queued;
     k_lt_init_backstep<<<nunit_blocks, kStepBlock, 0, s>>>(state(), C);
     APO_LAUNCH_CHECK();
     initialized = true;
@@ -748,8 +756,8 @@ struct LtSystem {
       const unsigned long long bq = nb->builds_queued;
       APO_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       try {
-        if (which) rebuild_body(s);
-        else step_body(s);
+        if (which == 1) rebuild_body(s);
+        else step_body(s, which == 2);
       } catch (...) {
         (void)hipStreamEndCapture(s, &g);
         if (g) (void)hipGraphDestroy(g);
@@ -815,19 +823,25 @@ struct LtSystem {
         builds_triggered++;
         steps_since_rebuild = 0;
       }
+      // prune on the schedule, and whenever the pruned copy belongs to an
+      // earlier build (a rebuild outside this integrator, a fused batch)
+      const bool prune = !rebuild && prune_every > 0 &&
+                         (steps_since_rebuild % prune_every == 0 ||
+                          nb->pruned_build != nb->builds_queued);
       const bool com = comfreq > 0 && step % (unsigned long long)comfreq == 0;
       // centre-of-mass removal only touches the half-step displacements, so
       // it may precede the force call
       if (com) remove_com(s);
       if (!graph_usable()) {
         if (rebuild) rebuild_body(s);
-        else step_body(s);
+        else step_body(s, prune);
       } else {
         // errors and capacity growth of earlier (device-side) rebuilds are
         // handled before the next one is queued
         if (rebuild) list_export_check(*nb, s, true);
-        replay(rebuild ? 1 : 0, s);
+        replay(rebuild ? 1 : (prune ? 2 : 0), s);
       }
+      if (prune_every > 0 && (rebuild || prune)) nb->pruned_build = nb->builds_queued;
       if (list_policy == 1) APO_HIP(hipEventRecord(step_ev[step % kLag], s));
       steps_since_rebuild++;
       step++;
@@ -991,6 +1005,16 @@ LtSystem *create_lt(NbSystem *nb, const apo_lt_params &p, const double *xyz, con
     if (L->list_policy != 0 && L->list_policy != 1)
       throw InvalidArgument("list_policy must be 0 (fixed cadence) or 1 (displacement check)");
     L->half_buffer = 0.5 * (nb->cutnb - nb->ctofnb);
+    L->prune_every = p.prune_every;
+    if (L->prune_every < 0) throw InvalidArgument("prune_every must be >= 0");
+    if (L->prune_every > 0 && L->list_policy != 0)
+      throw InvalidArgument("list pruning needs the fixed rebuild cadence (list_policy 0)");
+    if (L->prune_every > 0) {
+      const double rp = std::min(nb->cutnb, nb->ctofnb + (nb->cutnb - nb->ctofnb) *
+                                                             (double)L->prune_every / L->nbfreq);
+      nb->prune_r2 = (float)(rp * rp);
+      nb->generation++;  // a launch constant of captured steps
+    }
     for (hipEvent_t &e : L->step_ev)
       APO_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     APO_HIP(hipStreamSynchronize(s));

@@ -393,6 +393,74 @@ __global__ __launch_bounds__(64 * kGroupsPerCell) void k_grp_emit(
   }
 }
 
+// List pruning (no counterpart in the reference; the idea of GROMACS's
+// dynamic pruning).  Between builds, every K steps, each group's entries of
+// the build's list are tested against the current coordinates with the
+// radius r_p = r_off + (cutnb - r_off) K / nbfreq, which assumes the same
+// largest relative motion per step as the build's own buffer; pairs beyond
+// r_p gain their skip bit, entries left with no live pair are dropped, the
+// rest are packed to the front of the group's region of ent_p (same region
+// as in ent, null padding after them) and the group's items are shortened.
+// Every index stays inside the group's region, so an item that becomes
+// empty still points at valid entries.  One wave per group.
+__global__ __launch_bounds__(64 * kGroupsPerCell) void k_prune(
+    const int *__restrict__ meta, const int *__restrict__ icell_start,
+    const int *__restrict__ grp_start, const int *__restrict__ grp_itemstart,
+    const int *__restrict__ cand_j, const int4 *__restrict__ posq,
+    const uint32_t *__restrict__ ent, const int4 *__restrict__ items, float3 unit, float rp2,
+    uint32_t *__restrict__ ent_p, int4 *__restrict__ items_p) {
+  constexpr uint32_t kFull = (1u << kGroup) - 1u;
+  const int icell = blockIdx.x;
+  if (icell >= meta[0] || meta[8]) return;
+  const int lane = threadIdx.x & 63, h = threadIdx.x >> 6;
+  const int g = icell * kGroupsPerCell + h;
+  const int st = grp_start[g], en = grp_start[g + 1];
+  const int *tiles = cand_j + icell_start[icell];
+  const int4 pil = posq[g * kGroup + (lane & (kGroup - 1))];
+  unsigned pix[kGroup], piy[kGroup], piz[kGroup];
+#pragma unroll
+  for (int k = 0; k < kGroup; k++) {
+    pix[k] = (unsigned)__builtin_amdgcn_readlane(pil.x, k);
+    piy[k] = (unsigned)__builtin_amdgcn_readlane(pil.y, k);
+    piz[k] = (unsigned)__builtin_amdgcn_readlane(pil.z, k);
+  }
+  const unsigned long long below = (1ull << lane) - 1ull;
+  int run = 0;
+  for (int p0 = st; p0 < en; p0 += 64) {  // en - st is a multiple of 64
+    const uint32_t e = ent[p0 + lane];
+    uint32_t m = e >> 16;
+    const int4 pj = posq[tiles[(e & 0xffffu) >> 5] * kCell + (int)(e & 31u)];
+#pragma unroll
+    for (int k = 0; k < kGroup; k++) {
+      const float dx = (float)(int)(pix[k] - (unsigned)pj.x) * unit.x;
+      const float dy = (float)(int)(piy[k] - (unsigned)pj.y) * unit.y;
+      const float dz = (float)(int)(piz[k] - (unsigned)pj.z) * unit.z;
+      if (!(dx * dx + dy * dy + dz * dz < rp2)) m |= 1u << k;
+    }
+    const bool live = m != kFull;
+    const unsigned long long b = __ballot(live);
+    if (live) ent_p[st + run + __popcll(b & below)] = (e & 0xffffu) | (m << 16);
+    run += __popcll(b);
+  }
+  for (int p = st + run + lane; p < en; p += 64) ent_p[p] = kFull << 16;
+  const int end = st + (run + 63) / 64 * 64;
+  for (int it = grp_itemstart[g] + lane; it < grp_itemstart[g + 1]; it += 64) {
+    int4 q = items[it];
+    q.z = max(q.y, min(q.z, end));
+    items_p[it] = q;
+  }
+}
+
+void nblist_prune(NbSystem &S, hipStream_t s) {
+  const float3 unit = make_float3((float)(S.box[0] / 4294967296.0), (float)(S.box[1] / 4294967296.0),
+                                  (float)(S.box[2] / 4294967296.0));
+  k_prune<<<S.ncell_cap, 64 * kGroupsPerCell, 0, s>>>(
+      S.meta.get(), S.icell_start.get(), S.grp_start.get(), S.grp_itemstart.get(),
+      S.cand_j.get(), S.posq.get(), S.ent.get(), S.items.get(), unit, S.prune_r2, S.ent_p.get(),
+      S.items_p.get());
+  APO_LAUNCH_CHECK();
+}
+
 // 10. After the counts are known: any capacity overflow or input error
 // leaves the list empty (no pair kernel work, no out-of-range write) and sets
 // meta[8]; the host learns of it from the exported counts.
@@ -459,11 +527,13 @@ void list_reserve(NbSystem &S, int ncand, int nent, int nitem) {
   if ((size_t)nent > S.ent_cap * 0.85 || S.ent_cap == 0) {
     S.ent_cap = (size_t)(nent * 1.5) + 64 * kItemSteps;
     S.ent.reserve(S.ent_cap);
+    S.ent_p.reserve(S.ent_cap);
     S.generation++;
   }
   if ((size_t)nitem > S.item_cap * 0.85 || S.item_cap == 0) {
     S.item_cap = (size_t)(nitem * 1.5) + 64;
     S.items.reserve(S.item_cap);
+    S.items_p.reserve(S.item_cap);
     S.generation++;
   }
 }

@@ -289,15 +289,17 @@ __global__ __launch_bounds__(64) void kb_pair(const PairArgs *__restrict__ tab,
 template <typename CT>
 static void launch_pair_t(NbSystem &S, const typename Vec2<CT>::T *tab,
                           const typename Vec2<CT>::T *sq, const PairConsts<CT> &P, CT qfac, bool E,
-                          bool V, hipStream_t s) {
+                          bool V, hipStream_t s, bool pruned) {
   auto *f = reinterpret_cast<unsigned long long *>(S.fsort.get());
   double *part = S.parts.get() + S.part.pair_off;
   const int G = S.pair_blocks;
   const bool lds = S.ntypes <= kMaxTypesLds;
   const size_t lds_bytes = (size_t)S.ntypes * S.ntypes * sizeof(typename Vec2<CT>::T);
+  const int4 *items = pruned ? S.items_p.get() : S.items.get();
+  const uint32_t *ent = pruned ? S.ent_p.get() : S.ent.get();
 #define APO_PAIR_ARGS                                                                   \
-  S.posq.get(), S.type_s.get(), S.items.get(), S.ent.get(), S.cand_j.get(), S.meta.get(), tab, \
-      sq, S.ntypes, P, qfac, f, S.ssort, part
+  S.posq.get(), S.type_s.get(), items, ent, S.cand_j.get(), S.meta.get(), tab, sq, S.ntypes, P, \
+      qfac, f, S.ssort, part
 #define APO_PAIR_GO(EE, VV)                                                             \
   if (lds)                                                                              \
     k_pair<CT, EE, VV, true><<<G, 64, lds_bytes, s>>>(APO_PAIR_ARGS);                  \
@@ -317,11 +319,12 @@ static void launch_pair_t(NbSystem &S, const typename Vec2<CT>::T *tab,
   APO_LAUNCH_CHECK();
 }
 
-void direct_launch(NbSystem &S, bool E, bool V, hipStream_t s) {
+void direct_launch(NbSystem &S, bool E, bool V, hipStream_t s, bool pruned) {
   if (S.precision == 1)
-    launch_pair_t<double>(S, S.vdwtab_d.get(), S.vdwsq_d.get(), S.pc_d, kCcelec, E, V, s);
+    launch_pair_t<double>(S, S.vdwtab_d.get(), S.vdwsq_d.get(), S.pc_d, kCcelec, E, V, s, pruned);
   else
-    launch_pair_t<float>(S, S.vdwtab.get(), S.vdwsq.get(), S.pc_f, (float)kCcelec, E, V, s);
+    launch_pair_t<float>(S, S.vdwtab.get(), S.vdwsq.get(), S.pc_f, (float)kCcelec, E, V, s,
+                         pruned);
 }
 
 // Sorted -> global order (addSortedForce, src/CudaPMEDirectForce.cu:1375-1386).

@@ -466,9 +466,11 @@ void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int s
   // gather (side[0], atomics into force).
   const bool fork = !S.single_stream;
   hipStream_t sb = fork ? S.side[1] : s, sp = fork ? S.side[0] : s;
+  bool rebuilt = false;
   if (dir || rec) {
     // both the pair kernel and PME work on the cell-sorted coordinates
     if ((flags & APO_RESET_LIST) || !S.list_valid) {
+      rebuilt = true;
       S.tic(4, s, t0);
       nblist_build(S, x4, s, false);
       S.toc(4, s, t0);
@@ -499,8 +501,11 @@ void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int s
     if (fork) APO_HIP(hipEventRecord(S.ev_bond, sb));
   }
   if (dir) {
+    // a pruned list is refreshed after every build and when asked
+    const bool pruned = flags & kCalcUsePruned;
+    if (pruned && (rebuilt || (flags & kCalcPrune))) nblist_prune(S, s);
     S.tic(0, s, t0);
-    direct_launch(S, E, V, s);
+    direct_launch(S, E, V, s, pruned);
     S.toc(0, s, t0);
   }
   if (fork && rec) APO_HIP(hipStreamWaitEvent(s, S.ev_pme, 0));

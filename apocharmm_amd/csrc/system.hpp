@@ -74,6 +74,13 @@ struct NbSystem {
   DevBuf<int> grp_cnt, grp_start, grp_nitem, grp_itemstart;
   DevBuf<uint32_t> ent;
   DevBuf<int4> items;
+  // pruned copy of the list (same layout and regions), read by the pair
+  // kernel in calc_force calls with kCalcUsePruned; pruned_build is the build
+  // (builds_queued) it was last derived from, kept by its user
+  DevBuf<uint32_t> ent_p;
+  DevBuf<int4> items_p;
+  float prune_r2 = 0.f;
+  unsigned long long pruned_build = ~0ull;
   size_t cand_cap = 0, ent_cap = 0, item_cap = 0;
   int host_ncell = 0, host_ncand = 0, host_nent = 0, host_nitem = 0;
   // counts and flags of each build, exported to pinned host memory (slot 15

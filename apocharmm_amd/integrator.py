@@ -39,6 +39,7 @@ class LangevinThermostatIntegrator:
         self._comfreq = 1000
         self._use_graph = True
         self._list_policy = 0
+        self._prune_every = 0
         self._handle = ctypes.c_void_p()
         self._fm = None
         self._keep = []
@@ -67,6 +68,12 @@ class LangevinThermostatIntegrator:
         "displacement": also when an atom may have moved more than half the
         list buffer since the last build."""
         self._list_policy = {"fixed": 0, "displacement": 1}[policy]
+
+    def setListPruneFrequency(self, every):
+        """Prunes the pair kernel's list every `every` steps between builds
+        at r_off + (cutnb - r_off) * every / nbfreq (no counterpart in the
+        reference; needs the fixed cadence); 0 turns it off."""
+        self._prune_every = int(every)
 
     def getTimeStep(self):
         return self._dt
@@ -103,6 +110,7 @@ class LangevinThermostatIntegrator:
         p.nwater = len(waters) // 3
         p.use_graph = 1 if self._use_graph else 0
         p.list_policy = self._list_policy
+        p.prune_every = self._prune_every
         p.waters = waters.ctypes.data_as(_lib._i32p)
         if water_geom is not None:
             for k in range(4):

@@ -394,6 +394,8 @@ def main():
     ap.add_argument("--precision", default="mixed")
     ap.add_argument("--step", choices=("md", "force"), default="md")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--prune-every", type=int, default=0,
+                    help="MD: prune the pair list every K steps between rebuilds (0 = off)")
     ap.add_argument("--nsys", type=int, default=64, help="systems in the c4 batch")
     ap.add_argument("--hw-queues", type=int, default=4,
                     help="GPU_MAX_HW_QUEUES for the c4 run (HIP's default is 4)")
@@ -457,6 +459,7 @@ def main():
         vel = maxwell_velocities(spec["masses"], TEMP, vseed)
         lt = LangevinThermostatIntegrator(DT_PS, friction=FRICTION, temperature=TEMP, seed=SEED)
         lt.setNonbondedListUpdateFrequency(REBUILD_EVERY)
+        lt.setListPruneFrequency(a.prune_every)
         lt.setCharmmContext(fm, np.asarray(spec["xyz"], np.float64), vel)
         lt.initialize()
 
@@ -526,6 +529,7 @@ def main():
                    "cutoffs_A": [float(spec["ctonnb"]), float(spec["ctofnb"]), float(spec["cutnb"])],
                    "kappa": float(spec["kappa"]), "dt_fs": DT_PS * 1000.0,
                    "list_rebuild_every": REBUILD_EVERY,
+                   "list_prune_every": a.prune_every if md else 0,
                    "step": what, "friction_per_ps": FRICTION if md else None,
                    "temperature_K": TEMP if md else None, "precision": a.precision,
                    "parallelism": "single GPU"},

@@ -277,6 +277,13 @@ typedef struct apo_lt_params {
                             the host, which reads it at most 4 steps late,
                             adds the largest one-step displacement for each
                             step it has not seen.  Batches need 0. */
+  int32_t prune_every;   /* no counterpart in the reference: with list_policy
+                            0, every prune_every steps between builds the pair
+                            kernel's list is pruned on the current coordinates
+                            at r_off + (cutnb - r_off) * prune_every / nbfreq
+                            (the build's own buffer per step, so a pair the
+                            build's list would reach is kept); 0 = off.
+                            Fused batch steps use the build's list. */
 } apo_lt_params;
 
 /* Binds an integrator to `sys` (setCharmmContext) with host coordinates

@@ -311,6 +311,52 @@ def test_md_displacement_policy_with_default_buffer_keeps_all_pairs(c1_equilibra
     assert rel <= 1e-6
 
 
+def _pruned_vs_fresh(start, nbfreq, prune_every, nsteps):
+    """Gradient of the last of nsteps steps, which the integrator took with its
+    pruned list, against a list built from the same coordinates (the step
+    began at xyz - dprev; both are exact in double): FP32 rounding plus every
+    pair inside r_off that the pruned list lacks."""
+    from apocharmm_amd.force_manager import ForceManager
+    from apocharmm_amd.integrator import LangevinThermostatIntegrator
+    sp, xyz, vel = start
+    fm = ForceManager(sp).initialize()
+    lt = LangevinThermostatIntegrator(DT_PS, friction=FRICTION, temperature=TEMP, seed=SEED)
+    lt.setNonbondedListUpdateFrequency(nbfreq)
+    lt.setListPruneFrequency(prune_every)
+    lt.setCharmmContext(fm, xyz, vel)
+    lt.propagate(nsteps)
+    st = lt.getState()
+    lt.close()
+    x = (st["xyz"] - st["dprev"]).astype(np.float32)
+    q = np.asarray(sp["charges"], np.float32)[:, None]
+    x4 = torch.tensor(np.hstack([x, q]), device="cuda")
+    fm.calcForce(x4, reset=True)
+    torch.cuda.synchronize()
+    g_fresh = fm.getForces().cpu().numpy().copy()
+    fm.close()
+    g = st["grad"].reshape(-1, 3)
+    err = float(np.abs(g - g_fresh).max())
+    rel = float(np.sqrt(np.mean(np.sum((g - g_fresh) ** 2, 1))) /
+                np.sqrt(np.mean(np.sum(g_fresh ** 2, 1))))
+    return err, rel
+
+
+def test_md_list_pruning_keeps_every_pair(c1_equilibrated):
+    """Pruning every 5 steps at r_off + 2 A * 5/20 (12.5 A): the last step of a
+    20-step interval (list 19 steps old, prune 4 steps old) has every pair
+    inside r_off; a control pruned once at 12.2 A and used for 99 steps loses
+    pairs."""
+    err, rel = _pruned_vs_fresh(c1_equilibrated, 20, 5, 40)
+    print("\n[prune 5/20] max |dg| %.2e rms rel %.2e" % (err, rel), flush=True)
+    err_c, rel_c = _pruned_vs_fresh(c1_equilibrated, 1000, 100, 100)
+    print("[prune 100/1000 control] max |dg| %.2e rms rel %.2e" % (err_c, rel_c), flush=True)
+    err_0, rel_0 = _pruned_vs_fresh(c1_equilibrated, 20, 0, 40)
+    print("[no pruning] max |dg| %.2e rms rel %.2e" % (err_0, rel_0), flush=True)
+    assert rel <= 1e-6 and rel_0 <= 1e-6
+    assert err <= 2 * err_0 + 1e-4
+    assert err_c > 10 * err
+
+
 def test_md_batch_autotune_keeps_trajectories():
     """apo_lt_batch_autotune between propagate calls: the chosen launch shapes
     change no bit of any system's trajectory."""
