@@ -1177,8 +1177,8 @@ struct LtBatch {
   }
 
   // Autotuner of the launch shapes.  It times the fused spread + FFT
-  // launches for seven FFT block shapes, then the whole force step (fill,
-  // sort, the PME, bonded and pair branches, finalize) for eight pair-kernel
+  // launches for seven FFT block shapes, then the whole force step (sort,
+  // the PME, bonded and pair branches, finalize) for eight pair-kernel
   // block budgets, on the systems' current coordinates and lists, and keeps
   // the fastest.  Those launches rewrite per-step scratch and the gradient,
   // which each step recomputes before using it, and each row, column and item

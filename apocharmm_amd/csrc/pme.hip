@@ -210,7 +210,11 @@ struct BrickGeom {
 // offset 0 (64-bit LDS atomics need 8-byte alignment, and the dynamic region
 // would otherwise begin right after any static arrays): int64 rows, twiddles,
 // two complex FFT buffers, then the candidate cells and their count.
-constexpr int kSlabCand = 1024;  // cells tested per round (a multiple of 256)
+constexpr int kSlabCand = 1024;  // cells tested per round (a multiple of kSpreadThreads)
+#ifndef APO_SPREAD_THREADS
+#define APO_SPREAD_THREADS 256
+#endif
+constexpr int kSpreadThreads = APO_SPREAD_THREADS;  // threads per slab block
 // Row stride of the int64 slab, >= K0 and = 4 (mod 32) elements: the 64 lanes
 // of one stencil (4 x points, 4 y rows, 4 z rows) then fall on distinct LDS
 // banks in each half wave; a stride that is a multiple of 16 puts all 16 rows
@@ -297,8 +301,8 @@ __device__ __forceinline__ void spread_fftx_body(int bid, const int4 *__restrict
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int byi = bid % g.nby, bzi = bid / g.nby;
   const int y0 = byi * by, z0 = bzi * bz;
-  for (int k = tid; k < nrow * ss; k += 256) slab[k] = 0ull;
-  for (int e = tid; e < n; e += 256) tw[e] = twg[e];
+  for (int k = tid; k < nrow * ss; k += kSpreadThreads) slab[k] = 0ull;
+  for (int e = tid; e < n; e += kSpreadThreads) tw[e] = twg[e];
   const int ncell = meta[0];
   // atoms whose stencil base lies in [p0 - 3, p0 + B) reach the slab in y and z
   const float by0 = (float)(y0 - 3), bz0 = (float)(z0 - 3);
@@ -314,18 +318,18 @@ __device__ __forceinline__ void spread_fftx_body(int bid, const int4 *__restrict
   for (int c0 = 0; c0 < ncell; c0 += kSlabCand) {
     if (tid == 0) ncand = 0;
     __syncthreads();
-    float4 cc[kSlabCand / 256], hh[kSlabCand / 256];
+    float4 cc[kSlabCand / kSpreadThreads], hh[kSlabCand / kSpreadThreads];
 #pragma unroll
-    for (int u = 0; u < kSlabCand / 256; u++) {
-      const int c = c0 + u * 256 + tid;
+    for (int u = 0; u < kSlabCand / kSpreadThreads; u++) {
+      const int c = c0 + u * kSpreadThreads + tid;
       if (c < ncell) {
         cc[u] = bb[2 * c];
         hh[u] = bb[2 * c + 1];
       }
     }
 #pragma unroll
-    for (int u = 0; u < kSlabCand / 256; u++) {
-      const int c = c0 + u * 256 + tid;
+    for (int u = 0; u < kSlabCand / kSpreadThreads; u++) {
+      const int c = c0 + u * kSpreadThreads + tid;
       if (c < ncell) {
         // cell extent in grid units (x/L + 1/2 origin), +2 for the floor and
         // single-precision slack
@@ -345,11 +349,12 @@ __device__ __forceinline__ void spread_fftx_body(int bid, const int4 *__restrict
       const int w = wave * 64 + lane;
       nxt = posq[cand[w >> 5] * kCell + (w & 31)];
     }
-    for (int base = wave * 64; base < na; base += 256) {
+    for (int base = wave * 64; base < na; base += kSpreadThreads) {
       // lane prepares atom base + lane
       const int w = base + lane;
       const int4 q4 = nxt;
-      if (w + 256 < na) nxt = posq[cand[(w + 256) >> 5] * kCell + ((w + 256) & 31)];
+      if (w + kSpreadThreads < na)
+        nxt = posq[cand[(w + kSpreadThreads) >> 5] * kCell + ((w + kSpreadThreads) & 31)];
       int ix = 0, dy = by, dz = bz;
       double wx = 0.0, wy = 0.0, wz = 0.0, qs = 0.0;
       if (w < na) {
@@ -400,7 +405,7 @@ __device__ __forceinline__ void spread_fftx_body(int bid, const int4 *__restrict
   // rows (y, y + 1) of one z plane -> one complex row; rows past the grid
   // (edge slabs) hold only wrapped duplicates and enter as zero
   const int hb = by >> 1, ncx = nrow >> 1, H = n / 2 + 1;
-  for (int e = tid; e < ncx * n; e += 256) {
+  for (int e = tid; e < ncx * n; e += kSpreadThreads) {
     const int r = e / n, x = e - r * n;
     const int pz = r / hb, py = 2 * (r - pz * hb);
     const int y = y0 + py, z = z0 + pz;
@@ -413,7 +418,7 @@ __device__ __forceinline__ void spread_fftx_body(int bid, const int4 *__restrict
   }
   __syncthreads();
   C2 *res = lds_fft<C2>(a, b, p, ncx, tw, false);
-  for (int e = tid; e < ncx * H; e += 256) {
+  for (int e = tid; e < ncx * H; e += kSpreadThreads) {
     const int r = e / H, k = e - r * H;
     const int pz = r / hb, py = 2 * (r - pz * hb);
     const int y = y0 + py, z = z0 + pz;
@@ -429,7 +434,7 @@ __device__ __forceinline__ void spread_fftx_body(int bid, const int4 *__restrict
   }
 }
 template <typename CT>
-__global__ __launch_bounds__(256) void k_spread_fftx(const int4 *__restrict__ posq,
+__global__ __launch_bounds__(kSpreadThreads) void k_spread_fftx(const int4 *__restrict__ posq,
                                                      const float4 *__restrict__ bb,
                                                      const int *__restrict__ meta, BrickGeom g,
                                                      double qunscale, FftPlan p,
@@ -769,7 +774,7 @@ static void recip_t(NbSystem &S, const float4 *x, int64_t *force, int stride, bo
   const double qunscale = 1.0;  // sorted coordinates carry the raw charge
   hipEvent_t t0 = nullptr;
   S.tic(1, s, t0);
-  k_spread_fftx<CT><<<P.nslabs, 256, P.shs, s>>>(S.posq.get(), S.cell_bb.get(), S.meta.get(),
+  k_spread_fftx<CT><<<P.nslabs, kSpreadThreads, P.shs, s>>>(S.posq.get(), S.cell_bb.get(), S.meta.get(),
                                                   P.bg, qunscale, plan_of(S.fx), twx, cgrid);
   APO_LAUNCH_CHECK();
   S.toc(1, s, t0);
@@ -850,7 +855,7 @@ struct GatherArgs {
   int stride;
 };
 
-__global__ __launch_bounds__(256) void kb_spread_fftx(const SpreadArgs *__restrict__ tab,
+__global__ __launch_bounds__(kSpreadThreads) void kb_spread_fftx(const SpreadArgs *__restrict__ tab,
                                                       const int *__restrict__ bstart, int nsys) {
   const int s = batch_system(bstart, nsys);
   const SpreadArgs &a = tab[s];
@@ -936,7 +941,7 @@ void pme_batch_upload(PmeBatch &b, hipStream_t s) {
 // launches write only the batch's grids (the autotuner times them so).
 void pme_batch_launch(PmeBatch &b, hipStream_t s, bool gather) {
   if (!b.spread.nsys()) return;
-  kb_spread_fftx<<<b.spread.blocks(), 256, b.spread.lds, s>>>(
+  kb_spread_fftx<<<b.spread.blocks(), kSpreadThreads, b.spread.lds, s>>>(
       b.spread.d_args.get(), b.spread.d_start.get(), b.spread.nsys());
   kb_ffty<<<b.fy.blocks(), b.ft, b.fy.lds, s>>>(b.fy.d_args.get(), b.fy.d_start.get(), b.fy.nsys(),
                                                0);
