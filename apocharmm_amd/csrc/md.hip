@@ -652,6 +652,7 @@ struct LtSystem {
   ~LtSystem() {
     for (hipGraphExec_t g : gexec)
       if (g) (void)hipGraphExecDestroy(g);
+    if (gcycle) (void)hipGraphExecDestroy(gcycle);
     for (hipEvent_t e : step_ev)
       if (e) (void)hipEventDestroy(e);
     if (h_disp) (void)hipHostFree(h_disp);
@@ -772,14 +773,68 @@ struct LtSystem {
       graph_gen[which] = nb->generation;
     }
     APO_HIP(hipGraphLaunch(ex, s));
-    if (which) nb->builds_queued++;
+    if (which == 1) nb->builds_queued++;
+  }
+
+  // A whole list cycle (the rebuild step and the nbfreq - 1 steps after it,
+  // with their prunes) as one graph: one launch per nbfreq steps.  It needs
+  // the fixed cadence and no centre-of-mass removal inside the cycle.
+  hipGraphExec_t gcycle = nullptr;
+  unsigned long long gcycle_gen = ~0ull;
+  bool cycle_ok() const {
+    if (!graph_usable() || list_policy != 0 || nbfreq < 2) return false;
+    if (comfreq <= 0) return true;
+    const unsigned long long cf = (unsigned long long)comfreq;
+    return (step / cf + 1) * cf >= step + (unsigned long long)nbfreq;
+  }
+  void run_cycle(hipStream_t s) {
+    if (comfreq > 0 && step % (unsigned long long)comfreq == 0) remove_com(s);
+    list_export_check(*nb, s, true);
+    if (!gcycle || gcycle_gen != nb->generation) {
+      if (gcycle) {
+        APO_HIP(hipGraphExecDestroy(gcycle));
+        gcycle = nullptr;
+      }
+      hipGraph_t g = nullptr;
+      const unsigned long long bq = nb->builds_queued;
+      APO_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      try {
+        rebuild_body(s);
+        for (int i = 1; i < nbfreq; i++) step_body(s, prune_every > 0 && i % prune_every == 0);
+      } catch (...) {
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        nb->builds_queued = bq;
+        throw;
+      }
+      nb->builds_queued = bq;
+      APO_HIP(hipStreamEndCapture(s, &g));
+      hipError_t e = hipGraphInstantiate(&gcycle, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      APO_HIP(e);
+      gcycle_gen = nb->generation;
+    }
+    APO_HIP(hipGraphLaunch(gcycle, s));
+    nb->builds_queued++;
+    if (prune_every > 0) nb->pruned_build = nb->builds_queued;
+    steps_since_rebuild += nbfreq;
+    step += (unsigned long long)nbfreq;
   }
 
   // CudaIntegrator::propagate (src/CudaIntegrator.cu:140-230) with the
-  // thermostat's propagateOneStep (src/CudaLangevinThermostatIntegrator.cu:703-800).
+  // thermostat's propagateOneStep (src/CudaLangevinThermostatIntegrator.cu:703-800):
+  // whole list cycles as one graph where the schedule allows, else step by step.
   void propagate(int nsteps, hipStream_t s) {
     begin(s);
-    for (int k = 0; k < nsteps; k++) one_step(s);
+    for (int k = 0; k < nsteps;) {
+      if (nsteps - k >= nbfreq && steps_since_rebuild % nbfreq == 0 && cycle_ok()) {
+        run_cycle(s);
+        k += nbfreq;
+      } else {
+        one_step(s);
+        k++;
+      }
+    }
   }
 
   void begin(hipStream_t s) {
