@@ -220,13 +220,6 @@ struct SortArgs {
   int64_t *force;  // gradient to zero (nforce entries), or null
   size_t nforce;
 };
-struct FinalizeArgs {
-  const int64_t *fsort;
-  int ss;
-  const int *loc2glo, *meta;
-  int64_t *force;
-  int stride;
-};
 void launch_fill_f64(double *p, size_t n, hipStream_t s);
 void launch_fill_i32(int *p, size_t n, hipStream_t s);
 void launch_scan(const int *in, int *out, const int *n_dev, int n_max,
@@ -250,6 +243,9 @@ void nblist_prune(NbSystem &sys, hipStream_t s);
 // kCalcPrune, refreshes after the coordinate sort).
 constexpr int kCalcUsePruned = 1 << 27;
 constexpr int kCalcPrune = 1 << 28;
+// ... and leave the sorted pair gradient in fsort: the caller adds it (the
+// integrator's step kernel does, as it reads each atom's gradient).
+constexpr int kCalcNoFinalize = 1 << 26;
 
 void direct_launch(NbSystem &sys, bool energy, bool virial, hipStream_t s, bool pruned = false);
 void direct_finalize(NbSystem &sys, int64_t *d_force, int stride,

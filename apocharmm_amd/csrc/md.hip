@@ -36,8 +36,6 @@ void batch_sort(BatchLaunch<SortArgs> &b, hipStream_t s);
 void batch_add_pair(BatchLaunch<PairArgs> &b, NbSystem &S, int blocks);
 void batch_pair(BatchLaunch<PairArgs> &b, bool tab_lds, hipStream_t s);
 bool pair_tab_lds(const NbSystem &S);
-void batch_add_finalize(BatchLaunch<FinalizeArgs> &b, NbSystem &S, int64_t *force, int stride);
-void batch_finalize(BatchLaunch<FinalizeArgs> &b, hipStream_t s);
 struct PmeBatch;
 PmeBatch *pme_batch_create();
 void pme_batch_destroy(PmeBatch *b);
@@ -168,8 +166,14 @@ struct LtState {
   const double *invmass;
   const float *charge;
   float4 *xyzq;
-  const int64_t *grad;       // SoA, stride gstride, 2^-40
+  int64_t *grad;             // SoA, stride gstride, 2^-40
   int gstride;
+  // when set, the pair part of the gradient is still in the force system's
+  // sorted buffer (stride ss, slot glo2loc[i]): the step kernel adds it and
+  // stores the total back into grad
+  const int64_t *fsort;
+  const int *glo2loc;
+  int ss;
   int n;
   const int *waters;         // 3 * nwater
   int nwater;
@@ -198,8 +202,17 @@ __device__ inline void st3(double *a, int n, int i, double3 v) {
 }
 __device__ inline double3 ldgrad(const LtState &S, int i) {
   const double s = 9.094947017729282e-13;  // 2^-40
-  return make_double3((double)S.grad[i] * s, (double)S.grad[S.gstride + i] * s,
-                      (double)S.grad[2 * (size_t)S.gstride + i] * s);
+  int64_t gx = S.grad[i], gy = S.grad[S.gstride + i], gz = S.grad[2 * (size_t)S.gstride + i];
+  if (S.fsort) {  // sorted -> global order (the calc_force finalize, fused here)
+    const int l = S.glo2loc[i];
+    gx += S.fsort[l];
+    gy += S.fsort[S.ss + l];
+    gz += S.fsort[2 * (size_t)S.ss + l];
+    S.grad[i] = gx;
+    S.grad[S.gstride + i] = gy;
+    S.grad[2 * (size_t)S.gstride + i] = gz;
+  }
+  return make_double3((double)gx * s, (double)gy * s, (double)gz * s);
 }
 __device__ inline void store_xyzq(const LtState &S, int i, double3 x) {
   S.xyzq[i] = make_float4((float)x.x, (float)x.y, (float)x.z, S.charge[i]);
@@ -661,7 +674,9 @@ struct LtSystem {
     if (st) (void)hipStreamDestroy(st);
   }
 
-  LtState state() {
+  // fused: the last force call left the pair gradient in the sorted buffer
+  // (kCalcNoFinalize) and the kernel adds it (the step kernels only)
+  LtState state(bool fused = false) {
     LtState s;
     s.x = x.get();
     s.dp = dp.get();
@@ -677,11 +692,15 @@ struct LtSystem {
     s.singles = singles.get();
     s.nsingle = nsingle;
     s.xref = nb->xref.get();
+    s.fsort = fused ? nb->fsort.get() : nullptr;
+    s.glo2loc = nb->glo2loc.get();
+    s.ss = nb->ssort;
     return s;
   }
 
-  void force(hipStream_t s, bool reset, bool prune = false) {
-    int flags = reset ? APO_RESET_LIST : 0;
+  // fused: the step kernel that follows adds the sorted pair gradient
+  void force(hipStream_t s, bool reset, bool prune = false, bool fused = false) {
+    int flags = (reset ? APO_RESET_LIST : 0) | (fused ? kCalcNoFinalize : 0);
     if (prune_every > 0) flags |= kCalcUsePruned | (prune ? kCalcPrune : 0);
     calc_force(*nb, reinterpret_cast<const float *>(xyzq.get()), flags, grad.get(), gstride,
                nullptr, nullptr, s);
@@ -710,7 +729,7 @@ struct LtSystem {
   void update(hipStream_t s) {
     hipEvent_t t0 = nullptr;
     nb->tic(8, s, t0);
-    k_lt_step<<<nunit_blocks, kStepThreads, 0, s>>>(state(), C, counters.get(), ke_part.get(),
+    k_lt_step<<<nunit_blocks, kStepThreads, 0, s>>>(state(true), C, counters.get(), ke_part.get(),
                                                   dpart.get());
     k_lt_finalize<<<1, 256, 0, s>>>(ke_part.get(), nunit_blocks, ndof, thermo.get(),
                                     counters.get(), dpart.get(), nb->meta.get(), h_disp);
@@ -720,13 +739,13 @@ struct LtSystem {
 
   // force + integrator update + kinetic energy (after a list prune)
   void step_body(hipStream_t s, bool prune = false) {
-    force(s, false, prune);
+    force(s, false, prune, true);
     update(s);
   }
   // the same after image centering and a list rebuild
   void rebuild_body(hipStream_t s) {
     center(s);
-    force(s, true);
+    force(s, true, false, true);
     update(s);
   }
 
@@ -1126,7 +1145,6 @@ struct LtBatch {
   // fused step: one launch per kernel type for all systems
   BatchLaunch<SortArgs> sort;
   BatchLaunch<PairArgs> pair;
-  BatchLaunch<FinalizeArgs> fin;
   BatchLaunch<LtStepArgs> lstep;
   BatchLaunch<LtFinArgs> lfin;
   PmeBatch *pme = nullptr;
@@ -1165,7 +1183,6 @@ struct LtBatch {
   void build_tables() {
     sort.clear();
     pair.clear();
-    fin.clear();
     lstep.clear();
     lfin.clear();
     pme_batch_clear(*pme);
@@ -1184,8 +1201,8 @@ struct LtBatch {
       tab_lds = tab_lds && pair_tab_lds(S);
       pme_batch_add(*pme, S, g, L->gstride, fft_rows, fft_cols, fft_threads);
       bonded_batch_add(*bond, S, x, g, L->gstride);
-      batch_add_finalize(fin, S, g, L->gstride);
-      lstep.add(LtStepArgs{L->state(), L->C, L->counters.get(), L->ke_part.get(), L->dpart.get()},
+      lstep.add(LtStepArgs{L->state(true), L->C, L->counters.get(), L->ke_part.get(),
+                           L->dpart.get()},
                 L->nunit_blocks);
       lfin.add(LtFinArgs{L->ke_part.get(), L->nunit_blocks, L->ndof, L->thermo.get(),
                          L->counters.get(), L->dpart.get(), S.meta.get(), L->h_disp},
@@ -1193,7 +1210,6 @@ struct LtBatch {
     }
     sort.upload(m);
     pair.upload(m);
-    fin.upload(m);
     lstep.upload(m);
     lfin.upload(m);
     pme_batch_upload(*pme, m);
@@ -1203,7 +1219,7 @@ struct LtBatch {
     for (size_t i = 0; i < sys.size(); i++) tab_gens[i] = sys[i]->nb->generation;
   }
 
-  // force path of every system + integrator update: 13 launches in all.
+  // force path of every system + integrator update: 12 launches in all.
   // After the sort, the PME chain and the bonded terms fork onto side
   // streams and the pair kernel runs on s; all three write int64 sums, so
   // their overlap changes no bit.
@@ -1214,8 +1230,9 @@ struct LtBatch {
     kb_lt_finalize<<<lfin.nsys(), 256, 0, s>>>(lfin.d_args.get());
     APO_LAUNCH_CHECK();
   }
-  // the force part: every system's gradient from its current coordinates
-  // (rewrites per-step scratch and the gradient only)
+  // the force part: every system's gradient from its current coordinates,
+  // with the pair part left in the sorted buffer for kb_lt_step to add
+  // (rewrites per-step scratch and the gradient buffers only)
   void force_step(hipStream_t s) {
     batch_sort(sort, s);  // also zeroes every system's gradient
     APO_HIP(hipEventRecord(sort_ev, s));
@@ -1228,16 +1245,16 @@ struct LtBatch {
     batch_pair(pair, tab_lds, s);
     APO_HIP(hipStreamWaitEvent(s, side_ev[0], 0));
     APO_HIP(hipStreamWaitEvent(s, side_ev[1], 0));
-    batch_finalize(fin, s);
+    // the sorted pair gradient is added by the step kernel (kb_lt_step)
   }
 
   // Autotuner of the launch shapes.  It times the fused spread + FFT
   // launches for seven FFT block shapes, then the whole force step (sort,
-  // the PME, bonded and pair branches, finalize) for eight pair-kernel
-  // block budgets, on the systems' current coordinates and lists, and keeps
-  // the fastest.  Those launches rewrite per-step scratch and the gradient,
-  // which each step recomputes before using it, and each row, column and item
-  // is computed the same way in any blocking, so trajectories do not change.
+  // the PME, bonded and pair branches) for eight pair-kernel block budgets,
+  // on the systems' current coordinates and lists, and keeps the fastest.
+  // Those launches rewrite per-step scratch and the gradient buffers, which
+  // each step recomputes before using them, and each row, column and item is
+  // computed the same way in any blocking, so trajectories do not change.
   // report[9]:
   // chosen blocks per CU, FFT threads, rows and columns, the force-step and
   // PME-chain times (ms) of the choice and of the default shapes, candidates

@@ -346,12 +346,6 @@ __global__ void k_direct_finalize(const int64_t *__restrict__ fsort, int ss,
                                   int64_t *__restrict__ force, int stride) {
   finalize_body(blockIdx.x, fsort, ss, loc2glo, meta, force, stride);
 }
-__global__ __launch_bounds__(256) void kb_direct_finalize(const FinalizeArgs *__restrict__ tab,
-                                                          const int *__restrict__ bstart, int nsys) {
-  const int s = batch_system(bstart, nsys);
-  const FinalizeArgs a = tab[s];
-  finalize_body(blockIdx.x - bstart[s], a.fsort, a.ss, a.loc2glo, a.meta, a.force, a.stride);
-}
 
 void direct_finalize(NbSystem &S, int64_t *force, int stride, hipStream_t s) {
   k_direct_finalize<<<(S.ssort + 255) / 256, 256, 0, s>>>(S.fsort.get(), S.ssort, S.loc2glo.get(),
@@ -376,14 +370,5 @@ void batch_pair(BatchLaunch<PairArgs> &b, bool tab_lds, hipStream_t s) {
   APO_LAUNCH_CHECK();
 }
 bool pair_tab_lds(const NbSystem &S) { return S.ntypes <= kMaxTypesLds; }
-void batch_add_finalize(BatchLaunch<FinalizeArgs> &b, NbSystem &S, int64_t *force, int stride) {
-  FinalizeArgs a{S.fsort.get(), S.ssort, S.loc2glo.get(), S.meta.get(), force, stride};
-  b.add(a, (S.ssort + 255) / 256);
-}
-void batch_finalize(BatchLaunch<FinalizeArgs> &b, hipStream_t s) {
-  if (!b.blocks()) return;
-  kb_direct_finalize<<<b.blocks(), 256, 0, s>>>(b.d_args.get(), b.d_start.get(), b.nsys());
-  APO_LAUNCH_CHECK();
-}
 
 }  // namespace apo

@@ -510,7 +510,7 @@ void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int s
   }
   if (fork && rec) APO_HIP(hipStreamWaitEvent(s, S.ev_pme, 0));
   if (fork && bnd) APO_HIP(hipStreamWaitEvent(s, S.ev_bond, 0));
-  if (dir || rec) {
+  if ((dir || rec) && !(flags & kCalcNoFinalize)) {
     S.tic(7, s, t0);
     direct_finalize(S, force, stride, s);
     S.toc(7, s, t0);

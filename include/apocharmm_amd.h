@@ -343,9 +343,10 @@ APO_API apo_status apo_lt_batch_destroy(apo_lt_batch *b);
  * report9 = blocks per CU, FFT threads, FFT rows, FFT columns, the chosen
  * force-step and PME-chain times (ms), the default shapes' force-step and
  * PME-chain times (ms), launches timed.  Trajectories are unchanged: the timed
- * launches rewrite per-step scratch, and the gradient, which every step
- * recomputes before it uses it (until the next step, apo_lt_get_state's grad
- * then holds the gradient at the current coordinates).  Synchronises. */
+ * launches rewrite per-step scratch and the gradient buffers, which every step
+ * recomputes before it uses them (until the next step, apo_lt_get_state's
+ * grad holds those partial sums instead of the last step's gradient).
+ * Synchronises. */
 APO_API apo_status apo_lt_batch_autotune(apo_lt_batch *b, void *stream, double *report9);
 
 /* Copies the state to the host (synchronises): any output may be NULL.
