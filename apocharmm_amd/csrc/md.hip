@@ -6,16 +6,18 @@
 // State lives in HBM in FP64 SoA (x, previous half-step displacement, on-step
 // velocity), plus the float4 xyzq the force path reads and the int64 2^-40
 // gradient it writes.  One step is:
-//   [rebuild steps: image centering + list rebuild, eager]
-//   calc_force(xyzq) -> int64 gradient
-//   k_lt_step: one thread per water (3 atoms, SETTLE in registers) or per
-//              free atom: leapfrog Langevin update with Philox noise, SETTLE
-//              against the start-of-step coordinates, velocity, kinetic
-//              energy partials, the float xyzq for the next force call
+//   [rebuild steps: image centering + list rebuild]
+//   calc_force(xyzq) -> int64 gradient (the pair part left in sorted order)
+//   k_lt_step: three waves per 64 waters (one per atom; wave 0 alone for
+//              free atoms): the sorted pair gradient added to the rest,
+//              leapfrog Langevin update with Philox noise, SETTLE against the
+//              start-of-step coordinates (one lane per water, from LDS),
+//              velocity, kinetic energy partials, the float xyzq for the next
+//              force call
 //   k_lt_finalize: fixed-order kinetic energy sum, temperatures, step + 1
 // so the integrator costs one pass over the state instead of the reference's
-// five kernels and two device copies.  Steps without a rebuild or centre of
-// mass removal replay a hipGraph of that sequence.
+// five kernels and two device copies.  Steps replay as hipGraphs: a whole
+// list cycle per graph where the schedule allows, else one step per graph.
 #include <algorithm>
 #include <cmath>
 #include <vector>
