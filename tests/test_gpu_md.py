@@ -28,12 +28,13 @@ def _centered_c1():
 
 
 def _make(sp, xyz, vel, dt=DT_PS, friction=FRICTION, temp=TEMP, seed=SEED, graph=True,
-          precision="mixed", nbfreq=20):
+          precision="mixed", nbfreq=20, prune=0):
     from apocharmm_amd.force_manager import ForceManager
     from apocharmm_amd.integrator import LangevinThermostatIntegrator
     fm = ForceManager(sp, precision).initialize()
     lt = LangevinThermostatIntegrator(dt, friction=friction, temperature=temp, seed=seed)
     lt.setNonbondedListUpdateFrequency(nbfreq)
+    lt.setListPruneFrequency(prune)
     lt.setUseGraph(graph)
     lt.setCharmmContext(fm, xyz, vel)
     return fm, lt
@@ -101,6 +102,21 @@ def test_md_graph_replay_bitwise_equals_eager(c1):
     for k in ("xyz", "vel", "dprev", "grad"):
         assert np.array_equal(out[0][k], out[1][k]), k
     assert np.array_equal(out[0]["thermo"], out[1]["thermo"])
+
+
+def test_md_pruned_cycle_graphs_bitwise_equal_eager(c1):
+    """With list pruning every 5 steps, whole-cycle graphs (two cycles of 20
+    steps, prunes inside them) and step-by-step eager calls give the same bits."""
+    sp, xyz, vel = c1
+    out = []
+    for graph in (True, False):
+        fm, lt = _make(sp, xyz, vel, graph=graph, prune=5)
+        lt.propagate(47)
+        out.append(lt.getState())
+        lt.close()
+        fm.close()
+    for k in ("xyz", "vel", "dprev", "grad"):
+        assert np.array_equal(out[0][k], out[1][k]), k
 
 
 def test_md_seed_reproducible_and_settle_geometry(c1):
