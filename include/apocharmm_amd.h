@@ -217,7 +217,8 @@ APO_API apo_status apo_nb_list_stats(apo_nb_system *sys, int64_t *out6,
  * src/ForceManager.cu:606-616).  enable != 0 starts recording and clears the
  * accumulators.  apo_nb_profile_read synchronises the recorded events and
  * returns, per slot, the summed milliseconds and the number of launches:
- * slot 0 pair kernel, 1 PME spread, 2 PME FFT + convolution (5 passes),
+ * slot 0 pair kernel, 1 PME spread with the forward x FFT, 2 PME FFT +
+ * convolution (the other 4 passes),
  * 3 PME gather, 4 neighbor-list rebuild, 5 whole calc_force, 6 bonded +
  * 1-4/exclusions, 7 sort/finalize of the direct-space coordinates/forces,
  * 8 integrator update (apo_lt_propagate: leapfrog + SETTLE + kinetic energy),
