@@ -239,3 +239,20 @@ def test_config2_full_size_against_oracle(oracle):
     # to zero; the PME gradient sum is the small discretisation residue
     g = got["grad_fixed"].astype(np.float64) / 2.0 ** 40
     assert np.abs(g.sum(0)).max() < 1e-2 * np.sqrt(len(g))
+
+
+def test_calc_force_rejects_unknown_flag_bits():
+    """apo_nb_calc_force accepts the APO_CALC_*, APO_RESET_LIST and APO_TERM_*
+    bits only (the others are reserved for the library's own step path)."""
+    from apocharmm_amd.force_manager import ForceManager
+    sp = systems.config1()
+    fm = ForceManager(sp).initialize()
+    x = torch.tensor(systems.xyzq_of(sp), device="cuda")
+    fm.calcForce(x, reset=True)
+    for bit in (3, 7, 16, 26, 27, 28, 30):
+        with pytest.raises(_lib.ApoInvalidArgument):
+            fm.calcForce(x, terms=1 << bit)
+    fm.calcForce(x)  # the system stays usable
+    torch.cuda.synchronize()
+    assert np.all(np.isfinite(fm.getForces().cpu().numpy()))
+    fm.close()
