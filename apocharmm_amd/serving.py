@@ -12,6 +12,11 @@ runs alone or in any mix, so its result is bitwise the lone one (the fused
 kernels run each system's arithmetic unchanged).  Step counts are rounded up
 to whole chunks.
 
+StreamServer serves the same job stream without the fused batch: each job is
+its own single-system integrator on its own stream, and jobs overlap on the
+GPU through their streams.  Server (the fused batch re-formed at every
+membership change) is disabled, see Server.run.
+
 No counterpart in the reference (one CharmmContext per process); this is the
 serving loop for many independent jobs on one GPU.
 """
@@ -44,6 +49,7 @@ class Job:
         self.done = 0
         self.t_admit = None
         self.t_done = None
+        self.stream = None  # StreamServer: the job's HIP stream while it runs
 
     @property
     def natom(self):
@@ -94,6 +100,13 @@ def make_jobs(n, rate, atoms=(1500, 100000), steps=(1000, 50000), step_scale=1.0
     return out
 
 
+def _note(log, t, what, j, nactive):
+    """One line per admission / completion to `log` (a file) if given."""
+    if log is not None:
+        print("[serve %8.3f s] %s job %d (%d atoms, %d steps), %d active" %
+              (t, what, j.id, j.natom, j.nsteps, nactive), file=log, flush=True)
+
+
 class Server:
     def __init__(self, max_slots=64, hbm_budget=None, chunk=100, lanes=4, keep_state=False,
                  log=None):
@@ -105,9 +118,7 @@ class Server:
         self.log = log  # file for one line per admission / completion
 
     def _note(self, t, what, j, nactive):
-        if self.log is not None:
-            print("[serve %8.3f s] %s job %d (%d atoms, %d steps), %d active" %
-                  (t, what, j.id, j.natom, j.nsteps, nactive), file=self.log, flush=True)
+        _note(self.log, t, what, j, nactive)
 
     def run(self, jobs):
         """Serves `jobs` (prepared, sorted by arrival) in real time; returns
@@ -167,6 +178,65 @@ class Server:
                 active = [j for j in active if j.done < j.nsteps]
         if batch is not None:
             batch.close()
+        return jobs
+
+
+class StreamServer:
+    """Serving without the fused batch: every admitted job is its own
+    single-system integrator on its own HIP stream.  Each round queues `chunk`
+    steps of every active job (graph launches on the job's stream, no host
+    synchronisation between jobs, so the jobs overlap on the GPU), then waits
+    for the round; jobs are admitted (by arrival, slots and HBM budget) and
+    retired between rounds.  A job's trajectory only ever sees
+    propagate(chunk) calls on the single-system path, so it is bitwise its lone
+    result."""
+
+    def __init__(self, max_slots=16, hbm_budget=None, chunk=100, keep_state=False, log=None):
+        self.max_slots = max_slots
+        self.hbm_budget = hbm_budget
+        self.chunk = chunk
+        self.keep_state = keep_state
+        self.log = log
+
+    def run(self, jobs):
+        """Serves `jobs` (prepared) in real time; returns them with t_admit /
+        t_done set (seconds since the first arrival window opened)."""
+        import torch
+        pending = sorted(jobs, key=lambda j: j.arrival)
+        if self.hbm_budget is None:
+            free, _ = torch.cuda.mem_get_info()
+            self.hbm_budget = int(0.8 * free)
+        active, used = [], 0
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0
+        note = lambda t, what, j, n: _note(self.log, t, what, j, n)
+        while pending or active:
+            while (pending and pending[0].arrival <= now() and len(active) < self.max_slots
+                   and used + pending[0].hbm_bytes() <= self.hbm_budget):
+                j = pending.pop(0)
+                j.start()
+                j.stream = torch.cuda.Stream()
+                j.t_admit = now()
+                used += j.hbm_bytes()
+                active.append(j)
+                note(j.t_admit, "admit", j, len(active))
+            if not active:
+                time.sleep(max(0.0, min(0.01, pending[0].arrival - now())))
+                continue
+            for j in active:
+                with torch.cuda.stream(j.stream):
+                    j.lt.propagate(self.chunk)
+            for j in active:
+                j.stream.synchronize()
+                j.done += self.chunk
+            finished = [j for j in active if j.done >= j.nsteps]
+            for j in finished:
+                j.t_done = now()
+                j.finish(self.keep_state)
+                j.stream = None
+                used -= j.hbm_bytes()
+                note(j.t_done, "done", j, len(active) - len(finished))
+            active = [j for j in active if j.done < j.nsteps]
         return jobs
 
 
