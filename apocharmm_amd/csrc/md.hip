@@ -670,6 +670,8 @@ struct LtSystem {
     if (gcycle) (void)hipGraphExecDestroy(gcycle);
     for (hipEvent_t e : step_ev)
       if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : prio_ev)
+      if (e) (void)hipEventDestroy(e);
     if (h_disp) (void)hipHostFree(h_disp);
     if (ev_in) (void)hipEventDestroy(ev_in);
     if (ev_out) (void)hipEventDestroy(ev_out);
@@ -842,6 +844,37 @@ struct LtSystem {
     step += (unsigned long long)nbfreq;
   }
 
+  // PME issue-priority trial (NbSystem::pme_prio): the first whole list
+  // cycles run with the raised priority off, then on, each setting for one
+  // untimed cycle and kPrioTimed timed ones, and the faster setting stays.
+  // Priority orders instruction issue only, so these are ordinary cycles of
+  // the trajectory.
+  static constexpr int kPrioTimed = 2;
+  static constexpr int kPrioCycles = 2 * (1 + kPrioTimed);
+  int prio_trial = 0;  // trial cycles run so far
+  float prio_ms[2] = {0.f, 0.f};
+  hipEvent_t prio_ev[2] = {nullptr, nullptr};
+  void set_pme_prio(int p) {
+    if (nb->pme_prio == p) return;
+    nb->pme_prio = p;
+    nb->generation++;  // captured graphs hold it as a kernel argument
+  }
+  void trial_cycle(hipStream_t s) {
+    const int want = prio_trial / (1 + kPrioTimed);
+    const bool timed = prio_trial % (1 + kPrioTimed) != 0;
+    set_pme_prio(want);
+    if (timed) APO_HIP(hipEventRecord(prio_ev[0], s));
+    run_cycle(s);
+    if (timed) {
+      APO_HIP(hipEventRecord(prio_ev[1], s));
+      APO_HIP(hipEventSynchronize(prio_ev[1]));
+      float ms = 0.f;
+      APO_HIP(hipEventElapsedTime(&ms, prio_ev[0], prio_ev[1]));
+      prio_ms[want] += ms;
+    }
+    if (++prio_trial == kPrioCycles) set_pme_prio(prio_ms[1] < prio_ms[0] ? 1 : 0);
+  }
+
   // CudaIntegrator::propagate (src/CudaIntegrator.cu:140-230) with the
   // thermostat's propagateOneStep (src/CudaLangevinThermostatIntegrator.cu:703-800):
   // whole list cycles as one graph where the schedule allows, else step by step.
@@ -849,7 +882,8 @@ struct LtSystem {
     begin(s);
     for (int k = 0; k < nsteps;) {
       if (nsteps - k >= nbfreq && steps_since_rebuild % nbfreq == 0 && cycle_ok()) {
-        run_cycle(s);
+        if (prio_trial < kPrioCycles) trial_cycle(s);
+        else run_cycle(s);
         k += nbfreq;
       } else {
         one_step(s);
@@ -1093,6 +1127,7 @@ LtSystem *create_lt(NbSystem *nb, const apo_lt_params &p, const double *xyz, con
     }
     for (hipEvent_t &e : L->step_ev)
       APO_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (hipEvent_t &e : L->prio_ev) APO_HIP(hipEventCreate(&e));
     APO_HIP(hipStreamSynchronize(s));
   } catch (...) {
     delete L;

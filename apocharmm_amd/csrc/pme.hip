@@ -17,6 +17,7 @@
 //   * gather uses 16 lanes per atom and writes int64 2^-40 gradients.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "system.hpp"
 
@@ -433,13 +434,14 @@ __device__ __forceinline__ void spread_fftx_body(int bid, const int4 *__restrict
     if (z < g.K2 && y + 1 < g.K1) out[o + H] = B;
   }
 }
-template <typename CT>
+template <typename CT, bool PRIO>
 __global__ __launch_bounds__(kSpreadThreads) void k_spread_fftx(const int4 *__restrict__ posq,
                                                      const float4 *__restrict__ bb,
                                                      const int *__restrict__ meta, BrickGeom g,
                                                      double qunscale, FftPlan p,
                                                      const typename Cx<CT>::T *__restrict__ twg,
                                                      typename Cx<CT>::T *__restrict__ out) {
+  if constexpr (PRIO) pme_wave_priority();
   spread_fftx_body<CT>(blockIdx.x, posq, bb, meta, g, qunscale, p, twg, out);
 }
 
@@ -500,13 +502,14 @@ __device__ __forceinline__ void gather_body(int bid, const int4 *__restrict__ po
     }
   }
 }
-template <typename CT>
+template <typename CT, bool PRIO>
 __global__ __launch_bounds__(256) void k_gather_sorted(const int4 *__restrict__ posq,
                                                        const int *__restrict__ meta,
                                                        const int *__restrict__ loc2glo, GridGeom g,
                                                        double qunscale, const CT *__restrict__ phi,
                                                        unsigned long long *__restrict__ force,
                                                        int stride) {
+  if constexpr (PRIO) pme_wave_priority();
   gather_body<CT>(blockIdx.x, posq, meta, loc2glo, g, qunscale, phi, force, stride);
 }
 
@@ -556,11 +559,12 @@ __device__ __forceinline__ void fftx_inv_body(int bid, const typename Cx<CT>::T 
     if (rbw < nrows) phi[(size_t)rbw * n + x] = res[e].y;
   }
 }
-template <typename CT>
+template <typename CT, bool PRIO>
 __global__ __launch_bounds__(256) void k_fftx_inv(const typename Cx<CT>::T *__restrict__ in,
                                                   int nrows, FftPlan p, int rb,
                                                   const typename Cx<CT>::T *__restrict__ twg,
                                                   CT *__restrict__ phi) {
+  if constexpr (PRIO) pme_wave_priority();
   fftx_inv_body<CT>(blockIdx.x, in, nrows, p, rb, twg, phi);
 }
 
@@ -591,10 +595,11 @@ __device__ __forceinline__ void ffty_body(int bid, typename Cx<CT>::T *__restric
     if (c < wc) data[((size_t)z * K1 + y) * H + kx0 + c] = res[c * n + y];
   }
 }
-template <typename CT>
+template <typename CT, bool PRIO>
 __global__ __launch_bounds__(256) void k_ffty(typename Cx<CT>::T *__restrict__ data, int H, int K1,
                                               int W, FftPlan p,
                                               const typename Cx<CT>::T *__restrict__ twg, int inv) {
+  if constexpr (PRIO) pme_wave_priority();
   ffty_body<CT>(blockIdx.x, data, H, K1, W, p, twg, inv);
 }
 
@@ -698,12 +703,13 @@ __device__ __forceinline__ void fftz_conv_body(int bid, typename Cx<CT>::T *__re
     }
   }
 }
-template <typename CT, bool CALC_EV>
+template <typename CT, bool CALC_EV, bool PRIO>
 __global__ __launch_bounds__(256) void k_fftz_conv(typename Cx<CT>::T *__restrict__ data, ConvArgs c,
                                                    FftPlan p,
                                                    const typename Cx<CT>::T *__restrict__ twg,
                                                    const CT *__restrict__ prefac,
                                                    double *__restrict__ part) {
+  if constexpr (PRIO) pme_wave_priority();
   fftz_conv_body<CT, CALC_EV>(blockIdx.x, data, c, p, twg, prefac, part);
 }
 
@@ -763,7 +769,7 @@ static PmeShapes<CT> pme_shapes(const NbSystem &S, int rows = kFftRowsPerBlock,
   return P;
 }
 
-template <typename CT>
+template <typename CT, bool PRIO>
 static void recip_t(NbSystem &S, const float4 *x, int64_t *force, int stride, bool EV,
                     typename Cx<CT>::T *cgrid, CT *phi, const typename Cx<CT>::T *twx,
                     const typename Cx<CT>::T *twy, const typename Cx<CT>::T *twz, const CT *pre,
@@ -774,31 +780,31 @@ static void recip_t(NbSystem &S, const float4 *x, int64_t *force, int stride, bo
   const double qunscale = 1.0;  // sorted coordinates carry the raw charge
   hipEvent_t t0 = nullptr;
   S.tic(1, s, t0);
-  k_spread_fftx<CT><<<P.nslabs, kSpreadThreads, P.shs, s>>>(S.posq.get(), S.cell_bb.get(), S.meta.get(),
+  k_spread_fftx<CT, PRIO><<<P.nslabs, kSpreadThreads, P.shs, s>>>(S.posq.get(), S.cell_bb.get(), S.meta.get(),
                                                   P.bg, qunscale, plan_of(S.fx), twx, cgrid);
   APO_LAUNCH_CHECK();
   S.toc(1, s, t0);
   S.tic(2, s, t0);
-  k_ffty<CT><<<K2 * P.ncy, P.ft, P.shy, s>>>(cgrid, H, K1, P.wy, plan_of(S.fy), twy, 0);
+  k_ffty<CT, PRIO><<<K2 * P.ncy, P.ft, P.shy, s>>>(cgrid, H, K1, P.wy, plan_of(S.fy), twy, 0);
   APO_LAUNCH_CHECK();
   if (K1 * P.ncz > S.part.recip_n) throw RuntimeError("reciprocal partial buffer too small");
   double *part = S.parts.get() + S.part.recip_off;
   if (EV) {
     launch_fill_f64(part, (size_t)S.part.recip_n * kRecipPart, s);
-    k_fftz_conv<CT, true><<<K1 * P.ncz, P.ft, P.shz, s>>>(cgrid, P.c, plan_of(S.fz), twz, pre, part);
+    k_fftz_conv<CT, true, PRIO><<<K1 * P.ncz, P.ft, P.shz, s>>>(cgrid, P.c, plan_of(S.fz), twz, pre, part);
   } else {
-    k_fftz_conv<CT, false><<<K1 * P.ncz, P.ft, P.shz, s>>>(cgrid, P.c, plan_of(S.fz), twz, pre,
+    k_fftz_conv<CT, false, PRIO><<<K1 * P.ncz, P.ft, P.shz, s>>>(cgrid, P.c, plan_of(S.fz), twz, pre,
                                                           part);
   }
   APO_LAUNCH_CHECK();
-  k_ffty<CT><<<K2 * P.ncy, P.ft, P.shy, s>>>(cgrid, H, K1, P.wy, plan_of(S.fy), twy, 1);
+  k_ffty<CT, PRIO><<<K2 * P.ncy, P.ft, P.shy, s>>>(cgrid, H, K1, P.wy, plan_of(S.fy), twy, 1);
   APO_LAUNCH_CHECK();
-  k_fftx_inv<CT><<<(P.npair + P.rb - 1) / P.rb, P.ft, P.shx, s>>>(cgrid, P.nrows, plan_of(S.fx),
+  k_fftx_inv<CT, PRIO><<<(P.npair + P.rb - 1) / P.rb, P.ft, P.shx, s>>>(cgrid, P.nrows, plan_of(S.fx),
                                                                   P.rb, twx, phi);
   APO_LAUNCH_CHECK();
   S.toc(2, s, t0);
   S.tic(3, s, t0);
-  k_gather_sorted<CT><<<(S.ssort * 16 + 255) / 256, 256, 0, s>>>(
+  k_gather_sorted<CT, PRIO><<<(S.ssort * 16 + 255) / 256, 256, 0, s>>>(
       S.posq.get(), S.meta.get(), S.loc2glo.get(), P.g, qunscale, phi,
       reinterpret_cast<unsigned long long *>(force), stride);
   APO_LAUNCH_CHECK();
@@ -807,12 +813,19 @@ static void recip_t(NbSystem &S, const float4 *x, int64_t *force, int stride, bo
 
 void recip_launch(NbSystem &S, const float4 *x, int64_t *force, int stride, bool E, bool V,
                   hipStream_t s) {
-  if (S.precision == 1)
-    recip_t<double>(S, x, force, stride, E || V, S.cgrid_d.get(), S.phi_d.get(), S.fx.tw64.get(),
-                    S.fy.tw64.get(), S.fz.tw64.get(), S.prefac_d.get(), s);
-  else
-    recip_t<float>(S, x, force, stride, E || V, S.cgrid.get(), S.phi.get(), S.fx.tw.get(),
-                   S.fy.tw.get(), S.fz.tw.get(), S.prefac.get(), s);
+  // the issue priority is a template parameter: as a kernel argument, every
+  // wave waited for its load before issuing the kernel's own first loads
+  auto run = [&](auto prio) {
+    constexpr bool P = decltype(prio)::value;
+    if (S.precision == 1)
+      recip_t<double, P>(S, x, force, stride, E || V, S.cgrid_d.get(), S.phi_d.get(),
+                         S.fx.tw64.get(), S.fy.tw64.get(), S.fz.tw64.get(), S.prefac_d.get(), s);
+    else
+      recip_t<float, P>(S, x, force, stride, E || V, S.cgrid.get(), S.phi.get(), S.fx.tw.get(),
+                        S.fy.tw.get(), S.fz.tw.get(), S.prefac.get(), s);
+  };
+  if (S.pme_prio) run(std::true_type{});
+  else run(std::false_type{});
 }
 
 // ---- fused launches over several systems (mixed precision, no energy) ------

@@ -56,6 +56,10 @@ struct NbSystem {
   // bumped whenever a buffer or a launch constant of the per-step path
   // changes (list buffers regrown, new box); captured step graphs compare it
   unsigned long long generation = 0;
+  // 1: the PME kernels raise their wave issue priority (a launch constant;
+  // the integrator's first list cycles pick it by timing, results are the
+  // same either way)
+  int pme_prio = 0;
   std::vector<int> h_bonds;  // 2 * nbond, host copy for molecule grouping
   DevBuf<int> meta;   // [0] ncell [1] ncand [2] nent [3] nitem [4] maxcol
   DevBuf<int> col_count, col_start, col_ncell, col_cellstart;

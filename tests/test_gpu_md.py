@@ -119,6 +119,25 @@ def test_md_pruned_cycle_graphs_bitwise_equal_eager(c1):
         assert np.array_equal(out[0][k], out[1][k]), k
 
 
+def test_md_priority_trial_cycles_bitwise_equal_eager(c1):
+    """The first six whole-cycle graphs alternate the PME kernels' issue
+    priority (off, then on; timed) and the faster setting stays.  Across two
+    propagate calls (eight cycles: the trial, the switch and cycles after it)
+    the trajectory is bitwise the eager one, which never raises priority."""
+    sp, xyz, vel = c1
+    out = []
+    for graph in (True, False):
+        fm, lt = _make(sp, xyz, vel, graph=graph)
+        lt.propagate(70)
+        lt.propagate(105)
+        out.append(lt.getState())
+        lt.close()
+        fm.close()
+    for k in ("xyz", "vel", "dprev", "grad"):
+        assert np.array_equal(out[0][k], out[1][k]), k
+    assert np.array_equal(out[0]["thermo"], out[1]["thermo"])
+
+
 def test_md_seed_reproducible_and_settle_geometry(c1):
     sp, xyz, vel = c1
     states = []
