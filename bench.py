@@ -11,7 +11,11 @@ direct-space pair kernel + 1-4/exclusion corrections + PME spread, FFT,
 convolution, gather) and the integrator update, through the C ABI
 (apo_lt_propagate), with the neighbor list rebuilt and images centred every
 20 steps (the reference's default, src/CudaIntegrator.cu:29) and the other
-steps replayed as a hipGraph.  All state is resident in HBM.
+steps replayed as a hipGraph.  For systems of 50,000 atoms and more, the pair
+list is pruned every 10 steps between rebuilds (--prune-every; 0 runs the
+reference's list unchanged): entries whose pairs all lie beyond
+r_off + 2 A * 10/20 are dropped, so every pair inside r_off is still computed.
+All state is resident in HBM.
 ns/day = steps * dt[ps] / 1000 / (wall_s / 86400) (unittest-benchmarks.cpp:286-287).
 --step force times force evaluations alone.
 
@@ -34,6 +38,10 @@ FRICTION = 1.0        # 1/ps (unittest-langevinThermostat.cpp:49)
 TEMP = 300.0
 SEED = 2003
 REBUILD_EVERY = 20
+# pruning the pair list every 10 steps pays from here up (measured: 98,304
+# atoms 572.6 -> 587.2 ns/day; 3,000 atoms 1,770 -> 1,706; 23,558 atoms within
+# run-to-run spread)
+PRUNE_MIN_ATOMS = 50000
 FLOP_PER_PAIR = 61
 PEAK_FP32_TFLOPS = 157.3
 KBOLTZ = 1.987191e-3  # kcal/mol/K
@@ -395,8 +403,10 @@ def main():
     ap.add_argument("--precision", default="mixed")
     ap.add_argument("--step", choices=("md", "force"), default="md")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--prune-every", type=int, default=0,
-                    help="MD: prune the pair list every K steps between rebuilds (0 = off)")
+    ap.add_argument("--prune-every", type=int, default=None,
+                    help="MD: prune the pair list every K steps between rebuilds (0 = off, "
+                         "the reference's list unchanged; default 10 from %d atoms up, else 0)"
+                         % PRUNE_MIN_ATOMS)
     ap.add_argument("--nsys", type=int, default=64, help="systems in the c4 batch")
     ap.add_argument("--hw-queues", type=int, default=4,
                     help="GPU_MAX_HW_QUEUES for the c4 run (HIP's default is 4)")
@@ -461,6 +471,8 @@ def main():
         vel = maxwell_velocities(spec["masses"], TEMP, vseed)
         lt = LangevinThermostatIntegrator(DT_PS, friction=FRICTION, temperature=TEMP, seed=SEED)
         lt.setNonbondedListUpdateFrequency(REBUILD_EVERY)
+        if a.prune_every is None:
+            a.prune_every = 10 if n >= PRUNE_MIN_ATOMS else 0
         lt.setListPruneFrequency(a.prune_every)
         lt.setCharmmContext(fm, np.asarray(spec["xyz"], np.float64), vel)
         lt.initialize()
