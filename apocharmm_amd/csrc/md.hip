@@ -670,7 +670,7 @@ struct LtSystem {
     if (gcycle) (void)hipGraphExecDestroy(gcycle);
     for (hipEvent_t e : step_ev)
       if (e) (void)hipEventDestroy(e);
-    for (hipEvent_t e : prio_ev)
+    for (hipEvent_t e : trial_ev)
       if (e) (void)hipEventDestroy(e);
     if (h_disp) (void)hipHostFree(h_disp);
     if (ev_in) (void)hipEventDestroy(ev_in);
@@ -844,35 +844,41 @@ struct LtSystem {
     step += (unsigned long long)nbfreq;
   }
 
-  // PME issue-priority trial (NbSystem::pme_prio): the first whole list
-  // cycles run with the raised priority off, then on, each setting for one
-  // untimed cycle and kPrioTimed timed ones, and the faster setting stays.
-  // Priority orders instruction issue only, so these are ordinary cycles of
-  // the trajectory.
-  static constexpr int kPrioTimed = 2;
-  static constexpr int kPrioCycles = 2 * (1 + kPrioTimed);
-  int prio_trial = 0;  // trial cycles run so far
-  float prio_ms[2] = {0.f, 0.f};
-  hipEvent_t prio_ev[2] = {nullptr, nullptr};
-  void set_pme_prio(int p) {
-    if (nb->pme_prio == p) return;
-    nb->pme_prio = p;
-    nb->generation++;  // captured graphs hold it as a kernel argument
+  // Launch-settings trial (NbSystem::pme_prio, NbSystem::pair_first): the
+  // first whole list cycles run under each of the four settings (bit 0: PME
+  // issue priority, bit 1: pair kernel first), one untimed cycle and one
+  // timed cycle each, and the fastest setting stays.  The settings order
+  // instruction issue and dispatch only, so these are ordinary cycles of the
+  // trajectory.
+  static constexpr int kTrialSettings = 4;
+  static constexpr int kTrialCycles = 2 * kTrialSettings;
+  int trial = 0;  // trial cycles run so far
+  float trial_ms[kTrialSettings] = {};
+  hipEvent_t trial_ev[2] = {nullptr, nullptr};
+  void set_launch(int setting) {
+    const int prio = setting & 1, first = (setting >> 1) & 1;
+    if (nb->pme_prio == prio && nb->pair_first == first) return;
+    nb->pme_prio = prio;
+    nb->pair_first = first;
+    nb->generation++;  // captured graphs hold both as launch constants
   }
   void trial_cycle(hipStream_t s) {
-    const int want = prio_trial / (1 + kPrioTimed);
-    const bool timed = prio_trial % (1 + kPrioTimed) != 0;
-    set_pme_prio(want);
-    if (timed) APO_HIP(hipEventRecord(prio_ev[0], s));
+    const int setting = trial / 2;
+    const bool timed = trial % 2 != 0;
+    set_launch(setting);
+    if (timed) APO_HIP(hipEventRecord(trial_ev[0], s));
     run_cycle(s);
     if (timed) {
-      APO_HIP(hipEventRecord(prio_ev[1], s));
-      APO_HIP(hipEventSynchronize(prio_ev[1]));
-      float ms = 0.f;
-      APO_HIP(hipEventElapsedTime(&ms, prio_ev[0], prio_ev[1]));
-      prio_ms[want] += ms;
+      APO_HIP(hipEventRecord(trial_ev[1], s));
+      APO_HIP(hipEventSynchronize(trial_ev[1]));
+      APO_HIP(hipEventElapsedTime(&trial_ms[setting], trial_ev[0], trial_ev[1]));
     }
-    if (++prio_trial == kPrioCycles) set_pme_prio(prio_ms[1] < prio_ms[0] ? 1 : 0);
+    if (++trial == kTrialCycles) {
+      int best = 0;
+      for (int k = 1; k < kTrialSettings; k++)
+        if (trial_ms[k] < trial_ms[best]) best = k;
+      set_launch(best);
+    }
   }
 
   // CudaIntegrator::propagate (src/CudaIntegrator.cu:140-230) with the
@@ -882,7 +888,7 @@ struct LtSystem {
     begin(s);
     for (int k = 0; k < nsteps;) {
       if (nsteps - k >= nbfreq && steps_since_rebuild % nbfreq == 0 && cycle_ok()) {
-        if (prio_trial < kPrioCycles) trial_cycle(s);
+        if (trial < kTrialCycles) trial_cycle(s);
         else run_cycle(s);
         k += nbfreq;
       } else {
@@ -1127,7 +1133,7 @@ LtSystem *create_lt(NbSystem *nb, const apo_lt_params &p, const double *xyz, con
     }
     for (hipEvent_t &e : L->step_ev)
       APO_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    for (hipEvent_t &e : L->prio_ev) APO_HIP(hipEventCreate(&e));
+    for (hipEvent_t &e : L->trial_ev) APO_HIP(hipEventCreate(&e));
     APO_HIP(hipStreamSynchronize(s));
   } catch (...) {
     delete L;

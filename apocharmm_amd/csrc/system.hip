@@ -478,36 +478,41 @@ void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int s
     S.tic(7, s, t0);
     nblist_sort_coords(S, x4, s, force, 3 * (size_t)stride);
     S.toc(7, s, t0);
-    if (rec) {
-      if (fork) {
-        APO_HIP(hipEventRecord(S.ev_sorted, s));
-        APO_HIP(hipStreamWaitEvent(sp, S.ev_sorted, 0));
-      }
-      recip_launch(S, x4, force, stride, E, V, sp);
-      if (fork) APO_HIP(hipEventRecord(S.ev_pme, sp));
-    }
   }
-  // the bonded branch forks after the sort and the PME launch (measured:
-  // launched first, its blocks delay the sort and the PME chain)
-  if (bnd) {
-    if (fork) {
-      APO_HIP(hipEventRecord(S.ev_fork, s));
-      APO_HIP(hipStreamWaitEvent(sb, S.ev_fork, 0));
-    }
-    S.tic(6, sb, t1);
-    if (terms & APO_TERM_PAIRS14) pairs14_launch(S, x4, force, stride, E, V, sb);
-    if (terms & APO_TERM_BONDED) bonded_launch(S, x4, force, stride, E, V, sb);
-    S.toc(6, sb, t1);
-    if (fork) APO_HIP(hipEventRecord(S.ev_bond, sb));
+  // Both side branches fork right after the sort.  The capture order of the
+  // three branches is a launch setting (pair_first): a graph keeps the first
+  // captured child of the sort on the caller's queue and dispatches it first.
+  if (fork && rec) {
+    APO_HIP(hipEventRecord(S.ev_sorted, s));
+    APO_HIP(hipStreamWaitEvent(sp, S.ev_sorted, 0));
   }
-  if (dir) {
+  if (fork && bnd) {
+    APO_HIP(hipEventRecord(S.ev_fork, s));
+    APO_HIP(hipStreamWaitEvent(sb, S.ev_fork, 0));
+  }
+  auto pair_branch = [&] {
     // a pruned list is refreshed after every build and when asked
     const bool pruned = flags & kCalcUsePruned;
     if (pruned && (rebuilt || (flags & kCalcPrune))) nblist_prune(S, s);
     S.tic(0, s, t0);
     direct_launch(S, E, V, s, pruned);
     S.toc(0, s, t0);
+  };
+  if (dir && S.pair_first) pair_branch();
+  if (rec) {
+    recip_launch(S, x4, force, stride, E, V, sp);
+    if (fork) APO_HIP(hipEventRecord(S.ev_pme, sp));
   }
+  // bonded after PME (measured: launched first, its blocks delay the sort
+  // and the PME chain)
+  if (bnd) {
+    S.tic(6, sb, t1);
+    if (terms & APO_TERM_PAIRS14) pairs14_launch(S, x4, force, stride, E, V, sb);
+    if (terms & APO_TERM_BONDED) bonded_launch(S, x4, force, stride, E, V, sb);
+    S.toc(6, sb, t1);
+    if (fork) APO_HIP(hipEventRecord(S.ev_bond, sb));
+  }
+  if (dir && !S.pair_first) pair_branch();
   if (fork && rec) APO_HIP(hipStreamWaitEvent(s, S.ev_pme, 0));
   if (fork && bnd) APO_HIP(hipStreamWaitEvent(s, S.ev_bond, 0));
   if ((dir || rec) && !(flags & kCalcNoFinalize)) {

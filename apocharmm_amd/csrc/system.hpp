@@ -56,10 +56,13 @@ struct NbSystem {
   // bumped whenever a buffer or a launch constant of the per-step path
   // changes (list buffers regrown, new box); captured step graphs compare it
   unsigned long long generation = 0;
-  // 1: the PME kernels raise their wave issue priority (a launch constant;
-  // the integrator's first list cycles pick it by timing, results are the
-  // same either way)
+  // Launch settings that change no result bits (launch constants of captured
+  // graphs; the integrator's first list cycles pick them by timing):
+  // pme_prio 1 = the PME kernels raise their wave issue priority;
+  // pair_first 1 = the pair kernel is launched (captured) before PME and
+  // bonded, so a graph dispatches it first from the caller's queue.
   int pme_prio = 0;
+  int pair_first = 0;
   std::vector<int> h_bonds;  // 2 * nbond, host copy for molecule grouping
   DevBuf<int> meta;   // [0] ncell [1] ncand [2] nent [3] nitem [4] maxcol
   DevBuf<int> col_count, col_start, col_ncell, col_cellstart;

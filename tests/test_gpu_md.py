@@ -119,17 +119,18 @@ def test_md_pruned_cycle_graphs_bitwise_equal_eager(c1):
         assert np.array_equal(out[0][k], out[1][k]), k
 
 
-def test_md_priority_trial_cycles_bitwise_equal_eager(c1):
-    """The first six whole-cycle graphs alternate the PME kernels' issue
-    priority (off, then on; timed) and the faster setting stays.  Across two
-    propagate calls (eight cycles: the trial, the switch and cycles after it)
-    the trajectory is bitwise the eager one, which never raises priority."""
+def test_md_launch_trial_cycles_bitwise_equal_eager(c1):
+    """The first eight whole-cycle graphs run under four launch settings (PME
+    issue priority off/on x pair kernel captured last/first; one untimed and
+    one timed cycle each) and the fastest stays.  Across two propagate calls
+    (ten cycles: the trial, the switch and cycles after it) the trajectory is
+    bitwise the eager one, which keeps the default setting."""
     sp, xyz, vel = c1
     out = []
     for graph in (True, False):
         fm, lt = _make(sp, xyz, vel, graph=graph)
         lt.propagate(70)
-        lt.propagate(105)
+        lt.propagate(145)
         out.append(lt.getState())
         lt.close()
         fm.close()
