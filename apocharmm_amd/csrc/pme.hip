@@ -598,7 +598,8 @@ __device__ __forceinline__ void ffty_body(int bid, typename Cx<CT>::T *__restric
 template <typename CT, bool PRIO>
 __global__ __launch_bounds__(256) void k_ffty(typename Cx<CT>::T *__restrict__ data, int H, int K1,
                                               int W, FftPlan p,
-                                              const typename Cx<CT>::T *__restrict__ twg, int inv) {
+                                              const typename Cx<CT>::T *__restrict__ twg,
+                                              int inv) {
   if constexpr (PRIO) pme_wave_priority();
   ffty_body<CT>(blockIdx.x, data, H, K1, W, p, twg, inv);
 }
@@ -726,8 +727,9 @@ template <typename CT> struct PmeShapes {
 };
 
 // rows: x-inverse row pairs per block; cols: y/z-pass columns per block (both
-// capped by 64 KiB of LDS); threads: block size of those passes (64..256).  The spread slab is kSlabBy x kSlabBz rows, halved
-// (z first, then y down to 2) until its LDS fits.
+// capped by 64 KiB of LDS); threads: block size of those passes (64..256).
+// The spread slab is kSlabBy x kSlabBz rows, halved (z first, then y down to
+// 2) until its LDS fits.
 template <typename CT>
 static PmeShapes<CT> pme_shapes(const NbSystem &S, int rows = kFftRowsPerBlock,
                                 int cols = kFftColsPerBlock, int threads = kFftThreads) {
@@ -780,8 +782,8 @@ static void recip_t(NbSystem &S, const float4 *x, int64_t *force, int stride, bo
   const double qunscale = 1.0;  // sorted coordinates carry the raw charge
   hipEvent_t t0 = nullptr;
   S.tic(1, s, t0);
-  k_spread_fftx<CT, PRIO><<<P.nslabs, kSpreadThreads, P.shs, s>>>(S.posq.get(), S.cell_bb.get(), S.meta.get(),
-                                                  P.bg, qunscale, plan_of(S.fx), twx, cgrid);
+  k_spread_fftx<CT, PRIO><<<P.nslabs, kSpreadThreads, P.shs, s>>>(
+      S.posq.get(), S.cell_bb.get(), S.meta.get(), P.bg, qunscale, plan_of(S.fx), twx, cgrid);
   APO_LAUNCH_CHECK();
   S.toc(1, s, t0);
   S.tic(2, s, t0);
@@ -791,16 +793,17 @@ static void recip_t(NbSystem &S, const float4 *x, int64_t *force, int stride, bo
   double *part = S.parts.get() + S.part.recip_off;
   if (EV) {
     launch_fill_f64(part, (size_t)S.part.recip_n * kRecipPart, s);
-    k_fftz_conv<CT, true, PRIO><<<K1 * P.ncz, P.ft, P.shz, s>>>(cgrid, P.c, plan_of(S.fz), twz, pre, part);
+    k_fftz_conv<CT, true, PRIO><<<K1 * P.ncz, P.ft, P.shz, s>>>(cgrid, P.c, plan_of(S.fz), twz,
+                                                                pre, part);
   } else {
-    k_fftz_conv<CT, false, PRIO><<<K1 * P.ncz, P.ft, P.shz, s>>>(cgrid, P.c, plan_of(S.fz), twz, pre,
-                                                          part);
+    k_fftz_conv<CT, false, PRIO><<<K1 * P.ncz, P.ft, P.shz, s>>>(cgrid, P.c, plan_of(S.fz), twz,
+                                                                 pre, part);
   }
   APO_LAUNCH_CHECK();
   k_ffty<CT, PRIO><<<K2 * P.ncy, P.ft, P.shy, s>>>(cgrid, H, K1, P.wy, plan_of(S.fy), twy, 1);
   APO_LAUNCH_CHECK();
-  k_fftx_inv<CT, PRIO><<<(P.npair + P.rb - 1) / P.rb, P.ft, P.shx, s>>>(cgrid, P.nrows, plan_of(S.fx),
-                                                                  P.rb, twx, phi);
+  k_fftx_inv<CT, PRIO><<<(P.npair + P.rb - 1) / P.rb, P.ft, P.shx, s>>>(
+      cgrid, P.nrows, plan_of(S.fx), P.rb, twx, phi);
   APO_LAUNCH_CHECK();
   S.toc(2, s, t0);
   S.tic(3, s, t0);
