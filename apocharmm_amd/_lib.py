@@ -69,7 +69,8 @@ class LtParams(ctypes.Structure):
                 ("nwater", ctypes.c_int32), ("use_graph", ctypes.c_int32),
                 ("waters", _i32p), ("water_geom", ctypes.c_double * 4),
                 ("mass", ctypes.POINTER(ctypes.c_double)), ("charge", _f32p),
-                ("list_policy", ctypes.c_int32), ("prune_every", ctypes.c_int32)]
+                ("list_policy", ctypes.c_int32), ("prune_every", ctypes.c_int32),
+                ("launch_setting", ctypes.c_int32)]
 
 
 _LIB = None
@@ -108,13 +109,14 @@ def lib():
         L.apo_lt_get_state.argtypes = [vp, vp, vp, vp, vp, vp, i64p]
         L.apo_lt_xyzq.argtypes = [vp, ctypes.POINTER(vp)]
         L.apo_lt_list_info.argtypes = [vp, dp]
+        L.apo_lt_launch_info.argtypes = [vp, dp]
         for name in ("apo_nb_create", "apo_nb_destroy", "apo_nb_set_box", "apo_nb_reset_list",
                      "apo_nb_calc_force", "apo_nb_force_stride", "apo_nb_num_atoms",
                      "apo_nb_force_to_double", "apo_nb_list_stats", "apo_topology_exclusions",
                      "apo_lt_create", "apo_lt_destroy", "apo_lt_initialize", "apo_lt_propagate",
                      "apo_lt_propagate_batch", "apo_lt_batch_create", "apo_lt_batch_propagate",
                      "apo_lt_batch_destroy", "apo_lt_batch_autotune",
-                     "apo_lt_get_state", "apo_lt_xyzq", "apo_lt_list_info"):
+                     "apo_lt_get_state", "apo_lt_xyzq", "apo_lt_list_info", "apo_lt_launch_info"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
     return _LIB

@@ -263,6 +263,9 @@ template <> struct Tab2<double> { using T = double2; };
 template <typename CT> __device__ __forceinline__ CT erfc_ct(CT x);
 template <> __device__ __forceinline__ float erfc_ct<float>(float x) { return erfcf(x); }
 template <> __device__ __forceinline__ double erfc_ct<double>(double x) { return erfc(x); }
+template <typename CT> __device__ __forceinline__ CT erf_ct(CT x);
+template <> __device__ __forceinline__ float erf_ct<float>(float x) { return erff(x); }
+template <> __device__ __forceinline__ double erf_ct<double>(double x) { return erf(x); }
 template <typename CT> __device__ __forceinline__ CT exp_ct(CT x);
 template <> __device__ __forceinline__ float exp_ct<float>(float x) { return expf(x); }
 template <> __device__ __forceinline__ double exp_ct<double>(double x) { return exp(x); }
@@ -274,11 +277,16 @@ struct P14BatchArgs {
   unsigned long long *f;
 };
 
+// Energies of energy steps are evaluated in FP64 from the same float xyzq (a
+// 1-4 or exclusion term is ~1e2 larger than the sums it cancels into):
+// PE / tab14_e are the FP64 constants and (6 c6, 12 c12) table.
 template <typename CT, bool CALC_E, bool CALC_V>
 __device__ __forceinline__ void pairs14_body(int bid, const P14Args &a, const PairConsts<CT> &P,
                                              const typename Tab2<CT>::T *__restrict__ tab14,
                                              unsigned long long *__restrict__ f,
-                                             double *__restrict__ part) {
+                                             double *__restrict__ part,
+                                             const PairConsts<double> &PE,
+                                             const double2 *__restrict__ tab14_e) {
   __shared__ double sh[4 * 16];
   double acc[kP14Part];
 #pragma unroll
@@ -292,11 +300,32 @@ __device__ __forceinline__ void pairs14_body(int bid, const P14Args &a, const Pa
     const CT r2 = dx * dx + dy * dy + dz * dz;
     const CT r = sqrt_ct<CT>(r2), rinv = (CT)1 / r;
     const CT qq = (CT)332.0716 * (CT)xi.w * (CT)xj.w;
-    // Ewald with e14fac: qq (erfc + e14fac - 1)/r; exclusions use e14fac = 0
+    // Ewald with e14fac: qq (erfc + e14fac - 1)/r; exclusions use e14fac = 0,
+    // evaluated as -qq erf/r (no cancellation of erfc against 1 at small kappa r)
     const CT efac = ex ? (CT)0 : (CT)a.e14fac;
-    const CT erfcv = erfc_ct<CT>(P.kappa * r);
-    const CT u_el = qq * (erfcv + efac - (CT)1) * rinv;
+    const CT u_el = ex ? -qq * erf_ct<CT>(P.kappa * r) * rinv
+                       : qq * (erfc_ct<CT>(P.kappa * r) + efac - (CT)1) * rinv;
     CT fr = -qq * P.two_kappa_sqrtpi * exp_ct<CT>(-P.kappa2 * r2) - u_el;
+    double ed = 0.0, evd = 0.0;  // FP64 energies
+    if (CALC_E) {
+      const double ex_ = (double)xi.x - (double)xj.x, ey_ = (double)xi.y - (double)xj.y,
+                   ez_ = (double)xi.z - (double)xj.z;
+      const double er2 = ex_ * ex_ + ey_ * ey_ + ez_ * ez_, er = sqrt(er2), erinv = 1.0 / er;
+      const double qqd = 332.0716 * (double)xi.w * (double)xj.w;
+      ed = ex ? -qqd * erf(PE.kappa * er) * erinv
+              : qqd * (erfc(PE.kappa * er) + a.e14fac - 1.0) * erinv;
+      if (!ex) {
+        const int ti = a.vdwtype[pr.x], tj = a.vdwtype[pr.y];
+        const int aa = ti > tj ? ti : tj, bb = ti > tj ? tj : ti;
+        const double2 c = tab14_e[aa * (aa + 1) / 2 + bb];
+        const double er3 = erinv * erinv * erinv, er6 = er3 * er3;
+        const bool eo = er2 > PE.ron2;
+        const double e6 = eo ? er3 - PE.roffinv3 : er3, e12 = eo ? er6 - PE.roffinv6 : er6;
+        const double A6 = eo ? PE.k6 : 1.0, A12 = eo ? PE.k12 : 1.0;
+        const double C6 = eo ? 0.0 : PE.dv6, C12 = eo ? 0.0 : PE.dv12;
+        evd = c.y * (1.0 / 12.0) * (A12 * e12 * e12 + C12) - c.x * (1.0 / 6.0) * (A6 * e6 * e6 + C6);
+      }
+    }
     if (!ex) {
       const int ti = a.vdwtype[pr.x], tj = a.vdwtype[pr.y];
       const int aa = ti > tj ? ti : tj, bb = ti > tj ? tj : ti;
@@ -307,13 +336,11 @@ __device__ __forceinline__ void pairs14_body(int bid, const P14Args &a, const Pa
       const CT A12 = outer ? P.k12 : (CT)1, B12 = outer ? P.roffinv6 : (CT)0;
       const CT t6 = rinv3 - B6, t12 = rinv6 - B12;
       fr += c.x * A6 * t6 * rinv3 - c.y * A12 * t12 * rinv6;
-      if (CALC_E) {
-        const CT C6 = outer ? (CT)0 : P.dv6, C12 = outer ? (CT)0 : P.dv12;
-        acc[1] += (double)(c.y * ((CT)1 / (CT)12) * (A12 * t12 * t12 + C12) -
-                           c.x * ((CT)1 / (CT)6) * (A6 * t6 * t6 + C6));
-      }
     }
-    if (CALC_E) acc[ex ? 2 : 0] += (double)u_el;
+    if (CALC_E) {
+      acc[1] += evd;
+      acc[ex ? 2 : 0] += ed;
+    }
     const CT fij = fr * rinv * rinv;
     add_grad<CT>(f, a.stride, pr.x, fij * dx, fij * dy, fij * dz);
     add_grad<CT>(f, a.stride, pr.y, -fij * dx, -fij * dy, -fij * dz);
@@ -325,14 +352,18 @@ template <typename CT, bool CALC_E, bool CALC_V>
 __global__ __launch_bounds__(256) void k_pairs14(P14Args a, const PairConsts<CT> P,
                                                  const typename Tab2<CT>::T *__restrict__ tab14,
                                                  unsigned long long *__restrict__ f,
-                                                 double *__restrict__ part) {
-  pairs14_body<CT, CALC_E, CALC_V>(blockIdx.x, a, P, tab14, f, part);
+                                                 double *__restrict__ part,
+                                                 const PairConsts<double> PE,
+                                                 const double2 *__restrict__ tab14_e) {
+  pairs14_body<CT, CALC_E, CALC_V>(blockIdx.x, a, P, tab14, f, part, PE, tab14_e);
 }
 __global__ __launch_bounds__(256) void kb_pairs14(const P14BatchArgs *__restrict__ tab,
                                                   const int *__restrict__ bstart, int nsys) {
   const int s = batch_system(bstart, nsys);
   const P14BatchArgs &a = tab[s];
-  pairs14_body<float, false, false>(blockIdx.x - bstart[s], a.a, a.P, a.tab14, a.f, nullptr);
+  const PairConsts<double> no_energy{};
+  pairs14_body<float, false, false>(blockIdx.x - bstart[s], a.a, a.P, a.tab14, a.f, nullptr,
+                                    no_energy, nullptr);
 }
 
 
@@ -382,10 +413,11 @@ void pairs14_launch(NbSystem &S, const float4 *x, int64_t *force, int stride, bo
     constexpr int m = decltype(ev)::value;
     if (S.precision == 1)
       k_pairs14<double, (m & 1) != 0, (m & 2) != 0>
-          <<<G, 256, 0, s>>>(a, S.pc_d, S.vdwtab14_d.get(), f, part);
+          <<<G, 256, 0, s>>>(a, S.pc_d, S.vdwtab14_d.get(), f, part, S.pc_d,
+                             S.vdwtab14_d.get());
     else
       k_pairs14<float, (m & 1) != 0, (m & 2) != 0>
-          <<<G, 256, 0, s>>>(a, S.pc_f, S.vdwtab14.get(), f, part);
+          <<<G, 256, 0, s>>>(a, S.pc_f, S.vdwtab14.get(), f, part, S.pc_d, S.vdwtab14_d.get());
   };
   dispatch_ev<float>(E, V, go);
   APO_LAUNCH_CHECK();

@@ -31,6 +31,7 @@ void lt_get_state(LtSystem &L, double *xyz, double *vel, double *dprev, double *
                   double *thermo, int64_t *step);
 const float *lt_xyzq(LtSystem &L);
 void lt_list_info(LtSystem &L, double *out4);
+void lt_launch_info(LtSystem &L, double *out6);
 void batch_autotune(LtBatch &B, hipStream_t caller, double *report9);
 }  // namespace apo
 
@@ -105,6 +106,9 @@ APO_API apo_status apo_nb_create(const apo_nb_params *params, const apo_nb_topol
 APO_API apo_status apo_nb_destroy(apo_nb_system *sys) {
   return guard([&] {
     if (!sys) return;
+    if (sys->sys && sys->sys->n_integrators > 0)
+      throw apo::InvalidArgument("an integrator is still bound to this force system: destroy "
+                                 "the integrator first");
     delete sys->sys;
     delete sys;
   });
@@ -293,6 +297,13 @@ APO_API apo_status apo_lt_list_info(apo_lt_integrator *lt, double *out4) {
   return guard([&] {
     if (!out4) throw apo::InvalidArgument("null output");
     apo::lt_list_info(get(lt), out4);
+  });
+}
+
+APO_API apo_status apo_lt_launch_info(apo_lt_integrator *lt, double *out6) {
+  return guard([&] {
+    if (!out6) throw apo::InvalidArgument("null output");
+    apo::lt_launch_info(get(lt), out6);
   });
 }
 

@@ -656,6 +656,8 @@ struct LtSystem {
   unsigned long long step = 0;           // host mirror of counters[0]
   long steps_since_rebuild = 0;
   bool initialized = false;
+  // live batches that hold this integrator (it must not be destroyed before them)
+  int n_batches = 0;
   hipStream_t st = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
   // captured step bodies: [0] plain step, [1] step with image centering and
@@ -665,6 +667,11 @@ struct LtSystem {
   unsigned long long graph_gen[3] = {~0ull, ~0ull, ~0ull};
 
   ~LtSystem() {
+    // kernels of this integrator write the pinned h_disp and run on st and on
+    // the force system's side streams: drain the device before releasing any
+    // of them (destruction is rare; the reference's dealloc synchronises too)
+    (void)hipDeviceSynchronize();
+    if (nb) nb->n_integrators--;
     for (hipGraphExec_t g : gexec)
       if (g) (void)hipGraphExecDestroy(g);
     if (gcycle) (void)hipGraphExecDestroy(gcycle);
@@ -846,15 +853,21 @@ struct LtSystem {
 
   // Launch-settings trial (NbSystem::pme_prio, NbSystem::pair_first): the
   // first whole list cycles run under each of the four settings (bit 0: PME
-  // issue priority, bit 1: pair kernel first), one untimed cycle and one
-  // timed cycle each, and the fastest setting stays.  The settings order
-  // instruction issue and dispatch only, so these are ordinary cycles of the
-  // trajectory.
+  // issue priority, bit 1: pair kernel first), one untimed cycle and
+  // kTrialTimed timed cycles each, and the setting with the fastest timed
+  // cycle stays.  The settings order instruction issue and dispatch only, so
+  // these are ordinary cycles of the trajectory.  The timed cycles wait for
+  // the GPU on the host; launch_fixed (apo_lt_params::launch_setting) skips
+  // the trial for callers that share the GPU or must not block.
   static constexpr int kTrialSettings = 4;
-  static constexpr int kTrialCycles = 2 * kTrialSettings;
+  static constexpr int kTrialTimed = 3;
+  static constexpr int kTrialPer = 1 + kTrialTimed;
+  static constexpr int kTrialCycles = kTrialPer * kTrialSettings;
   int trial = 0;  // trial cycles run so far
+  int launch_fixed = -1;  // >= 0: the setting, no trial
   float trial_ms[kTrialSettings] = {};
   hipEvent_t trial_ev[2] = {nullptr, nullptr};
+  bool trial_pending() const { return launch_fixed < 0 && trial < kTrialCycles; }
   void set_launch(int setting) {
     const int prio = setting & 1, first = (setting >> 1) & 1;
     if (nb->pme_prio == prio && nb->pair_first == first) return;
@@ -863,15 +876,17 @@ struct LtSystem {
     nb->generation++;  // captured graphs hold both as launch constants
   }
   void trial_cycle(hipStream_t s) {
-    const int setting = trial / 2;
-    const bool timed = trial % 2 != 0;
+    const int setting = trial / kTrialPer;
+    const bool timed = trial % kTrialPer != 0;
     set_launch(setting);
     if (timed) APO_HIP(hipEventRecord(trial_ev[0], s));
     run_cycle(s);
     if (timed) {
       APO_HIP(hipEventRecord(trial_ev[1], s));
       APO_HIP(hipEventSynchronize(trial_ev[1]));
-      APO_HIP(hipEventElapsedTime(&trial_ms[setting], trial_ev[0], trial_ev[1]));
+      float ms = 0.f;
+      APO_HIP(hipEventElapsedTime(&ms, trial_ev[0], trial_ev[1]));
+      if (trial % kTrialPer == 1 || ms < trial_ms[setting]) trial_ms[setting] = ms;
     }
     if (++trial == kTrialCycles) {
       int best = 0;
@@ -888,7 +903,7 @@ struct LtSystem {
     begin(s);
     for (int k = 0; k < nsteps;) {
       if (nsteps - k >= nbfreq && steps_since_rebuild % nbfreq == 0 && cycle_ok()) {
-        if (trial < kTrialCycles) trial_cycle(s);
+        if (trial_pending()) trial_cycle(s);
         else run_cycle(s);
         k += nbfreq;
       } else {
@@ -1010,6 +1025,7 @@ LtSystem *create_lt(NbSystem *nb, const apo_lt_params &p, const double *xyz, con
   auto *L = new LtSystem();
   try {
     L->nb = nb;
+    nb->n_integrators++;
     L->n = n;
     L->nbfreq = p.nbfreq > 0 ? p.nbfreq : 20;
     L->comfreq = p.comfreq == 0 ? 1000 : p.comfreq;
@@ -1121,6 +1137,12 @@ LtSystem *create_lt(NbSystem *nb, const apo_lt_params &p, const double *xyz, con
     if (L->list_policy != 0 && L->list_policy != 1)
       throw InvalidArgument("list_policy must be 0 (fixed cadence) or 1 (displacement check)");
     L->half_buffer = 0.5 * (nb->cutnb - nb->ctofnb);
+    if (p.launch_setting < 0 || p.launch_setting > LtSystem::kTrialSettings)
+      throw InvalidArgument("launch_setting must be 0 (timing trial) or 1..4");
+    if (p.launch_setting > 0) {
+      L->launch_fixed = p.launch_setting - 1;
+      L->set_launch(L->launch_fixed);
+    }
     L->prune_every = p.prune_every;
     if (L->prune_every < 0) throw InvalidArgument("prune_every must be >= 0");
     if (L->prune_every > 0 && L->list_policy != 0)
@@ -1142,7 +1164,11 @@ LtSystem *create_lt(NbSystem *nb, const apo_lt_params &p, const double *xyz, con
   return L;
 }
 
-void destroy_lt(LtSystem *L) { delete L; }
+void destroy_lt(LtSystem *L) {
+  if (L && L->n_batches > 0)
+    throw InvalidArgument("the integrator is still a member of a batch: destroy the batch first");
+  delete L;
+}
 
 // Work runs on the integrator's own stream, ordered after and before the
 // caller's stream by events, so the non-rebuild steps can be captured.
@@ -1200,6 +1226,8 @@ struct LtBatch {
   int fft_threads = kFftThreads, fft_rows = kFftRowsPerBlock, fft_cols = kFftColsPerBlock;
 
   ~LtBatch() {
+    (void)hipDeviceSynchronize();
+    for (LtSystem *L : sys) L->n_batches--;
     for (hipGraphExec_t g : ex)
       if (g) (void)hipGraphExecDestroy(g);
     for (hipEvent_t e : lane_ev) (void)hipEventDestroy(e);
@@ -1488,6 +1516,7 @@ LtBatch *create_batch(LtSystem **Ls, int n, int lanes) {
   auto *B = new LtBatch();
   try {
     B->sys.assign(Ls, Ls + n);
+    for (LtSystem *L : B->sys) L->n_batches++;  // ~LtBatch gives them back
     B->lanes = std::max(1, std::min(lanes > 0 ? lanes : 4, n));
     // one stream per system inside a step: no fork/join within a system
     for (LtSystem *L : B->sys) {
@@ -1577,6 +1606,12 @@ void lt_get_state(LtSystem &L, double *xyz, double *vel, double *dprev, double *
   if (thermo)
     APO_HIP(hipMemcpy(thermo, L.thermo.get(), 6 * sizeof(double), hipMemcpyDeviceToHost));
   if (step) *step = (int64_t)L.step;
+}
+
+void lt_launch_info(LtSystem &L, double *out6) {
+  out6[0] = L.trial_pending() ? -1.0 : (double)(L.nb->pme_prio | (L.nb->pair_first << 1));
+  out6[1] = (double)L.trial;
+  for (int k = 0; k < LtSystem::kTrialSettings; k++) out6[2 + k] = L.trial_ms[k];
 }
 
 const float *lt_xyzq(LtSystem &L) { return reinterpret_cast<const float *>(L.xyzq.get()); }

@@ -97,7 +97,7 @@ __device__ __forceinline__ void pair_body(
     const int *__restrict__ cand_j, const int *__restrict__ meta,
     const typename Vec2<CT>::T *__restrict__ vdwtab, const typename Vec2<CT>::T *__restrict__ vdwsq,
     int ntypes, const PairConsts<CT> &P, CT qfac, unsigned long long *__restrict__ fsort, int ss,
-    double *__restrict__ part) {
+    double *__restrict__ part, const PairConsts<double> &PE, const double2 *__restrict__ vdwtab_e) {
   using T2 = typename Vec2<CT>::T;
   extern __shared__ char smem_pair[];
   T2 *sh_tab = reinterpret_cast<T2 *>(smem_pair);  // ntypes^2 entries when TAB_LDS
@@ -204,11 +204,28 @@ __device__ __forceinline__ void pair_body(
           fi[3 * k + 1] += gy;
           fi[3 * k + 2] += gz;
           if (CALC_E) {
-            const CT A6 = outer ? P.k6 : (CT)1, A12 = outer ? P.k12 : (CT)1;
-            const CT C6 = outer ? (CT)0 : P.dv6, C12 = outer ? (CT)0 : P.dv12;
-            el += (double)u_el;
-            ev += (double)(c.y * ((CT)1 / (CT)12) * (A12 * t12 * t12 + C12) -
-                           c.x * ((CT)1 / (CT)6) * (A6 * t6 * t6 + C6));
+            // energies of energy steps in FP64 from the same fixed-point
+            // geometry: a pair energy is ~1e2 larger than the sums it
+            // cancels into, and FP32 pair arithmetic (~2e-7 relative) would
+            // dominate the energy error
+            (void)u_el;
+            const double ex = (double)(int)(pix[k] - pj.x) * PE.sx;
+            const double ey = (double)(int)(piy[k] - pj.y) * PE.sy;
+            const double ez = (double)(int)(piz[k] - pj.z) * PE.sz;
+            const double er2 = ex * ex + ey * ey + ez * ez;
+            const double er = sqrt(er2), erinv = 1.0 / er;
+            el += (double)qi[k] * (double)charge_of<CT>(pj.w) * kCcelec * erfc(PE.kappa * er) *
+                  erinv;
+            const int ti = TAB_LDS ? rowi[k] / ntypes : rowi[k];
+            const int aa = ti > tj ? ti : tj, bb = ti > tj ? tj : ti;
+            const double2 ce = vdwtab_e[aa * (aa + 1) / 2 + bb];
+            const double er3 = erinv * erinv * erinv, er6 = er3 * er3;
+            const bool eo = er2 > PE.ron2;
+            const double e6 = eo ? er3 - PE.roffinv3 : er3, e12 = eo ? er6 - PE.roffinv6 : er6;
+            const double A6 = eo ? PE.k6 : 1.0, A12 = eo ? PE.k12 : 1.0;
+            const double C6 = eo ? 0.0 : PE.dv6, C12 = eo ? 0.0 : PE.dv12;
+            ev += ce.y * (1.0 / 12.0) * (A12 * e12 * e12 + C12) -
+                  ce.x * (1.0 / 6.0) * (A6 * e6 * e6 + C6);
           }
           if (CALC_V) {
             v0 -= gx * dx;
@@ -268,9 +285,10 @@ __global__ __launch_bounds__(64) void k_pair(
     const uint32_t *__restrict__ ent, const int *__restrict__ cand_j, const int *__restrict__ meta,
     const typename Vec2<CT>::T *__restrict__ vdwtab, const typename Vec2<CT>::T *__restrict__ vdwsq,
     int ntypes, const PairConsts<CT> P, CT qfac, unsigned long long *__restrict__ fsort, int ss,
-    double *__restrict__ part) {
+    double *__restrict__ part, const PairConsts<double> PE, const double2 *__restrict__ vdwtab_e) {
   pair_body<CT, CALC_E, CALC_V, TAB_LDS>(blockIdx.x, gridDim.x, posq, types, items, ent, cand_j,
-                                         meta, vdwtab, vdwsq, ntypes, P, qfac, fsort, ss, part);
+                                         meta, vdwtab, vdwsq, ntypes, P, qfac, fsort, ss, part, PE,
+                                         vdwtab_e);
 }
 
 // Fused launch over several systems: mixed precision, no energy/virial (the
@@ -280,10 +298,11 @@ __global__ __launch_bounds__(64) void kb_pair(const PairArgs *__restrict__ tab,
                                               const int *__restrict__ bstart, int nsys) {
   const int s = batch_system(bstart, nsys);
   const PairArgs &a = tab[s];
+  const PairConsts<double> no_energy{};
   pair_body<float, false, false, TAB_LDS>(blockIdx.x - bstart[s], bstart[s + 1] - bstart[s],
                                           a.posq, a.types, a.items, a.ent, a.cand_j, a.meta,
                                           a.vdwtab, a.vdwsq, a.ntypes, a.P, a.qfac, a.fsort, a.ss,
-                                          nullptr);
+                                          nullptr, no_energy, nullptr);
 }
 
 template <typename CT>
@@ -299,7 +318,7 @@ static void launch_pair_t(NbSystem &S, const typename Vec2<CT>::T *tab,
   const uint32_t *ent = pruned ? S.ent_p.get() : S.ent.get();
 #define APO_PAIR_ARGS                                                                   \
   S.posq.get(), S.type_s.get(), items, ent, S.cand_j.get(), S.meta.get(), tab, sq, S.ntypes, P, \
-      qfac, f, S.ssort, part
+      qfac, f, S.ssort, part, S.pc_d, S.vdwtab_d.get()
 #define APO_PAIR_GO(EE, VV)                                                             \
   if (lds)                                                                              \
     k_pair<CT, EE, VV, true><<<G, 64, lds_bytes, s>>>(APO_PAIR_ARGS);                  \

@@ -570,6 +570,10 @@ void NbSystem::profile_flush() {
 }
 
 NbSystem::~NbSystem() {
+  // list builds write the pinned h_meta from whatever stream the caller gave
+  // them, and PME / bonded work runs on the side streams: drain the device
+  // before any of them is released
+  (void)hipDeviceSynchronize();
   if (h_meta) (void)hipHostFree(h_meta);
   for (auto e : ev_pool) (void)hipEventDestroy(e);
   for (hipEvent_t e : {ev_fork, ev_sorted, ev_pme, ev_bond})

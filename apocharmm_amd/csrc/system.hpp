@@ -63,6 +63,8 @@ struct NbSystem {
   // bonded, so a graph dispatches it first from the caller's queue.
   int pme_prio = 0;
   int pair_first = 0;
+  // integrators bound to this system (it must not be destroyed before them)
+  int n_integrators = 0;
   std::vector<int> h_bonds;  // 2 * nbond, host copy for molecule grouping
   DevBuf<int> meta;   // [0] ncell [1] ncand [2] nent [3] nitem [4] maxcol
   DevBuf<int> col_count, col_start, col_ncell, col_cellstart;

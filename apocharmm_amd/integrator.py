@@ -40,6 +40,7 @@ class LangevinThermostatIntegrator:
         self._use_graph = True
         self._list_policy = 0
         self._prune_every = 0
+        self._launch_setting = 0
         self._handle = ctypes.c_void_p()
         self._fm = None
         self._keep = []
@@ -74,6 +75,21 @@ class LangevinThermostatIntegrator:
         at r_off + (cutnb - r_off) * every / nbfreq (no counterpart in the
         reference; needs the fixed cadence); 0 turns it off."""
         self._prune_every = int(every)
+
+    def setLaunchSetting(self, setting):
+        """None: the first list cycles time the four launch orders of the
+        force branches and keep the fastest (waits for the GPU on the host
+        during those cycles); 0..3: use that order without a trial (bit 0 =
+        PME kernels at raised wave issue priority, bit 1 = pair kernel
+        captured first).  No result bit depends on it."""
+        self._launch_setting = 0 if setting is None else int(setting) + 1
+
+    def getLaunchInfo(self):
+        """dict: setting (0..3, -1 while the trial runs), trial_cycles, and
+        the fastest timed cycle (ms) of each setting."""
+        out = (ctypes.c_double * 6)()
+        _lib.check(_lib.lib().apo_lt_launch_info(self._handle, out))
+        return dict(setting=int(out[0]), trial_cycles=int(out[1]), trial_ms=list(out[2:6]))
 
     def getTimeStep(self):
         return self._dt
@@ -111,6 +127,7 @@ class LangevinThermostatIntegrator:
         p.use_graph = 1 if self._use_graph else 0
         p.list_policy = self._list_policy
         p.prune_every = self._prune_every
+        p.launch_setting = self._launch_setting
         p.waters = waters.ctypes.data_as(_lib._i32p)
         if water_geom is not None:
             for k in range(4):

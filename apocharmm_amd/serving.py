@@ -201,6 +201,9 @@ class StreamServer:
     def run(self, jobs):
         """Serves `jobs` (prepared) in real time; returns them with t_admit /
         t_done set (seconds since the first arrival window opened)."""
+        if not _SERVING_ENABLED:
+            raise RuntimeError("stream serving is disabled: an unresolved host segmentation "
+                               "fault followed many create/propagate/destroy cycles")
         import torch
         pending = sorted(jobs, key=lambda j: j.arrival)
         if self.hbm_budget is None:

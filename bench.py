@@ -43,7 +43,13 @@ REBUILD_EVERY = 20
 # run-to-run spread)
 PRUNE_MIN_ATOMS = 50000
 FLOP_PER_PAIR = 61
-PEAK_FP32_TFLOPS = 157.3
+PEAK_FP32_TFLOPS = 157.3          # packed FP32 (v_pk_fma_f32)
+PEAK_FP32_NONPACKED_TFLOPS = 78.6  # what a build without packed FP32 can issue
+# MD runs start from a seeded lattice, which releases potential energy as it
+# melts (the first steps heat the box to ~900 K): a Langevin equilibration at
+# strong friction brings it to the 300 K state before anything is timed
+EQUIL_STEPS = 2000
+EQUIL_FRICTION = 10.0
 KBOLTZ = 1.987191e-3  # kcal/mol/K
 
 
@@ -98,6 +104,33 @@ def config_spec(name):
     return {"c1": systems.config1, "c2": systems.config2, "c3": systems.config3}[name]()
 
 
+def host_cpus():
+    """(threads to use, description): the CPUs this process may run on,
+    capped by a cgroup CPU quota when one is set, and the host's CPU model."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    use = avail if quota is None else max(1, min(avail, int(quota)))
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return use, {"cpu_model": model, "host_cpus": os.cpu_count(), "affinity_cpus": avail,
+                 "cgroup_cpu_quota": quota}
+
+
 def cpu_baseline(spec, threads, md):
     import numpy as np
     from apocharmm_amd import systems
@@ -126,6 +159,35 @@ def cpu_baseline(spec, threads, md):
     return {"value": DT_PS * 86.4 / t_eval, "unit": "ns/day", "cores": threads, "kind": "port",
             "sample": "%d FP64 %s of the same system (oracle, cell list + PME, %d OpenMP "
                       "threads), %.3f s each; ns/day = dt*86.4/t_step" % (reps, what, threads, t_eval)}
+
+
+def check_list(lt, fm, spec, prune_every):
+    """List validity at the end of the timed run: one more whole list cycle
+    (rebuild, prunes), then the gradient of its last step, which used a list
+    nbfreq - 1 steps old (pruned prune_every - 1 steps before), against a list
+    built fresh from the same coordinates.  They differ by FP32 rounding
+    (pairs grouped differently) plus any pair inside r_off the list lacked."""
+    import numpy as np
+    import torch
+    from apocharmm_amd import systems
+    lt.propagate(REBUILD_EVERY)
+    st = lt.getState()
+    info = lt.getListInfo()
+    x = st["xyz"] - st["dprev"]  # where the last step evaluated the force
+    fm.calcForce(torch.tensor(systems.xyzq_of(spec, x), device="cuda"), reset=True)
+    torch.cuda.synchronize()
+    g_fresh = fm.getForces().cpu().numpy()
+    g = st["grad"]
+    rel = float(np.sqrt(np.mean(np.sum((g - g_fresh) ** 2, 1)) /
+                        np.mean(np.sum(g_fresh ** 2, 1))))
+    return {"max_displacement_A": round(info["max_displacement"], 4),
+            "max_step_displacement_A": round(info["max_step_displacement"], 4),
+            "half_buffer_A": info["half_buffer"],
+            "list_age_steps": REBUILD_EVERY - 1,
+            "prune_age_steps": (prune_every - 1) if prune_every else None,
+            "grad_vs_fresh_list_rms_rel": rel,
+            "grad_vs_fresh_list_max_abs": float(np.abs(g - g_fresh).max()),
+            "valid": bool(rel <= 1e-6 and info["max_displacement"] <= info["half_buffer"])}
 
 
 def copy_calibration():
@@ -423,6 +485,9 @@ def main():
                     help="c4: keep the default launch shapes (no timing pass)")
     ap.add_argument("--nve", action="store_true",
                     help="energy-conservation run: Langevin equilibration, then --steps NVE steps")
+    ap.add_argument("--equil-md-steps", type=int, default=EQUIL_STEPS,
+                    help="MD: untimed Langevin equilibration at friction %g/ps before the warmup "
+                         "(reported in config.equilibration; 0 = none)" % EQUIL_FRICTION)
     ap.add_argument("--equil-steps", type=int, default=10000)
     ap.add_argument("--equil-friction", type=float, default=5.0)
     ap.add_argument("--sample-every", type=int, default=500)
@@ -466,15 +531,40 @@ def main():
     torch.cuda.set_device(0)
     fm = ForceManager(spec, a.precision).initialize()
     md = a.step == "md"
+    equil = None
     if md:
         vseed = VSEED[a.config]
         vel = maxwell_velocities(spec["masses"], TEMP, vseed)
-        lt = LangevinThermostatIntegrator(DT_PS, friction=FRICTION, temperature=TEMP, seed=SEED)
-        lt.setNonbondedListUpdateFrequency(REBUILD_EVERY)
         if a.prune_every is None:
             a.prune_every = 10 if n >= PRUNE_MIN_ATOMS else 0
+        xyz = np.asarray(spec["xyz"], np.float64)
+        launch = None
+        if a.equil_md_steps > 0:
+            # equilibration (untimed): the lattice start relaxes to 300 K; its
+            # first list cycles also time the launch orders of the force
+            # branches, and the timed integrator keeps the chosen one
+            eq = LangevinThermostatIntegrator(DT_PS, friction=EQUIL_FRICTION, temperature=TEMP,
+                                              seed=SEED + 1)
+            eq.setNonbondedListUpdateFrequency(REBUILD_EVERY)
+            eq.setListPruneFrequency(a.prune_every)
+            eq.setCharmmContext(fm, xyz, vel)
+            t_eq0 = time.perf_counter()
+            eq.propagate(a.equil_md_steps)
+            st = eq.getState()
+            t_eq = time.perf_counter() - t_eq0
+            T_eq = eq.getTemperature()
+            launch = eq.getLaunchInfo()
+            eq.close()
+            xyz, vel = st["xyz"], st["vel"]
+            equil = {"steps": a.equil_md_steps, "friction_per_ps": EQUIL_FRICTION,
+                     "T_end_K": round(float(T_eq[1]), 2), "wall_s": round(t_eq, 3),
+                     "launch_setting": launch["setting"]}
+        lt = LangevinThermostatIntegrator(DT_PS, friction=FRICTION, temperature=TEMP, seed=SEED)
+        lt.setNonbondedListUpdateFrequency(REBUILD_EVERY)
         lt.setListPruneFrequency(a.prune_every)
-        lt.setCharmmContext(fm, np.asarray(spec["xyz"], np.float64), vel)
+        if launch is not None and launch["setting"] >= 0:
+            lt.setLaunchSetting(launch["setting"])
+        lt.setCharmmContext(fm, xyz, vel)
         lt.initialize()
 
         def run(k):
@@ -512,10 +602,12 @@ def main():
     achieved = pairs * FLOP_PER_PAIR / (pair_ms * 1e-3) / 1e12
     stats = fm.listStats()
     thermo = None
+    list_check = None
     if md:
         th = lt._state(())["thermo"]
         thermo = {"T_K": round(float(th[2]), 2), "T_avg_K": round(float(th[4]), 2),
                   "steps_total": int(lt.getCurrentStep())}
+        list_check = check_list(lt, fm, spec, a.prune_every)
     what = "NVT Langevin + SETTLE step" if md else "force evaluation only (no integrator)"
     workload = "%s_%d_atoms_pme%d" % ("jac_dhfr" if a.config == "jac" else a.config + "_tip3p", n,
                                        int(spec["fft_grid"][0]))
@@ -550,17 +642,23 @@ def main():
         "kernels_ms_avg": {k: round(v[0], 5) for k, v in kern.items()},
         "kernel_launches": {k: v[1] for k, v in kern.items()},
         "list": stats,
+        "list_check": list_check,
         "thermo": thermo,
-        "roofline": {"bound": "mfma", "engine": "fp32 VALU (no dense contraction on this path)",
+        "roofline": {"bound": "valu", "engine": "fp32 VALU (no dense contraction on this path)",
                      "kernel": "k_pair (direct-space pair kernel)", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
+                     "peak_nonpacked": PEAK_FP32_NONPACKED_TFLOPS,
+                     "frac_nonpacked": round(achieved / PEAK_FP32_NONPACKED_TFLOPS, 5),
                      "traffic": pair_traffic(workload),
                      "work": "%d pairs inside r_off x %d flop" % (pairs, FLOP_PER_PAIR)},
     }
+    if md:
+        out["config"]["equilibration"] = equil
     if not a.no_cpu_baseline:
-        threads = min(os.cpu_count() or 1, 16)
+        threads, host = host_cpus()
         out["cpu_baseline"] = cpu_baseline(spec, threads, md)
+        out["cpu_baseline"].update(host)
     else:
         out["cpu_baseline"] = None
     if md:

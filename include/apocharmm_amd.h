@@ -285,6 +285,15 @@ typedef struct apo_lt_params {
                             (the build's own buffer per step, so a pair the
                             build's list would reach is kept); 0 = off.
                             Fused batch steps use the build's list. */
+  int32_t launch_setting; /* no counterpart in the reference: launch order of
+                            the force branches inside captured steps (it
+                            changes no result bit).  0: the first list cycles
+                            time the four settings (1 untimed + 3 timed cycles
+                            each, waiting for the GPU on the host) and keep
+                            the fastest; 1..4: use setting (launch_setting - 1)
+                            without a trial, bit 0 = PME kernels at raised
+                            wave issue priority, bit 1 = pair kernel captured
+                            first. */
 } apo_lt_params;
 
 /* Binds an integrator to `sys` (setCharmmContext) with host coordinates
@@ -368,6 +377,13 @@ APO_API apo_status apo_lt_get_state(apo_lt_integrator *lt, double *xyz, double *
  * half the list buffer (cutnb - ctofnb)/2 (A).  No counterpart in the
  * reference, which rebuilds at a fixed cadence without a check. */
 APO_API apo_status apo_lt_list_info(apo_lt_integrator *lt, double *out4);
+
+/* Launch settings of the integrator's captured steps (no counterpart in the
+ * reference): out6 = chosen setting (0..3, bit 0 PME priority, bit 1 pair
+ * kernel first; -1 while the timing trial is still running), trial cycles
+ * run so far, and the fastest timed cycle of each of the four settings (ms,
+ * 0 if not timed). */
+APO_API apo_status apo_lt_launch_info(apo_lt_integrator *lt, double *out6);
 
 /* Device float xyzq[4*natom] the integrator keeps for the force path
  * (CharmmContext::getXYZQ); valid until apo_lt_destroy. */

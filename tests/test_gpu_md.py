@@ -431,3 +431,89 @@ def test_md_batch_autotune_keeps_trajectories():
             assert np.array_equal(st[k], ref[k]), k
         lt.close()
         fm.close()
+
+
+def test_centering_moves_molecules_like_oracle(oracle):
+    """Image centering where it acts (src/CharmmContext.cu:276-372): the C1
+    lattice starts in [0, L), so most molecule centroids lie beyond +L/2 on
+    some axis.  The first propagate step centres them (k_center) and steps;
+    the oracle centres with orc_center and steps with the gradient the GPU
+    step used.  Coordinates agree to FP64 rounding."""
+    from apocharmm_amd.integrator import maxwell_velocities
+    sp = systems.config1()
+    xyz = np.asarray(sp["xyz"], np.float64)
+    vel = maxwell_velocities(sp["masses"], TEMP, 1002)
+    fm, lt = _make(sp, xyz, vel)
+    lt.initialize()
+    st = lt.getState()
+    ol = _oracle_lt(oracle, sp)
+    x, v = xyz.copy(), vel.copy()
+    ol.init_constrain(x, v)
+    dprev = ol.init_backstep(st["grad"], x, v)
+    assert np.abs(st["xyz"] - x).max() < 1e-11
+    w = np.sort(np.asarray(sp["waters"], np.int32), axis=1)
+    gptr = np.arange(0, 3 * len(w) + 1, 3, dtype=np.int32)
+    L = np.asarray(sp["box"], np.float64)
+    xc = oracle.center(gptr, w.reshape(-1).copy(), L, x.copy())
+    moved = int((np.abs(xc - x) > 0.5 * L.min()).reshape(len(w), -1).any(1).sum())
+    print("\n[centering] %d of %d molecules move" % (moved, len(w)), flush=True)
+    assert moved > len(w) // 2
+    lt.propagate(1)
+    st = lt.getState()
+    ol.remove_com(dprev)
+    ol.step(st["grad"], xc, dprev, v, 0)
+    dx = np.abs(st["xyz"] - xc).max()
+    print("[centering] max |x - x_oracle| = %.2e A" % dx, flush=True)
+    assert dx < 1e-10
+    cent = st["xyz"][w].mean(1)
+    assert np.all(np.abs(cent) <= 0.5 * L + 0.05)
+    lt.close()
+    fm.close()
+
+
+def test_destroy_order_is_enforced():
+    """A force system outlives its integrators and an integrator its batches:
+    destroying them out of order is refused with APO_INVALID_ARGUMENT and
+    leaves every handle usable."""
+    from apocharmm_amd import _lib
+    from apocharmm_amd.integrator import LangevinBatch
+    sp, xyz = _centered_c1()
+    fm, lt = _make(sp, xyz, None)
+    lt.propagate(2)
+    with pytest.raises(_lib.ApoInvalidArgument, match="integrator"):
+        fm.close()
+    b = LangevinBatch([lt])
+    with pytest.raises(_lib.ApoInvalidArgument, match="batch"):
+        lt.close()
+    b.propagate(2)
+    b.close()
+    lt.propagate(2)
+    assert lt.getCurrentStep() == 6
+    lt.close()
+    fm.close()
+
+
+def test_launch_settings_change_no_bit(c1):
+    """The four fixed launch orders of the force branches and the timing trial
+    give the same trajectory bit for bit; the trial reports its choice."""
+    sp, xyz, vel = c1
+    ref = None
+    for setting in (None, 0, 1, 2, 3):
+        fm, lt = _make(sp, xyz, vel)
+        lt.setLaunchSetting(setting)
+        lt.setCharmmContext(fm, xyz, vel)
+        lt.propagate(400)
+        st = lt.getState()
+        info = lt.getLaunchInfo()
+        print("\n[launch %s] %s" % (setting, info), flush=True)
+        if setting is None:
+            assert info["trial_cycles"] == 16 and info["setting"] in (0, 1, 2, 3)
+            assert all(t > 0 for t in info["trial_ms"])
+        else:
+            assert info["setting"] == setting and info["trial_cycles"] == 0
+        if ref is None:
+            ref = st
+        for k in ("xyz", "vel", "dprev", "grad"):
+            assert np.array_equal(st[k], ref[k]), (setting, k)
+        lt.close()
+        fm.close()

@@ -131,11 +131,11 @@ def test_fp64_instantiation_matches_oracle(oracle, name):
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
 @pytest.mark.parametrize("name", golden_util.NAMES)
 def test_gpu_matches_c47b1_golden(name, precision):
-    """The reference's own tolerances against CHARMM c47b1.  One exception for
-    the mixed path: the water dimer's elec energy (0.122 kcal/mol) is a sum of
-    9 pair terms of ~30 kcal/mol, below the resolution of FP32 pair
-    arithmetic at the reference's 1e-5 margin, so the mixed path is held to
-    2e-5 there; the FP64 instantiation meets 1e-5."""
+    """The reference's own tolerances against CHARMM c47b1
+    (test/unittests/unittest-eneForces.cpp:110-365), for both precisions.  The
+    water dimer's elec energy (0.122 kcal/mol) is a sum of 9 pair terms of
+    ~30 kcal/mol: the mixed path meets the 1e-5 margin because energy steps
+    evaluate pair energies in FP64 from the fixed-point geometry."""
     spec, xyzq, ref = golden_util.load(name)
     spec = dict(spec)
     spec.update(xyz=xyzq[:, :3], masses=np.ones(len(xyzq)), name=name)
@@ -144,8 +144,6 @@ def test_gpu_matches_c47b1_golden(name, precision):
     got = gpu_eval(spec, precision=precision)
     for key, val in ref["energy"].items():
         tol = max(ref["energy_atol"], ref["energy_rtol"] * abs(val))
-        if precision == "mixed" and name == "waterDimer":
-            tol = max(tol, 2e-5)
         assert abs(got["energy"][key] - val) <= tol, (key, got["energy"][key], val)
     assert np.abs(got["grad"] - ref["grad"]).max() <= ref["force_atol"]
     assert np.abs(got["virial"].reshape(-1) - ref["virial"]).max() <= ref["virial_atol"]
@@ -235,10 +233,34 @@ def test_config2_full_size_against_oracle(oracle):
     ref = oracle_eval(oracle, spec)
     err = report("c2 mixed", got, ref)
     assert err <= MIXED_FORCE_RTOL
+    scale = energy_scale(ref)
+    for k in ("bond", "angle", "ewks", "ewse", "ewex", "elec", "vdw", "epot"):
+        assert abs(got["energy"][k] - ref["energy"][k]) <= MIXED_ENERGY_RTOL * scale, k
+    wmax = np.abs(ref["virial"]).max()
+    print("   virial max |dW| / max |W| = %.2e" % (np.abs(got["virial"] - ref["virial"]).max()
+                                                   / wmax), flush=True)
+    assert np.abs(got["virial"] - ref["virial"]).max() <= 1e-5 * wmax
     # size-independent invariant: the pair, exclusion and bonded gradients sum
     # to zero; the PME gradient sum is the small discretisation residue
     g = got["grad_fixed"].astype(np.float64) / 2.0 ** 40
     assert np.abs(g.sum(0)).max() < 1e-2 * np.sqrt(len(g))
+
+
+def test_config3_full_size_against_oracle(oracle):
+    """The 23,558-atom solvated chain (2,489-atom chain with 2,491 LJ types,
+    bonds, angles, dihedrals and 1-4 pairs, in 7,023 waters, 64^3 grid) at
+    step 0: every force term and energy against the oracle."""
+    spec = systems.config3()
+    print("case c3 %d atoms, %d LJ types" % (len(spec["charges"]), len(spec["eps"])), flush=True)
+    got = gpu_eval(spec)
+    ref = oracle_eval(oracle, spec)
+    err = report("c3 mixed", got, ref)
+    assert err <= MIXED_FORCE_RTOL
+    scale = energy_scale(ref)
+    for k in ("bond", "angle", "dihe", "ewks", "ewse", "ewex", "elec", "vdw", "epot"):
+        assert abs(got["energy"][k] - ref["energy"][k]) <= MIXED_ENERGY_RTOL * scale, k
+    wmax = np.abs(ref["virial"]).max()
+    assert np.abs(got["virial"] - ref["virial"]).max() <= 1e-5 * wmax
 
 
 def test_calc_force_rejects_unknown_flag_bits():

@@ -189,6 +189,62 @@ def test_dihedral_sign_convention_is_iupac(oracle):
         assert abs(e[3] - k * (1 + math.cos(n * phi - delta))) < 1e-12
 
 
+def _iupac_phi(p):
+    b1, b2, b3 = p[1] - p[0], p[2] - p[1], p[3] - p[2]
+    return math.atan2(np.linalg.norm(b2) * b1 @ np.cross(b2, b3),
+                      np.cross(b1, b2) @ np.cross(b2, b3))
+
+
+def _oracle_improper(oracle, p, par):
+    import ctypes
+    xyzq = np.concatenate([p, np.zeros((4, 1))], 1).reshape(-1).copy()
+    e, g, v = np.zeros(5), np.zeros(12), np.zeros(9)
+    P = lambda a: np.ascontiguousarray(a).ctypes.data_as(ctypes.c_void_p)
+    ii = np.ascontiguousarray([[0, 1, 2, 3]], np.int32)
+    ip = np.ascontiguousarray([par], np.float64)
+    oracle.lib().orc_bonded(xyzq, 0, None, None, 0, None, None, 0, None, None, 0, None, None, 1,
+                            P(ii), P(ip), g, e, v)
+    return e, g.reshape(4, 3)
+
+
+def test_improper_energy_closed_form(oracle):
+    """Impropers (src/CudaBondedForce.cu:444-513): multiplicity 0 is the
+    harmonic k (phi - phi0)^2 with phi - phi0 taken in (-pi, pi]; n > 0 is
+    k (1 + cos(n phi - delta)).  phi is the IUPAC angle computed here
+    independently, and the gradient must be dE/dphi times dphi/dx, with
+    dE/dphi = 2 k (phi - phi0) (harmonic) or -n k sin(n phi - delta) and
+    dphi/dx from central differences of that angle.  A factor of 1/2 in the
+    energy, a sign of phi or a wrap error fails one of the two checks."""
+    rng = np.random.default_rng(17)
+    for trial in range(12):
+        p = rng.normal(0, 1.5, (4, 3))
+        phi = _iupac_phi(p)
+        dphi = np.zeros((4, 3))
+        h = 1e-6
+        for a in range(4):
+            for c in range(3):
+                q = p.copy()
+                q[a, c] += h
+                fp = _iupac_phi(q)
+                q[a, c] -= 2 * h
+                fm = _iupac_phi(q)
+                dphi[a, c] = (fp - fm) / (2 * h)
+        k = 1.0 + trial * 0.37
+        # harmonic: reference angles either side of phi, including a wrap
+        # across +-pi
+        phi0 = phi + (2.9 if trial % 3 == 0 else (-0.4 if trial % 3 == 1 else 0.0))
+        e, g = _oracle_improper(oracle, p, [0.0, k, phi0])
+        dp = math.remainder(phi - phi0, 2 * math.pi)
+        assert abs(e[4] - k * dp * dp) < 1e-12 * max(1.0, k * dp * dp), (trial, e[4], k * dp * dp)
+        np.testing.assert_allclose(g, 2 * k * dp * dphi, rtol=0, atol=1e-7 * max(1.0, 2 * k))
+        # periodic improper
+        n, delta = 1 + trial % 3, 0.3 * trial
+        e, g = _oracle_improper(oracle, p, [float(n), k, delta])
+        assert abs(e[4] - k * (1 + math.cos(n * phi - delta))) < 1e-12
+        np.testing.assert_allclose(g, -n * k * math.sin(n * phi - delta) * dphi, rtol=0,
+                                   atol=1e-7 * max(1.0, n * k))
+
+
 def test_pairs14_equal_direct_pair_when_e14fac_is_one(oracle):
     """With e14fac = 1 a 1-4 pair carries exactly the direct-space pair
     interaction (erfc Coulomb + switched LJ) that the exclusion removed."""
