@@ -15,6 +15,7 @@ APO_OK, APO_INVALID_ARGUMENT, APO_RUNTIME_ERROR, APO_GPU_ERROR, APO_NOT_INITIALI
 APO_CALC_ENERGY = 1 << 0
 APO_CALC_VIRIAL = 1 << 1
 APO_RESET_LIST = 1 << 2
+APO_ACCUMULATE = 1 << 3
 APO_TERM_BONDED = 1 << 8
 APO_TERM_DIRECT = 1 << 9
 APO_TERM_PAIRS14 = 1 << 10
@@ -95,6 +96,8 @@ def lib():
         L.apo_nb_force_stride.argtypes = [vp, _i32p]
         L.apo_nb_num_atoms.argtypes = [vp, _i32p]
         L.apo_nb_force_to_double.argtypes = [vp, i32, i32, vp, vp]
+        L.apo_nb_add_force.argtypes = [vp, i32, i32, vp, vp]
+        L.apo_nb_add_energy.argtypes = [vp, vp, vp]
         L.apo_nb_list_stats.argtypes = [vp, i64p, vp]
         L.apo_topology_exclusions.argtypes = [i32, i32, vp, _i32p, vp, vp, vp, vp]
         L.apo_lt_create.argtypes = [vp, ctypes.POINTER(LtParams), vp, vp, ctypes.POINTER(vp)]
@@ -112,7 +115,8 @@ def lib():
         L.apo_lt_launch_info.argtypes = [vp, dp]
         for name in ("apo_nb_create", "apo_nb_destroy", "apo_nb_set_box", "apo_nb_reset_list",
                      "apo_nb_calc_force", "apo_nb_force_stride", "apo_nb_num_atoms",
-                     "apo_nb_force_to_double", "apo_nb_list_stats", "apo_topology_exclusions",
+                     "apo_nb_force_to_double", "apo_nb_add_force", "apo_nb_add_energy",
+                     "apo_nb_list_stats", "apo_topology_exclusions",
                      "apo_lt_create", "apo_lt_destroy", "apo_lt_initialize", "apo_lt_propagate",
                      "apo_lt_propagate_batch", "apo_lt_batch_create", "apo_lt_batch_propagate",
                      "apo_lt_batch_destroy", "apo_lt_batch_autotune",

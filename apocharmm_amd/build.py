@@ -17,6 +17,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libapocharmm_amd.so")
 OBJ = os.path.join(HERE, "_build")
+CPP_CALLER = os.path.join(OBJ, "fm_plugin")
 
 SOURCES = ["system.hip", "util.hip", "nblist.hip", "pair.hip", "bonded.hip", "pme.hip", "md.hip", "capi.cpp"]
 
@@ -50,18 +51,17 @@ def build(verbose=False, jobs=None):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s" % r.stderr[-4000:])
     os.replace(tmp, OUT)
-    # the C++ ForceManager mirror must compile against the C ABI header
-    probe = os.path.join(OBJ, "fm_header_check.cpp")
-    with open(probe, "w") as f:
-        f.write('#include "apocharmm_amd/ForceManager.hpp"\n'
-                'int fm_header_check() { apocharmm_amd::Topology t; '
-                'apocharmm_amd::ForceManager fm(t); fm.setKappa(0.34); return fm.getForceStride(); }\n')
-    r = subprocess.run([HIPCC] + FLAGS + ["-x", "hip", "-fsyntax-only", probe], capture_output=True,
-                       text=True)
+    # a C++ caller of the library through the ForceManager mirror, with a
+    # plugin force (run by the GPU tests)
+    cmd = [HIPCC, "-O2", "-std=c++17", "--offload-arch=" + ARCH, "-Xclang", "-target-feature",
+           "-Xclang", "-packed-fp32-ops", "-I" + os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "cpp", "fm_plugin.cpp"), "-L" + HERE, "-lapocharmm_amd",
+           "-Wl,-rpath,$ORIGIN/..", "-o", CPP_CALLER]
+    r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError("ForceManager.hpp does not compile:\n%s" % r.stderr[-4000:])
+        raise RuntimeError("C++ caller (ForceManager.hpp) does not build:\n%s" % r.stderr[-4000:])
     if verbose:
-        print("built", OUT)
+        print("built", OUT, "and", CPP_CALLER)
     return OUT
 
 

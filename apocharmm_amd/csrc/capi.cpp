@@ -133,10 +133,29 @@ APO_API apo_status apo_nb_calc_force(apo_nb_system *sys, const float *d_xyzq, in
                                      int64_t *d_force, int32_t stride, double *d_energy,
                                      double *d_virial, void *stream) {
   return guard([&] {
-    constexpr int32_t known = APO_CALC_ENERGY | APO_CALC_VIRIAL | APO_RESET_LIST | APO_TERM_ALL;
+    constexpr int32_t known =
+        APO_CALC_ENERGY | APO_CALC_VIRIAL | APO_RESET_LIST | APO_ACCUMULATE | APO_TERM_ALL;
     if (flags & ~known) throw apo::InvalidArgument("unknown calc_force flag bits");
     apo::calc_force(get(sys), d_xyzq, flags, d_force, stride, d_energy, d_virial,
                     reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+APO_API apo_status apo_nb_add_force(const int64_t *d_src, int32_t natom, int32_t stride,
+                                    int64_t *d_dst, void *stream) {
+  return guard([&] {
+    if (!d_src || !d_dst || natom < 0 || stride < natom)
+      throw apo::InvalidArgument("bad force addition arguments");
+    if (natom)
+      apo::launch_add_force(d_src, natom, stride, d_dst, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+APO_API apo_status apo_nb_add_energy(const double *d_plugin_energy, double *d_energy,
+                                     void *stream) {
+  return guard([&] {
+    if (!d_plugin_energy || !d_energy) throw apo::InvalidArgument("null energy pointer");
+    apo::launch_add_energy(d_plugin_energy, d_energy, reinterpret_cast<hipStream_t>(stream));
   });
 }
 

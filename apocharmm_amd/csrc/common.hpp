@@ -237,6 +237,8 @@ void launch_scan(const int *in, int *out, const int *n_dev, int n_max,
                  hipStream_t s);
 void launch_force_to_double(const int64_t *f, int natom, int stride,
                             double *out, hipStream_t s);
+void launch_add_force(const int64_t *src, int natom, int stride, int64_t *dst, hipStream_t s);
+void launch_add_energy(const double *src, double *energy, hipStream_t s);
 
 void nblist_build(NbSystem &sys, const float4 *d_xyzq, hipStream_t s, bool sizing);
 void list_export_check(NbSystem &sys, hipStream_t s, bool allow_growth);
@@ -270,7 +272,7 @@ void recip_launch(NbSystem &sys, const float4 *d_xyzq, int64_t *d_force,
 void self_launch(NbSystem &sys, const float4 *d_xyzq, hipStream_t s);
 void reduce_energy_launch(NbSystem &sys, int term_mask, bool energy,
                           bool virial, double *d_energy, double *d_virial,
-                          hipStream_t s);
+                          hipStream_t s, bool accumulate = false);
 
 // Host topology helper shared by the C ABI.
 void build_exclusions(int natom, int nbond, const int32_t *bonds,

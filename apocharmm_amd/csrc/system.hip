@@ -456,9 +456,10 @@ void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int s
   hipEvent_t t_all = nullptr, t0 = nullptr, t1 = nullptr;
   S.tic(5, s, t_all);
   const bool dir = terms & APO_TERM_DIRECT, rec = terms & APO_TERM_RECIP;
-  // the coordinate sort zeroes `force` when there is one; every term adds
-  // into it after the sort
-  if (!(dir || rec)) launch_fill_i64(force, 3 * (size_t)stride, s);
+  // the coordinate sort zeroes `force` (unless the call accumulates); every
+  // term adds into it after the sort
+  const bool acc = flags & APO_ACCUMULATE;
+  if (!(dir || rec) && !acc) launch_fill_i64(force, 3 * (size_t)stride, s);
   const bool bnd = terms & (APO_TERM_PAIRS14 | APO_TERM_BONDED);
   // Three branches that meet before the sorted pair gradient is folded into
   // `force`:  bonded + 1-4 (side[1], int64 atomics into force) | pair kernel
@@ -476,7 +477,7 @@ void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int s
       S.toc(4, s, t0);
     }
     S.tic(7, s, t0);
-    nblist_sort_coords(S, x4, s, force, 3 * (size_t)stride);
+    nblist_sort_coords(S, x4, s, acc ? nullptr : force, acc ? 0 : 3 * (size_t)stride);
     S.toc(7, s, t0);
   }
   // Both side branches fork right after the sort.  The capture order of the
@@ -521,7 +522,7 @@ void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int s
     S.toc(7, s, t0);
   }
   if ((terms & APO_TERM_SELF) && E) self_launch(S, x4, s);
-  if (E || V) reduce_energy_launch(S, terms, E, V, energy, virial, s);
+  if (E || V) reduce_energy_launch(S, terms, E, V, energy, virial, s, acc);
   S.toc(5, s, t_all);
 }
 

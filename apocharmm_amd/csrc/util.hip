@@ -95,6 +95,25 @@ void launch_force_to_double(const int64_t *f, int natom, int stride, double *out
   APO_LAUNCH_CHECK();
 }
 
+// Plugin gradient into the total (add_force, src/reduce.cu:177-189).
+__global__ void k_add_force(const int64_t *src, int natom, int stride, int64_t *dst) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= natom) return;
+  for (int c = 0; c < 3; c++) dst[(size_t)c * stride + i] += src[(size_t)c * stride + i];
+}
+void launch_add_force(const int64_t *src, int natom, int stride, int64_t *dst, hipStream_t s) {
+  k_add_force<<<(natom + 255) / 256, 256, 0, s>>>(src, natom, stride, dst);
+  APO_LAUNCH_CHECK();
+}
+// Plugin energy term 0 into the potential (src/ForceManager.cu:774-778).
+__global__ void k_add_energy(const double *src, double *energy) {
+  if (threadIdx.x == 0) energy[APO_E_EPOT] += src[0];
+}
+void launch_add_energy(const double *src, double *energy, hipStream_t s) {
+  k_add_energy<<<1, 64, 0, s>>>(src, energy);
+  APO_LAUNCH_CHECK();
+}
+
 // Block-level fixed-order double reduction (tree over LDS).
 template <int NT>
 __device__ double block_sum(double v, double *sh) {
@@ -130,6 +149,7 @@ struct ReduceArgs {
   PartLayout L;
   int terms;
   int E, V;
+  int acc;  // add into energy / virial instead of overwriting
   double self_fac;
 };
 
@@ -167,7 +187,7 @@ __global__ __launch_bounds__(256) void k_reduce_energy(const double *parts, Redu
     for (int k = 0; k < APO_E_EPOT; k++) tot += e[k];
     e[APO_E_EPOT] = tot;
     if (threadIdx.x == 0)
-      for (int k = 0; k < APO_NUM_ENERGY; k++) energy[k] = e[k];
+      for (int k = 0; k < APO_NUM_ENERGY; k++) energy[k] = a.acc ? energy[k] + e[k] : e[k];
   }
   if (a.V) {
     double v6[6] = {0};
@@ -180,26 +200,20 @@ __global__ __launch_bounds__(256) void k_reduce_energy(const double *parts, Redu
       v6[c] = s;
     }
     if (threadIdx.x == 0) {
-      virial[0] = v6[0];
-      virial[1] = v6[1];
-      virial[2] = v6[2];
-      virial[3] = v6[1];
-      virial[4] = v6[3];
-      virial[5] = v6[4];
-      virial[6] = v6[2];
-      virial[7] = v6[4];
-      virial[8] = v6[5];
+      const double w[9] = {v6[0], v6[1], v6[2], v6[1], v6[3], v6[4], v6[2], v6[4], v6[5]};
+      for (int k = 0; k < 9; k++) virial[k] = a.acc ? virial[k] + w[k] : w[k];
     }
   }
 }
 
 void reduce_energy_launch(NbSystem &S, int terms, bool E, bool V, double *energy,
-                          double *virial, hipStream_t s) {
+                          double *virial, hipStream_t s, bool accumulate) {
   ReduceArgs a;
   a.L = S.part;
   a.terms = terms;
   a.E = E;
   a.V = V;
+  a.acc = accumulate ? 1 : 0;
   a.self_fac = -S.kappa * kCcelec / std::sqrt(kPi);
   k_reduce_energy<<<1, 256, 0, s>>>(S.parts.get(), a, energy, virial);
   APO_LAUNCH_CHECK();

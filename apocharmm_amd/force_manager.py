@@ -38,6 +38,7 @@ class ForceManager:
         self._precision = _lib.PRECISION_FP64 if precision in ("fp64", 1) else _lib.PRECISION_MIXED
         self._handle = ctypes.c_void_p()
         self._keep = []
+        self._views = []  # subscribed plugin forces
         self.stream = None
         self.force = None
         self.energy = None
@@ -49,6 +50,8 @@ class ForceManager:
         if self._handle:
             arr = (ctypes.c_double * 3)(*self._box)
             _lib.check(_lib.lib().apo_nb_set_box(self._handle, arr))
+        for force, _ in self._views:
+            force.setBoxDimensions(list(self._box))
 
     def setFFTGrid(self, nx, ny, nz):
         self._grid = [int(nx), int(ny), int(nz)]
@@ -70,6 +73,20 @@ class ForceManager:
 
     def getBoxDimensions(self):
         return list(self._box)
+
+    # ---- plugin forces (ForceManager::subscribe, include/ForceManager.h:427-444) ----
+    def subscribe(self, force, tag=""):
+        """Adds a plugin force (ForceView, include/ForceManager.h:37-105).  It
+        provides initialize(natom, box, stride), clear(),
+        calcForce(xyzq_tensor, calcEnergy, calcVirial), setBoxDimensions(box),
+        getForce() -> int64 device tensor [3 * stride] (2^-40 gradient) and
+        getEnergy() -> float64 device tensor [1], and works on the current
+        stream.  After every calcForce its gradient and energy are added to
+        the totals (src/ForceManager.cu:678-682, 774-778); like the
+        reference, the total virial leaves it out."""
+        self._views.append((force, tag))
+        if self._handle:
+            force.initialize(self.natom, list(self._box), self.stride)
 
     # ---- set-up --------------------------------------------------------------
     def initialize(self, stream=None):
@@ -123,6 +140,8 @@ class ForceManager:
         self.virial = torch.zeros(9, dtype=torch.float64, device=dev)
         self._fdouble = torch.zeros(3 * self.stride, dtype=torch.float64, device=dev)
         self.stream = stream
+        for force, _ in self._views:
+            force.initialize(self.natom, list(self._box), self.stride)
         return self
 
     def _stream_ptr(self):
@@ -144,6 +163,8 @@ class ForceManager:
         Overwrites self.force with the int64 2^-40 gradient; energies/virial
         when requested."""
         if isinstance(xyzq, int):
+            if self._views:
+                raise ValueError("plugin forces need the xyzq tensor, not its address")
             ptr = xyzq
         else:
             assert xyzq.dtype == _torch().float32 and xyzq.is_cuda and xyzq.is_contiguous()
@@ -155,11 +176,22 @@ class ForceManager:
             flags |= _lib.APO_CALC_ENERGY
         if calcVirial:
             flags |= _lib.APO_CALC_VIRIAL
+        for force, _ in self._views:
+            force.clear()
+            force.calcForce(xyzq, calcEnergy, calcVirial)
         _lib.check(_lib.lib().apo_nb_calc_force(
             self._handle, ctypes.c_void_p(ptr), flags,
             ctypes.c_void_p(self.force.data_ptr()), self.stride,
             ctypes.c_void_p(self.energy.data_ptr()), ctypes.c_void_p(self.virial.data_ptr()),
             self._stream_ptr()))
+        for force, _ in self._views:
+            _lib.check(_lib.lib().apo_nb_add_force(
+                ctypes.c_void_p(force.getForce().data_ptr()), self.natom, self.stride,
+                ctypes.c_void_p(self.force.data_ptr()), self._stream_ptr()))
+            if calcEnergy:
+                _lib.check(_lib.lib().apo_nb_add_energy(
+                    ctypes.c_void_p(force.getEnergy().data_ptr()),
+                    ctypes.c_void_p(self.energy.data_ptr()), self._stream_ptr()))
 
     def getForces(self):
         """Gradient as a (natom, 3) float64 device tensor (Force<double>)."""

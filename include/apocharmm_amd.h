@@ -70,6 +70,11 @@ enum {
   APO_CALC_ENERGY = 1 << 0,
   APO_CALC_VIRIAL = 1 << 1,
   APO_RESET_LIST = 1 << 2,
+  /* add into d_force, d_energy and d_virial instead of overwriting them, so
+   * a caller's own forces (plugins, restraints) computed into the same
+   * buffers beforehand are kept (no counterpart flag in the reference, whose
+   * manager sums plugin buffers afterwards: apo_nb_add_force) */
+  APO_ACCUMULATE = 1 << 3,
   APO_TERM_BONDED = 1 << 8,
   APO_TERM_DIRECT = 1 << 9,
   APO_TERM_PAIRS14 = 1 << 10,
@@ -187,6 +192,21 @@ APO_API apo_status apo_nb_calc_force(apo_nb_system *sys, const float *d_xyzq,
                                      int32_t flags, int64_t *d_force,
                                      int32_t stride, double *d_energy,
                                      double *d_virial, void *stream);
+
+/* Plugin forces (ForceManager::subscribe / ForceView, include/ForceManager.h:
+ * 37-105, 427-444): the manager adds each subscribed force's int64 gradient
+ * into the total, d_dst[k] += d_src[k] for the three strided rows of natom
+ * entries (Force<long long>::add / add_force, src/reduce.cu:177-189, as
+ * calcForcePart3 does at src/ForceManager.cu:678-682).  Enqueued on stream. */
+APO_API apo_status apo_nb_add_force(const int64_t *d_src, int32_t natom,
+                                    int32_t stride, int64_t *d_dst,
+                                    void *stream);
+
+/* ... and a plugin's energy (its term 0) into the total potential:
+ * d_energy[APO_E_EPOT] += d_plugin_energy[0] (UpdatePotentialEnergyKernel2,
+ * src/ForceManager.cu:774-778).  Enqueued on stream. */
+APO_API apo_status apo_nb_add_energy(const double *d_plugin_energy,
+                                     double *d_energy, void *stream);
 
 /* ForceManager::getForceStride: the smallest valid stride for this system,
  * ceil((natom+31)*8/256)*256/8 (src/Force.cu:344-352). */

@@ -4,8 +4,8 @@
 // Method names, argument meaning and error behaviour follow the reference:
 // setters before initialize(), calcForce(xyzq, reset, calcEnergy, calcVirial)
 // with device xyzq, forces as the int64 2^-40 gradient (Force<long long>),
-// std::invalid_argument / std::runtime_error on failure.  Header-only; link
-// against libapocharmm_amd.so.
+// std::invalid_argument / std::runtime_error on failure, plugin forces
+// through subscribe().  Header-only; link against libapocharmm_amd.so.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -39,6 +39,52 @@ struct Topology {
   std::vector<float> bond_par, ub_par, angle_par, dihe_par, imdihe_par;
 };
 
+// Type-erased plugin force (ForceView, include/ForceManager.h:37-105).  A
+// plugin type F provides
+//   void initialize(int numAtoms, const std::vector<double> &box);
+//   void clear();
+//   void calcForce(const float4 *xyzq, bool calcEnergy, bool calcVirial);
+//   void setBoxDimensions(const std::vector<double> &box);
+//   int64_t *getForce();         // device int64 2^-40 gradient, laid out with
+//                                // the manager's getForceStride()
+//   double *getEnergy();         // device double[1]: the plugin's energy
+// and enqueues its work on the manager's stream (getStream(), e.g. handed to
+// the plugin when it is constructed), so the manager's sums are ordered after
+// it without a host synchronisation.
+class ForceView {
+ public:
+  template <typename F>
+  explicit ForceView(std::shared_ptr<F> f)
+      : obj_(f),
+        initialize_([](void *p, int n, const std::vector<double> &b) {
+          static_cast<F *>(p)->initialize(n, b);
+        }),
+        clear_([](void *p) { static_cast<F *>(p)->clear(); }),
+        calc_([](void *p, const float4 *x, bool e, bool v) {
+          static_cast<F *>(p)->calcForce(x, e, v);
+        }),
+        box_([](void *p, const std::vector<double> &b) {
+          static_cast<F *>(p)->setBoxDimensions(b);
+        }),
+        force_([](void *p) -> int64_t * { return static_cast<F *>(p)->getForce(); }),
+        energy_([](void *p) -> double * { return static_cast<F *>(p)->getEnergy(); }) {}
+  void initialize(int n, const std::vector<double> &box) { initialize_(obj_.get(), n, box); }
+  void clear() { clear_(obj_.get()); }
+  void calcForce(const float4 *x, bool e, bool v) { calc_(obj_.get(), x, e, v); }
+  void setBoxDimensions(const std::vector<double> &box) { box_(obj_.get(), box); }
+  int64_t *getForce() { return force_(obj_.get()); }
+  double *getEnergy() { return energy_(obj_.get()); }
+
+ private:
+  std::shared_ptr<void> obj_;
+  void (*initialize_)(void *, int, const std::vector<double> &);
+  void (*clear_)(void *);
+  void (*calc_)(void *, const float4 *, bool, bool);
+  void (*box_)(void *, const std::vector<double> &);
+  int64_t *(*force_)(void *);
+  double *(*energy_)(void *);
+};
+
 class ForceManager {
  public:
   explicit ForceManager(Topology topo) : topo_(std::move(topo)) {
@@ -58,6 +104,7 @@ class ForceManager {
   void setBoxDimensions(const std::vector<double> &box) {
     for (int d = 0; d < 3; d++) params_.box[d] = box.at(d);
     if (sys_) check(apo_nb_set_box(sys_, params_.box));
+    for (auto &v : views_) v.setBoxDimensions(getBoxDimensions());
   }
   std::vector<double> getBoxDimensions() const {
     return {params_.box[0], params_.box[1], params_.box[2]};
@@ -105,7 +152,22 @@ class ForceManager {
     alloc(reinterpret_cast<void **>(&force_d_), sizeof(double) * 3 * (size_t)stride_);
     alloc(reinterpret_cast<void **>(&energy_), sizeof(double) * APO_NUM_ENERGY);
     alloc(reinterpret_cast<void **>(&virial_), sizeof(double) * 9);
+    for (auto &v : views_) v.initialize(topo_.natom, getBoxDimensions());
   }
+
+  // ForceManager::subscribe (include/ForceManager.h:427-444): the plugin's
+  // gradient and energy are added to the totals after every calcForce
+  // (src/ForceManager.cu:678-682, 774-778).  Plugins subscribed before
+  // initialize() are initialised with it.
+  template <typename F>
+  void subscribe(std::shared_ptr<F> force, const std::string &tag = "") {
+    if (!force) throw std::invalid_argument("subscribe: null force");
+    views_.emplace_back(std::move(force));
+    tags_.push_back(tag);
+    if (sys_) views_.back().initialize(topo_.natom, getBoxDimensions());
+  }
+  size_t getNumSubscribedForces() const { return views_.size(); }
+  hipStream_t getStream() const { return stream_; }
 
   void resetNeighborList(const float4 *xyzq) {
     check(apo_nb_reset_list(sys_, reinterpret_cast<const float *>(xyzq), stream_));
@@ -116,11 +178,24 @@ class ForceManager {
                  bool calcVirial = false) {
     int32_t flags = (reset ? APO_RESET_LIST : 0) | (calcEnergy ? APO_CALC_ENERGY : 0) |
                     (calcVirial ? APO_CALC_VIRIAL : 0);
+    for (auto &v : views_) {
+      v.clear();
+      v.calcForce(xyzq, calcEnergy, calcVirial);
+    }
     check(apo_nb_calc_force(sys_, reinterpret_cast<const float *>(xyzq), flags, force_, stride_,
                             energy_, virial_, stream_));
+    for (auto &v : views_) {
+      check(apo_nb_add_force(v.getForce(), topo_.natom, stride_, force_, stream_));
+      if (calcEnergy) check(apo_nb_add_energy(v.getEnergy(), energy_, stream_));
+    }
   }
 
   int getForceStride() const { return stride_; }
+  int getNumAtoms() const { return topo_.natom; }
+  // The C-ABI handle of the force path (for apo_nb_* calls of its own).
+  apo_nb_system *handle() const { return sys_; }
+  double *getEnergyBuffer() const { return energy_; }
+  double *getVirialBuffer() const { return virial_; }
   // Device int64 gradient (Force<long long>).
   int64_t *getForcesFixed() const { return force_; }
   // Device double gradient, SoA with getForceStride() (Force<double>).
@@ -166,6 +241,8 @@ class ForceManager {
   }
 
   Topology topo_;
+  std::vector<ForceView> views_;
+  std::vector<std::string> tags_;
   apo_nb_params params_{};
   apo_nb_system *sys_ = nullptr;
   hipStream_t stream_ = nullptr;

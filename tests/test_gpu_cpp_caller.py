@@ -1,0 +1,87 @@
+"""The drop-in boundary from C++: a program compiled against
+include/apocharmm_amd/ForceManager.hpp and linked to libapocharmm_amd.so
+(tests/cpp/fm_plugin.cpp, built by apocharmm_amd/build.py) evaluates a water
+box with and without a subscribed harmonic-restraint plugin and checks, bit for
+bit, that the manager adds the plugin's gradient and energy to the totals as the
+reference's calcForcePart3 does (src/ForceManager.cu:678-682, 774-778), and
+that APO_ACCUMULATE adds the core terms onto a caller's buffer."""
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "apocharmm_amd", "_build", "fm_plugin")
+
+
+def test_cpp_caller_with_plugin_force():
+    assert os.path.exists(EXE), "build the library first (python -m apocharmm_amd.build)"
+    print("\n[cpp] running %s" % EXE, flush=True)
+    r = subprocess.run([EXE], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "all checks passed" in r.stdout
+
+
+def test_python_manager_with_plugin_force():
+    """The Python mirror's subscribe(): a restraint computed with torch on the
+    manager's stream is summed into the total gradient and epot."""
+    import numpy as np
+    import torch
+    from apocharmm_amd import systems
+    from apocharmm_amd.force_manager import ForceManager
+
+    class Restraint:
+        def __init__(self, ref, k):
+            self.ref = torch.tensor(ref, device="cuda", dtype=torch.float64)
+            self.k = k
+
+        def initialize(self, natom, box, stride):
+            self.n, self.stride = natom, stride
+            self.force = torch.zeros(3 * stride, dtype=torch.int64, device="cuda")
+            self.energy = torch.zeros(1, dtype=torch.float64, device="cuda")
+
+        def clear(self):
+            self.force.zero_()
+            self.energy.zero_()
+
+        def calcForce(self, xyzq, calcEnergy, calcVirial):
+            d = xyzq[:, :3].double() - self.ref
+            g = torch.round(2.0 * self.k * d * 2.0 ** 40).long()
+            self.force.view(3, self.stride)[:, : self.n] = g.t()
+            self.energy[0] = self.k * (d * d).sum()
+
+        def setBoxDimensions(self, box):
+            pass
+
+        def getForce(self):
+            return self.force
+
+        def getEnergy(self):
+            return self.energy
+
+    sp = systems.config1()
+    x = torch.tensor(systems.xyzq_of(sp), device="cuda")
+    core = ForceManager(sp).initialize()
+    core.calcForce(x, reset=True, calcEnergy=True)
+    torch.cuda.synchronize()
+    g0 = core.getForcesFixed().cpu().numpy().copy()
+    e0 = core.getEnergyComponents()
+    fm = ForceManager(sp)
+    plug = Restraint(np.asarray(sp["xyz"]) + 0.25, 3.0)
+    fm.subscribe(plug, "harm")
+    fm.initialize()
+    fm.calcForce(x, reset=True, calcEnergy=True)
+    torch.cuda.synchronize()
+    g1 = fm.getForcesFixed().cpu().numpy()
+    e1 = fm.getEnergyComponents()
+    gp = plug.getForce().view(3, fm.stride)[:, : fm.natom].t().cpu().numpy()
+    ep = float(plug.getEnergy().item())
+    print("\n[py plugin] epot %.6f + %.6f = %.6f" % (e0["epot"], ep, e1["epot"]), flush=True)
+    assert np.array_equal(g1, g0 + gp)
+    assert e1["epot"] == e0["epot"] + ep
+    assert all(e1[k] == e0[k] for k in e0 if k != "epot")
+    fm.close()
+    core.close()

@@ -271,7 +271,7 @@ def test_calc_force_rejects_unknown_flag_bits():
     fm = ForceManager(sp).initialize()
     x = torch.tensor(systems.xyzq_of(sp), device="cuda")
     fm.calcForce(x, reset=True)
-    for bit in (3, 7, 16, 26, 27, 28, 30):
+    for bit in (4, 7, 16, 26, 27, 28, 30):
         with pytest.raises(_lib.ApoInvalidArgument):
             fm.calcForce(x, terms=1 << bit)
     fm.calcForce(x)  # the system stays usable
