@@ -71,7 +71,8 @@ class LtParams(ctypes.Structure):
                 ("waters", _i32p), ("water_geom", ctypes.c_double * 4),
                 ("mass", ctypes.POINTER(ctypes.c_double)), ("charge", _f32p),
                 ("list_policy", ctypes.c_int32), ("prune_every", ctypes.c_int32),
-                ("launch_setting", ctypes.c_int32)]
+                ("launch_setting", ctypes.c_int32), ("nshake", ctypes.c_int32),
+                ("shake_groups", _i32p), ("shake_r2", ctypes.POINTER(ctypes.c_double))]
 
 
 _LIB = None

@@ -19,7 +19,8 @@ OUT = os.path.join(HERE, "libapocharmm_amd.so")
 OBJ = os.path.join(HERE, "_build")
 CPP_CALLER = os.path.join(OBJ, "fm_plugin")
 
-SOURCES = ["system.hip", "util.hip", "nblist.hip", "pair.hip", "bonded.hip", "pme.hip", "md.hip", "capi.cpp"]
+SOURCES = ["system.hip", "util.hip", "nblist.hip", "pair.hip", "bonded.hip", "pme.hip", "md.hip",
+           "capi.cpp", "charmm_files.cpp", "capi_charmm.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

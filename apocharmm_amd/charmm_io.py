@@ -360,13 +360,44 @@ class CharmmParameters:
                     vdwtab14=tab14.astype(np.float32).reshape(-1))
 
 
+def shake_groups(psf, prm):
+    """SHAKE groups of bonds to hydrogen, as
+    ForceManager::initializeHolonomicConstraintsVariables builds them
+    (src/ForceManager.cu:844-918): every bond with an atom whose type starts
+    with H, except water O-H (OT-HT) and H-H bonds, grouped by its heavy atom
+    in bond order; a group holds the heavy atom and up to three hydrogens and
+    one squared length, the b0 of its last hydrogen bond.  Returns
+    ((k, 4) int32 with -1 padding, (k,) float64)."""
+    t = psf.getAtomTypes()
+    hyd = {}
+    for i, j in psf.getBonds():
+        i, j = int(i), int(j)
+        hi, hj = t[i][0] == "H", t[j][0] == "H"
+        if not (hi or hj):
+            continue
+        if {t[i], t[j]} == {"OT", "HT"} or (hi and hj):
+            continue
+        heavy, h = (j, i) if hi else (i, j)
+        hyd.setdefault(heavy, []).append(h)
+    groups, r2 = [], []
+    for heavy in sorted(hyd):
+        hs = hyd[heavy]
+        if len(hs) > 3:
+            raise ValueError("atom %d carries more than three hydrogens" % (heavy + 1))
+        groups.append([heavy] + hs + [-1] * (3 - len(hs)))
+        key = tuple(sorted((t[heavy], t[hs[-1]])))
+        r2.append(prm.bonds[key][0] ** 2)
+    return (np.array(groups, np.int32).reshape(-1, 4), np.array(r2, np.float64))
+
+
 def charmm_system(psf, prm, xyz, box, fft_grid, kappa=0.34, cutnb=14.0, ctofnb=12.0,
                   ctonnb=10.0, e14fac=1.0, order=4, name=None):
     """System spec (the input of ForceManager, the integrator and the oracle)
     from a PSF, its parameters and coordinates.  xyz: (n, 3) array or a
     CharmmCrd; box: three edge lengths in A; fft_grid: int or three ints.
-    Rigid waters are the PSF's (OT, HT, HT) triples; cross terms (CMAP) are
-    not evaluated, as in the reference (src/CharmmParameters.cu:710)."""
+    Rigid waters are the PSF's (OT, HT, HT) triples and the other bonds to
+    hydrogen form SHAKE groups (shake_groups, shake_r2); cross terms (CMAP)
+    are not evaluated, as in the reference (src/CharmmParameters.cu:710)."""
     if isinstance(psf, str):
         psf = CharmmPSF(psf)
     if not isinstance(prm, CharmmParameters):
@@ -387,4 +418,5 @@ def charmm_system(psf, prm, xyz, box, fft_grid, kappa=0.34, cutnb=14.0, ctofnb=1
                 name=name or psf.fileName)
     spec.update(prm.getVdwParamsAndTypes(psf))
     spec.update(prm.getBondedParamsAndLists(psf))
+    spec["shake_groups"], spec["shake_r2"] = shake_groups(psf, prm)
     return spec

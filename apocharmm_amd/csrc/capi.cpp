@@ -7,6 +7,7 @@
 #include <string>
 
 #include "apocharmm_amd.h"
+#include "capi_guard.hpp"
 #include "system.hpp"
 
 namespace apo {
@@ -35,10 +36,6 @@ void lt_launch_info(LtSystem &L, double *out6);
 void batch_autotune(LtBatch &B, hipStream_t caller, double *report9);
 }  // namespace apo
 
-struct apo_nb_system {
-  apo::NbSystem *sys;
-};
-
 struct apo_lt_integrator {
   apo::LtSystem *lt;
 };
@@ -47,31 +44,13 @@ struct apo_lt_batch {
   apo::LtBatch *b;
 };
 
-namespace {
+namespace apo {
 thread_local std::string g_last_error;
+void capi_set_error(const std::string &msg) { g_last_error = msg; }
+}  // namespace apo
 
-template <typename F> apo_status guard(F &&f) {
-  try {
-    f();
-    g_last_error.clear();
-    return APO_OK;
-  } catch (const apo::InvalidArgument &e) {
-    g_last_error = e.what();
-    return APO_INVALID_ARGUMENT;
-  } catch (const apo::GpuError &e) {
-    g_last_error = e.what();
-    return APO_GPU_ERROR;
-  } catch (const std::invalid_argument &e) {
-    g_last_error = e.what();
-    return APO_INVALID_ARGUMENT;
-  } catch (const std::exception &e) {
-    g_last_error = e.what();
-    return APO_RUNTIME_ERROR;
-  } catch (...) {
-    g_last_error = "unknown error";
-    return APO_RUNTIME_ERROR;
-  }
-}
+namespace {
+using apo::guard;
 
 apo::NbSystem &get(apo_nb_system *h) {
   if (!h || !h->sys) throw apo::InvalidArgument("null system handle");
@@ -89,7 +68,7 @@ apo::LtSystem &get(apo_lt_integrator *h) {
 
 extern "C" {
 
-APO_API const char *apo_last_error(void) { return g_last_error.c_str(); }
+APO_API const char *apo_last_error(void) { return apo::g_last_error.c_str(); }
 
 APO_API const char *apo_version(void) { return "apocharmm_amd 0.1 (gfx950)"; }
 
@@ -106,6 +85,9 @@ APO_API apo_status apo_nb_create(const apo_nb_params *params, const apo_nb_topol
 APO_API apo_status apo_nb_destroy(apo_nb_system *sys) {
   return guard([&] {
     if (!sys) return;
+    if (sys->borrowed)
+      throw apo::InvalidArgument("this force system belongs to a ForceManager: destroy the "
+                                 "manager instead");
     if (sys->sys && sys->sys->n_integrators > 0)
       throw apo::InvalidArgument("an integrator is still bound to this force system: destroy "
                                  "the integrator first");

@@ -163,6 +163,80 @@ __device__ inline void settle(const SettleGeom &G, double3 A0, double3 B0, doubl
   C1 = frame_to_lab(com, ex, ey, ez, -Xb2 * cth - Yc2 * sth, -Xb2 * sth + Yc2 * cth, Zc1);
 }
 
+// SHAKE for one group: a heavy atom (slot 0) and m = 1..3 hydrogens bonded to
+// it, every bond held at squared length r2 (the reference's Shake2/3/4Kernel,
+// src/CudaHolonomicConstraint.cu:324-520, solve the same constraints).  The
+// constraint displacements lie along the reference bonds d_k = ref0 - refk:
+// x0 += im0 sum_k l_k d_k, xk -= imk l_k d_k.  The m equations
+// |x0 - xk|^2 = r2 are solved for l by Newton's method in FP64 (Jacobian
+// J_kj = 2 p_k . (im0 d_j + [k == j] imk d_k), p_k = x0 - xk), to a relative
+// residual of 1e-14 or 30 iterations.  Momentum is conserved by construction.
+__device__ inline void shake_group(int m, double r2, const double im[4], const double3 ref[4],
+                                   double3 x[4]) {
+  double3 d[3];
+  for (int k = 0; k < m; k++) d[k] = ref[0] - ref[k + 1];
+  double lam[3] = {0.0, 0.0, 0.0};
+  const double3 x0 = x[0];
+  double3 xs[4] = {x[0], x[1], x[2], x[3]};
+  for (int it = 0; it < 30; it++) {
+    // current positions from the multipliers
+    double3 c0 = x0;
+    for (int k = 0; k < m; k++) {
+      c0.x += im[0] * lam[k] * d[k].x;
+      c0.y += im[0] * lam[k] * d[k].y;
+      c0.z += im[0] * lam[k] * d[k].z;
+    }
+    double3 p[3];
+    double f[3], worst = 0.0;
+    for (int k = 0; k < m; k++) {
+      const double3 ck = make_double3(x[k + 1].x - im[k + 1] * lam[k] * d[k].x,
+                                      x[k + 1].y - im[k + 1] * lam[k] * d[k].y,
+                                      x[k + 1].z - im[k + 1] * lam[k] * d[k].z);
+      xs[k + 1] = ck;
+      p[k] = c0 - ck;
+      f[k] = dot3(p[k], p[k]) - r2;
+      worst = fmax(worst, fabs(f[k]));
+    }
+    xs[0] = c0;
+    if (worst <= 1e-14 * r2) break;
+    double J[3][3];
+    for (int k = 0; k < m; k++)
+      for (int j = 0; j < m; j++) {
+        double3 t = make_double3(im[0] * d[j].x, im[0] * d[j].y, im[0] * d[j].z);
+        if (k == j) {
+          t.x += im[k + 1] * d[k].x;
+          t.y += im[k + 1] * d[k].y;
+          t.z += im[k + 1] * d[k].z;
+        }
+        J[k][j] = 2.0 * dot3(p[k], t);
+      }
+    // solve J dl = -f (Gaussian elimination with partial pivoting, m <= 3)
+    double b[3] = {-f[0], -f[1], -f[2]};
+    int piv[3] = {0, 1, 2};
+    for (int c = 0; c < m; c++) {
+      int best = c;
+      for (int r = c + 1; r < m; r++)
+        if (fabs(J[piv[r]][c]) > fabs(J[piv[best]][c])) best = r;
+      const int t = piv[c];
+      piv[c] = piv[best];
+      piv[best] = t;
+      for (int r = c + 1; r < m; r++) {
+        const double g = J[piv[r]][c] / J[piv[c]][c];
+        for (int j = c; j < m; j++) J[piv[r]][j] -= g * J[piv[c]][j];
+        b[piv[r]] -= g * b[piv[c]];
+      }
+    }
+    double dl[3] = {0.0, 0.0, 0.0};
+    for (int c = m - 1; c >= 0; c--) {
+      double s = b[piv[c]];
+      for (int j = c + 1; j < m; j++) s -= J[piv[c]][j] * dl[j];
+      dl[c] = s / J[piv[c]][c];
+    }
+    for (int k = 0; k < m; k++) lam[k] += dl[k];
+  }
+  for (int k = 0; k <= m; k++) x[k] = xs[k];
+}
+
 struct LtState {
   double *x, *dp, *v;        // SoA, stride n
   const double *invmass;
@@ -179,7 +253,10 @@ struct LtState {
   int n;
   const int *waters;         // 3 * nwater
   int nwater;
-  const int *singles;        // atoms in no water
+  const int4 *shake;         // nshake SHAKE groups (heavy, h1, h2 / -1, h3 / -1)
+  const double *shake_r2;    // squared bond length per group
+  int nshake;
+  const int *singles;        // atoms in no water and no SHAKE group
   int nsingle;
   const float4 *xref;        // coordinates of the last list build
 };
@@ -292,13 +369,35 @@ __device__ __forceinline__ void lt_step_body(int bid, const LtState &S, const Lt
   const int u = bid * kStepBlock + lane;
   const unsigned long long step = stepc[0];
   const bool water = u < S.nwater;
-  const bool single = !water && k == 0 && u < S.nwater + S.nsingle;
-  const int i = water ? S.waters[3 * u + k] : (single ? S.singles[u - S.nwater] : -1);
+  const int ng = S.nwater + S.nshake;
+  const bool group = !water && k == 0 && u < ng;
+  const bool single = !water && k == 0 && u >= ng && u < ng + S.nsingle;
+  const int i = water ? S.waters[3 * u + k] : (single ? S.singles[u - ng] : -1);
   double3 r = make_double3(0, 0, 0), p = r, xn = r;
   if (i >= 0) {
     r = ld3(S.x, S.n, i);
     p = ld3(S.dp, S.n, i);
     xn = lt_position(S, C, step, i, r, p);
+  }
+  double ke0 = 0.0, ke1 = 0.0;
+  float d2 = 0.f, s2 = 0.f;
+  if (group) {  // a SHAKE group: one lane steps, constrains and finishes its atoms
+    const int4 g = S.shake[u - S.nwater];
+    const int at[4] = {g.x, g.y, g.z, g.w};
+    const int m = g.w >= 0 ? 3 : (g.z >= 0 ? 2 : 1);
+    double im[4] = {0, 0, 0, 0};
+    double3 ref[4], dp4[4], nw[4];
+    for (int a = 0; a <= m; a++) {
+      im[a] = S.invmass[at[a]];
+      ref[a] = ld3(S.x, S.n, at[a]);
+      dp4[a] = ld3(S.dp, S.n, at[a]);
+      nw[a] = lt_position(S, C, step, at[a], ref[a], dp4[a]);
+    }
+    shake_group(m, S.shake_r2[u - S.nwater], im, ref, nw);
+    for (int a = 0; a <= m; a++) {
+      lt_finish(S, C, at[a], ref[a], nw[a], dp4[a], ke0, ke1);
+      disp_track(S, at[a], ref[a], nw[a], d2, s2);
+    }
   }
   if (water) {
     old_x[k][0][lane] = r.x;
@@ -328,8 +427,6 @@ __device__ __forceinline__ void lt_step_body(int bid, const LtState &S, const Lt
     new_x[2][2][lane] = nc.z;
   }
   __syncthreads();
-  double ke0 = 0.0, ke1 = 0.0;
-  float d2 = 0.f, s2 = 0.f;
   if (i >= 0) {
     if (water) xn = make_double3(new_x[k][0][lane], new_x[k][1][lane], new_x[k][2][lane]);
     lt_finish(S, C, i, r, xn, p, ke0, ke1);
@@ -485,8 +582,28 @@ __global__ __launch_bounds__(kStepBlock) void k_lt_init_constrain(LtState S, LtC
       st3(S.x, S.n, idx[k], nn[k]);
       store_xyzq(S, idx[k], nn[k]);
     }
-  } else if (u < S.nwater + S.nsingle) {
-    const int i = S.singles[u - S.nwater];
+  } else if (u < S.nwater + S.nshake) {
+    const int4 g = S.shake[u - S.nwater];
+    const int at[4] = {g.x, g.y, g.z, g.w};
+    const int m = g.w >= 0 ? 3 : (g.z >= 0 ? 2 : 1);
+    double im[4] = {0, 0, 0, 0};
+    double3 o[4], nn[4];
+    for (int a = 0; a <= m; a++) {
+      im[a] = S.invmass[at[a]];
+      o[a] = nn[a] = ld3(S.x, S.n, at[a]);
+    }
+    shake_group(m, S.shake_r2[u - S.nwater], im, o, nn);
+    for (int a = 0; a <= m; a++) {
+      double3 v = ld3(S.v, S.n, at[a]);
+      v.x += (nn[a].x - o[a].x) / C.dt;
+      v.y += (nn[a].y - o[a].y) / C.dt;
+      v.z += (nn[a].z - o[a].z) / C.dt;
+      st3(S.v, S.n, at[a], v);
+      st3(S.x, S.n, at[a], nn[a]);
+      store_xyzq(S, at[a], nn[a]);
+    }
+  } else if (u < S.nwater + S.nshake + S.nsingle) {
+    const int i = S.singles[u - S.nwater - S.nshake];
     store_xyzq(S, i, ld3(S.x, S.n, i));
   }
 }
@@ -512,8 +629,21 @@ __global__ __launch_bounds__(kStepBlock) void k_lt_init_backstep(LtState S, LtCo
     st3(S.dp, S.n, ia, ra - pa);
     st3(S.dp, S.n, ib, rb - pb);
     st3(S.dp, S.n, ic, rc - pc);
-  } else if (u < S.nwater + S.nsingle) {
-    const int i = S.singles[u - S.nwater];
+  } else if (u < S.nwater + S.nshake) {
+    const int4 g = S.shake[u - S.nwater];
+    const int at[4] = {g.x, g.y, g.z, g.w};
+    const int m = g.w >= 0 ? 3 : (g.z >= 0 ? 2 : 1);
+    double im[4] = {0, 0, 0, 0};
+    double3 r[4], pb[4];
+    for (int a = 0; a <= m; a++) {
+      im[a] = S.invmass[at[a]];
+      r[a] = ld3(S.x, S.n, at[a]);
+      pb[a] = back(at[a], r[a]);
+    }
+    shake_group(m, S.shake_r2[u - S.nwater], im, r, pb);
+    for (int a = 0; a <= m; a++) st3(S.dp, S.n, at[a], r[a] - pb[a]);
+  } else if (u < S.nwater + S.nshake + S.nsingle) {
+    const int i = S.singles[u - S.nwater - S.nshake];
     const double3 r = ld3(S.x, S.n, i);
     st3(S.dp, S.n, i, r - back(i, r));
   }
@@ -624,7 +754,9 @@ struct LtSystem {
   int nbfreq = 20, comfreq = 1000;
   bool use_graph = true;
   double ndof = 0;
-  int nwater = 0, nsingle = 0, nunit_blocks = 0;
+  int nwater = 0, nshake = 0, nsingle = 0, nunit_blocks = 0;
+  DevBuf<int4> shake;
+  DevBuf<double> shake_r2;
   DevBuf<double> x, dp, v, invmass;
   DevBuf<float> charge;
   DevBuf<float4> xyzq;
@@ -700,6 +832,9 @@ struct LtSystem {
     s.n = n;
     s.waters = waters.get();
     s.nwater = nwater;
+    s.shake = shake.get();
+    s.shake_r2 = shake_r2.get();
+    s.nshake = nshake;
     s.singles = singles.get();
     s.nsingle = nsingle;
     s.xref = nb->xref.get();
@@ -1071,15 +1206,38 @@ LtSystem *create_lt(NbSystem *nb, const apo_lt_params &p, const double *xyz, con
       if (in_water[a]) throw InvalidArgument("atom listed in two waters");
       in_water[a] = 1;
     }
+    // SHAKE groups: a heavy atom and 1-3 hydrogens, none of them in a water
+    // or in another group
+    if (p.nshake < 0 || (p.nshake > 0 && (!p.shake_groups || !p.shake_r2)))
+      throw InvalidArgument("bad SHAKE group list");
+    std::vector<int4> sg(p.nshake);
+    std::vector<double> sr2(p.shake_r2, p.shake_r2 + (p.nshake > 0 ? p.nshake : 0));
+    int nshake_bonds = 0;
+    for (int g = 0; g < p.nshake; g++) {
+      const int32_t *a = p.shake_groups + 4 * (size_t)g;
+      if (a[1] < 0 || (a[2] < 0 && a[3] >= 0))
+        throw InvalidArgument("a SHAKE group needs its hydrogens in the first slots");
+      for (int k = 0; k < 4; k++) {
+        if (a[k] < 0) continue;
+        if (a[k] >= n) throw InvalidArgument("SHAKE atom index out of range");
+        if (in_water[a[k]]) throw InvalidArgument("atom listed in two constraint groups");
+        in_water[a[k]] = 1;
+        if (k > 0) nshake_bonds++;
+      }
+      if (!(sr2[g] > 0)) throw InvalidArgument("SHAKE bond lengths must be positive");
+      sg[g] = make_int4(a[0], a[1], a[2], a[3]);
+    }
     std::vector<int> singles;
     for (int i = 0; i < n; i++)
       if (!in_water[i]) singles.push_back(i);
     L->nwater = p.nwater;
+    L->nshake = p.nshake;
     L->nsingle = (int)singles.size();
-    L->nunit_blocks = nblk(L->nwater + L->nsingle, kStepBlock);
+    L->nunit_blocks = nblk(L->nwater + L->nshake + L->nsingle, kStepBlock);
     L->com_blocks = nblk(n, kStepBlock);
-    // P1 with SETTLE: 3N - 3 - 3 per rigid water (CharmmContext::useHolonomicConstraints)
-    L->ndof = 3.0 * n - 3.0 - 3.0 * p.nwater;
+    // P1: 3N - 3, less 3 per SETTLE water and 1 per SHAKE bond
+    // (CharmmContext::useHolonomicConstraints, src/CharmmContext.cu:919-945)
+    L->ndof = 3.0 * n - 3.0 - 3.0 * p.nwater - nshake_bonds;
     if (L->ndof <= 0) throw InvalidArgument("no degrees of freedom left");
 
     // centering groups: molecules over bonds and waters
@@ -1122,6 +1280,8 @@ LtSystem *create_lt(NbSystem *nb, const apo_lt_params &p, const double *xyz, con
     launch_fill_i64(L->grad.get(), 3 * (size_t)L->gstride, s);
     L->waters.upload(wl, s);
     L->singles.upload(singles, s);
+    L->shake.upload(sg, s);
+    L->shake_r2.upload(sr2, s);
     L->gptr.upload(gptr, s);
     L->gatom.upload(gatom, s);
     L->ke_part.reserve(L->nunit_blocks);
@@ -1606,6 +1766,43 @@ void lt_get_state(LtSystem &L, double *xyz, double *vel, double *dprev, double *
   if (thermo)
     APO_HIP(hipMemcpy(thermo, L.thermo.get(), 6 * sizeof(double), hipMemcpyDeviceToHost));
   if (step) *step = (int64_t)L.step;
+}
+
+// Restart: coordinates, velocities and previous half-step displacements
+// (atom-major, 3 * n each) replace the state, and the step counter -- the
+// position of the noise stream -- continues from `step`
+// (CudaLangevinThermostatIntegrator::initializeFromRestartFile,
+// src/CudaLangevinThermostatIntegrator.cu:264-540).  The integrator counts as
+// initialised: the next step uses these displacements as they are.
+void lt_set_state(LtSystem &L, const double *xyz, const double *vel, const double *dprev,
+                  unsigned long long step) {
+  if (!xyz || !vel || !dprev) throw InvalidArgument("restart state needs x, v and dprev");
+  if (L.n_batches > 0) throw InvalidArgument("the integrator is a member of a batch");
+  APO_HIP(hipStreamSynchronize(L.st));
+  const int n = L.n;
+  std::vector<double> soa(3 * (size_t)n);
+  auto up = [&](const double *a, DevBuf<double> &dst) {
+    for (int i = 0; i < n; i++)
+      for (int d = 0; d < 3; d++) soa[(size_t)d * n + i] = a[3 * (size_t)i + d];
+    APO_HIP(hipMemcpy(dst.get(), soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice));
+  };
+  up(xyz, L.x);
+  up(vel, L.v);
+  up(dprev, L.dp);
+  std::vector<float> q(n);
+  APO_HIP(hipMemcpy(q.data(), L.charge.get(), n * sizeof(float), hipMemcpyDeviceToHost));
+  std::vector<float4> q4(n);
+  for (int i = 0; i < n; i++)
+    q4[i] = make_float4((float)xyz[3 * (size_t)i], (float)xyz[3 * (size_t)i + 1],
+                        (float)xyz[3 * (size_t)i + 2], q[i]);
+  APO_HIP(hipMemcpy(L.xyzq.get(), q4.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  unsigned long long c[3];
+  APO_HIP(hipMemcpy(c, L.counters.get(), sizeof(c), hipMemcpyDeviceToHost));
+  c[0] = step;
+  APO_HIP(hipMemcpy(L.counters.get(), c, sizeof(c), hipMemcpyHostToDevice));
+  L.step = step;
+  L.steps_since_rebuild = 0;
+  L.initialized = true;
 }
 
 void lt_launch_info(LtSystem &L, double *out6) {

@@ -41,6 +41,7 @@ class LangevinThermostatIntegrator:
         self._list_policy = 0
         self._prune_every = 0
         self._launch_setting = 0
+        self._shake = True
         self._handle = ctypes.c_void_p()
         self._fm = None
         self._keep = []
@@ -83,6 +84,11 @@ class LangevinThermostatIntegrator:
         PME kernels at raised wave issue priority, bit 1 = pair kernel
         captured first).  No result bit depends on it."""
         self._launch_setting = 0 if setting is None else int(setting) + 1
+
+    def useHolonomicConstraints(self, use):
+        """SHAKE for the system's bonds to hydrogen (its shake_groups) on or
+        off; the SETTLE waters stay rigid either way."""
+        self._shake = bool(use)
 
     def getLaunchInfo(self):
         """dict: setting (0..3, -1 while the trial runs), trial_cycles, and
@@ -134,7 +140,16 @@ class LangevinThermostatIntegrator:
                 p.water_geom[k] = float(water_geom[k])
         p.mass = mass.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         p.charge = charge.ctypes.data_as(_lib._f32p)
-        self._keep = [waters, mass, charge]
+        # bonds to hydrogen held by SHAKE when the system lists them (the
+        # reference's context turns holonomic constraints on by default)
+        sg = np.ascontiguousarray(spec.get("shake_groups", np.zeros((0, 4))), np.int32).reshape(-1)
+        sr2 = np.ascontiguousarray(spec.get("shake_r2", np.zeros(0)), np.float64).reshape(-1)
+        if not self._shake:
+            sg, sr2 = sg[:0], sr2[:0]
+        p.nshake = len(sr2)
+        p.shake_groups = sg.ctypes.data_as(_lib._i32p)
+        p.shake_r2 = sr2.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        self._keep = [waters, mass, charge, sg, sr2]
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().apo_lt_create(
             force_manager._handle, ctypes.byref(p), xyz.ctypes.data_as(ctypes.c_void_p),
@@ -142,7 +157,7 @@ class LangevinThermostatIntegrator:
         self._handle = h
         self._fm = force_manager
         self.natom = n
-        self.ndegf = 3 * n - 3 - 3 * p.nwater
+        self.ndegf = 3 * n - 3 - 3 * p.nwater - int((sg.reshape(-1, 4)[:, 1:] >= 0).sum())
         return self
 
     def _stream(self):

@@ -170,6 +170,7 @@ def check_list(lt, fm, spec, prune_every):
     import numpy as np
     import torch
     from apocharmm_amd import systems
+    x_build = lt.getCoordinates()  # the cycle's rebuild step evaluates here
     lt.propagate(REBUILD_EVERY)
     st = lt.getState()
     info = lt.getListInfo()
@@ -180,6 +181,8 @@ def check_list(lt, fm, spec, prune_every):
     g = st["grad"]
     rel = float(np.sqrt(np.mean(np.sum((g - g_fresh) ** 2, 1)) /
                         np.mean(np.sum(g_fresh ** 2, 1))))
+    missed, inside = missed_pairs(float(spec["box"][0]), x, x_build, float(spec["ctofnb"]),
+                                  float(spec["cutnb"]))
     return {"max_displacement_A": round(info["max_displacement"], 4),
             "max_step_displacement_A": round(info["max_step_displacement"], 4),
             "half_buffer_A": info["half_buffer"],
@@ -187,7 +190,27 @@ def check_list(lt, fm, spec, prune_every):
             "prune_age_steps": (prune_every - 1) if prune_every else None,
             "grad_vs_fresh_list_rms_rel": rel,
             "grad_vs_fresh_list_max_abs": float(np.abs(g - g_fresh).max()),
-            "valid": bool(rel <= 1e-6 and info["max_displacement"] <= info["half_buffer"])}
+            "pairs_inside_roff": inside,
+            "pairs_inside_roff_beyond_list_cutoff_at_build": missed,
+            "strict_displacement_bound": bool(info["max_displacement"] <= info["half_buffer"]),
+            "valid": bool(missed == 0 and rel <= 2e-6)}
+
+
+def missed_pairs(L, x_eval, x_build, r_off, r_list):
+    """Exact count of the pairs inside r_off at x_eval whose minimum-image
+    distance at x_build (the list build) exceeded the list cutoff: pairs a
+    list built at x_build cannot hold (periodic KD-tree, cubic box)."""
+    import numpy as np
+    from scipy.spatial import cKDTree
+    xe = np.mod(x_eval, L)
+    pairs = cKDTree(xe, boxsize=L).query_pairs(r_off, output_type="ndarray")
+    missed = 0
+    for a in range(0, len(pairs), 1 << 22):
+        p = pairs[a:a + (1 << 22)]
+        d = x_build[p[:, 0]] - x_build[p[:, 1]]
+        d -= L * np.round(d / L)
+        missed += int(np.count_nonzero(np.einsum("ij,ij->i", d, d) > r_list * r_list))
+    return missed, int(len(pairs))
 
 
 def copy_calibration():

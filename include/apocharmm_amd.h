@@ -314,6 +314,15 @@ typedef struct apo_lt_params {
                             without a trial, bit 0 = PME kernels at raised
                             wave issue priority, bit 1 = pair kernel captured
                             first. */
+  int32_t nshake;        /* SHAKE groups of bonds to hydrogen (outside the
+                            SETTLE waters), each a heavy atom and up to three
+                            hydrogens bonded to it (ForceManager::
+                            initializeHolonomicConstraintsVariables,
+                            src/ForceManager.cu:844-918; solved as
+                            Shake2/3/4Kernel, src/CudaHolonomicConstraint.cu:
+                            324-520); 0 = none */
+  const int32_t *shake_groups; /* 4*nshake: heavy, h1, h2 or -1, h3 or -1 */
+  const double *shake_r2;      /* nshake: squared bond length of the group */
 } apo_lt_params;
 
 /* Binds an integrator to `sys` (setCharmmContext) with host coordinates

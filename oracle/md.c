@@ -210,14 +210,62 @@ int orc_shake_water(int nw, const int *w, const double *geom, const double *xref
   return worst;
 }
 
+/* SHAKE of bonds to hydrogen (src/CudaHolonomicConstraint.cu:324-520 solve
+ * the same constraints): groups of a heavy atom and up to three hydrogens
+ * (g[4k] heavy, g[4k+1..3] hydrogens or -1), every bond of group k held at
+ * squared length r2[k], displacements along the reference bonds weighted by
+ * the inverse masses.  Classic iterative SHAKE (Ryckaert, Ciccotti and
+ * Berendsen 1977): bonds corrected one after another until every
+ * |r^2 - d^2| <= tol d^2.  Returns the most iterations a group used, -1 if
+ * one did not converge in maxit sweeps. */
+int orc_shake_groups(int ng, const int *g, const double *r2, const double *invmass,
+                     const double *xref, double *x, double tol, int maxit) {
+  int worst = 0;
+  for (int k = 0; k < ng; k++) {
+    const int h0 = g[4 * k];
+    int it = 0;
+    for (; it < maxit; it++) {
+      int done = 1;
+      for (int c = 1; c < 4; c++) {
+        const int j = g[4 * k + c];
+        if (j < 0) continue;
+        double r[3], r0[3];
+        v3sub(x + 3 * h0, x + 3 * j, r);
+        v3sub(xref + 3 * h0, xref + 3 * j, r0);
+        const double diff = r2[k] - v3dot(r, r);
+        if (fabs(diff) > tol * r2[k]) done = 0;
+        const double mi = invmass[h0], mj = invmass[j];
+        const double gg = diff / (2.0 * (mi + mj) * v3dot(r, r0));
+        for (int d = 0; d < 3; d++) {
+          x[3 * h0 + d] += gg * mi * r0[d];
+          x[3 * j + d] -= gg * mj * r0[d];
+        }
+      }
+      if (done) break;
+    }
+    if (it == maxit) return -1;
+    if (it > worst) worst = it;
+  }
+  return worst;
+}
+
+/* SETTLE of the waters and SHAKE of the hydrogen groups, to FP64 rounding. */
+static void constrain_all(int nw, const int *w, const double *geom, int ns, const int *sg,
+                          const double *sr2, const double *invmass, const double *xref,
+                          double *x) {
+  orc_settle(nw, w, geom, xref, x);
+  if (ns > 0) orc_shake_groups(ns, sg, sr2, invmass, xref, x, 1e-15, 1000);
+}
+
 /* First half of CudaLangevinThermostatIntegrator::initialize
  * (src/CudaLangevinThermostatIntegrator.cu:217-232 with
  * src/CudaHolonomicConstraint.cu:94-108, 638-676): constrain the coordinates
  * against themselves and fold the correction into the velocities. */
-void orc_lt_init_constrain(int n, int nw, const int *w, const double *geom, double dt, double *x,
+void orc_lt_init_constrain(int n, int nw, const int *w, const double *geom, int ns, const int *sg,
+                           const double *sr2, const double *invmass, double dt, double *x,
                            double *v, double *scratch) {
   memcpy(scratch, x, sizeof(double) * 3 * (size_t)n);
-  orc_settle(nw, w, geom, scratch, x);
+  constrain_all(nw, w, geom, ns, sg, sr2, invmass, scratch, x);
   for (int k = 0; k < 3 * n; k++) v[k] += (x[k] - scratch[k]) / dt;
 }
 
@@ -225,8 +273,9 @@ void orc_lt_init_constrain(int n, int nw, const int *w, const double *geom, doub
  * 234-258): leapfrog back-step from the gradient at x, constrained against
  * x.  Writes the previous half-step displacement dprev; x is unchanged. */
 void orc_lt_init_backstep(int n, const double *invmass, int nw, const int *w, const double *geom,
-                          double dt, double gamma, const double *grad, const double *x,
-                          const double *v, double *dprev, double *scratch) {
+                          int ns, const int *sg, const double *sr2, double dt, double gamma,
+                          const double *grad, const double *x, const double *v, double *dprev,
+                          double *scratch) {
   const double hg = 0.5 * gamma;
   const double a1 = dt * (1.0 - hg) / sqrt(1.0 + hg);
   for (int i = 0; i < n; i++) {
@@ -237,7 +286,7 @@ void orc_lt_init_backstep(int n, const double *invmass, int nw, const int *w, co
       scratch[k] = x[k] - dprev[k];
     }
   }
-  orc_settle(nw, w, geom, x, scratch);
+  constrain_all(nw, w, geom, ns, sg, sr2, invmass, x, scratch);
   for (int k = 0; k < 3 * n; k++) dprev[k] = x[k] - scratch[k];
 }
 
@@ -260,8 +309,9 @@ void orc_lt_remove_com(int n, const double *invmass, double *dprev) {
  * start-of-step coordinates, velocity from the two half-step displacements,
  * kinetic energies ke[0] (mean of the old half-step, on-step and new
  * half-step estimates) and ke[1] (on-step), then dprev <- new displacement. */
-void orc_lt_step(int n, const double *invmass, int nw, const int *w, const double *geom,
-                 double dt, double gamma, double kbt, uint64_t seed, uint64_t step,
+void orc_lt_step(int n, const double *invmass, int nw, const int *w, const double *geom, int ns,
+                 const int *sg, const double *sr2, double dt, double gamma, double kbt,
+                 uint64_t seed, uint64_t step,
                  const double *grad, double *x, double *dprev, double *v, double *ke,
                  double *scratch) {
   const double hg = 0.5 * gamma;
@@ -280,7 +330,7 @@ void orc_lt_step(int n, const double *invmass, int nw, const int *w, const doubl
       x[k] += alpha * dprev[k] - fact * g;
     }
   }
-  orc_settle(nw, w, geom, xref, x);
+  constrain_all(nw, w, geom, ns, sg, sr2, invmass, xref, x);
   const double fv = 0.5 * sqrt(1.0 + hg) / dt;
   const double idt = 1.0 / dt;
   ke[0] = ke[1] = 0.0;

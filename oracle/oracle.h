@@ -88,16 +88,20 @@ void orc_settle_geometry(const double *geom, double *abc);
 void orc_settle(int nw, const int *w, const double *geom, const double *xref, double *x);
 int orc_shake_water(int nw, const int *w, const double *geom, const double *xref, double *x,
                     double tol, int maxit);
-void orc_lt_init_constrain(int n, int nw, const int *w, const double *geom, double dt, double *x,
+int orc_shake_groups(int ng, const int *g, const double *r2, const double *invmass,
+                     const double *xref, double *x, double tol, int maxit);
+void orc_lt_init_constrain(int n, int nw, const int *w, const double *geom, int ns, const int *sg,
+                           const double *sr2, const double *invmass, double dt, double *x,
                            double *v, double *scratch);
 void orc_lt_init_backstep(int n, const double *invmass, int nw, const int *w, const double *geom,
-                          double dt, double gamma, const double *grad, const double *x,
-                          const double *v, double *dprev, double *scratch);
+                          int ns, const int *sg, const double *sr2, double dt, double gamma,
+                          const double *grad, const double *x, const double *v, double *dprev,
+                          double *scratch);
 void orc_lt_remove_com(int n, const double *invmass, double *dprev);
-void orc_lt_step(int n, const double *invmass, int nw, const int *w, const double *geom,
-                 double dt, double gamma, double kbt, uint64_t seed, uint64_t step,
-                 const double *grad, double *x, double *dprev, double *v, double *ke,
-                 double *scratch);
+void orc_lt_step(int n, const double *invmass, int nw, const int *w, const double *geom, int ns,
+                 const int *sg, const double *sr2, double dt, double gamma, double kbt,
+                 uint64_t seed, uint64_t step, const double *grad, double *x, double *dprev,
+                 double *v, double *ke, double *scratch);
 void orc_center(int ng, const int *gptr, const int *gatom, const double *box, double *x);
 
 #ifdef __cplusplus

@@ -48,11 +48,13 @@ def lib():
         L.orc_settle_geometry.argtypes = [_dp, _dp]
         L.orc_settle.argtypes = [i, _ip, _dp, _dp, _dp]
         L.orc_shake_water.argtypes = [i, _ip, _dp, _dp, _dp, d, i]
-        L.orc_lt_init_constrain.argtypes = [i, i, _ip, _dp, d, _dp, _dp, _dp]
-        L.orc_lt_init_backstep.argtypes = [i, _dp, i, _ip, _dp, d, d, _dp, _dp, _dp, _dp, _dp]
+        L.orc_shake_groups.argtypes = [i, _ip, _dp, _dp, _dp, _dp, d, i]
+        L.orc_lt_init_constrain.argtypes = [i, i, _ip, _dp, i, _ip, _dp, _dp, d, _dp, _dp, _dp]
+        L.orc_lt_init_backstep.argtypes = [i, _dp, i, _ip, _dp, i, _ip, _dp, d, d, _dp, _dp, _dp,
+                                           _dp, _dp]
         L.orc_lt_remove_com.argtypes = [i, _dp, _dp]
-        L.orc_lt_step.argtypes = [i, _dp, i, _ip, _dp, d, d, d, u64, u64, _dp, _dp, _dp, _dp,
-                                  _dp, _dp]
+        L.orc_lt_step.argtypes = [i, _dp, i, _ip, _dp, i, _ip, _dp, d, d, d, u64, u64, _dp, _dp,
+                                  _dp, _dp, _dp, _dp]
         L.orc_center.argtypes = [i, _ip, _ip, _dp, _dp]
         _LIB = L
     return _LIB
@@ -330,9 +332,12 @@ class Langevin:
     x, v, dprev are (n, 3) float64 arrays updated in place; grad is dU/dx
     (n, 3).  dt_ps, friction_ps (1/ps), temperature (K) as in the reference
     setters; internally dt is in AKMA units and gamma = dt_ps * friction.
+    shake: optional (groups (k, 4) int, r2 (k,)) SHAKE groups of bonds to
+    hydrogen, constrained with the waters after every update.
     """
 
-    def __init__(self, invmass, waters, dt_ps, temperature, friction_ps, seed, geom=TIP3P_GEOM):
+    def __init__(self, invmass, waters, dt_ps, temperature, friction_ps, seed, geom=TIP3P_GEOM,
+                 shake=None):
         self.invmass = np.ascontiguousarray(invmass, np.float64)
         self.n = len(self.invmass)
         self.w = np.ascontiguousarray(waters, np.int32).reshape(-1)
@@ -343,15 +348,26 @@ class Langevin:
         self.kbt = KBOLTZ * temperature
         self.seed = int(seed)
         self.scratch = np.zeros(3 * self.n)
+        if shake is None:
+            shake = (np.zeros((0, 4), np.int32), np.zeros(0))
+        self.sg = np.ascontiguousarray(shake[0], np.int32).reshape(-1)
+        self.sr2 = np.ascontiguousarray(shake[1], np.float64).reshape(-1)
+        if len(self.sg) == 0:  # ndpointer arguments need a real array
+            self.sg, self.sr2 = np.full(4, -1, np.int32), np.ones(1)
+            self.ns = 0
+        else:
+            self.ns = len(self.sr2)
 
     def init_constrain(self, x, v):
-        lib().orc_lt_init_constrain(self.n, self.nw, self.w, self.geom, self.dt, x.reshape(-1),
-                                    v.reshape(-1), self.scratch)
+        lib().orc_lt_init_constrain(self.n, self.nw, self.w, self.geom, self.ns, self.sg,
+                                    self.sr2, self.invmass, self.dt, x.reshape(-1), v.reshape(-1),
+                                    self.scratch)
 
     def init_backstep(self, grad, x, v):
         dprev = np.zeros((self.n, 3))
-        lib().orc_lt_init_backstep(self.n, self.invmass, self.nw, self.w, self.geom, self.dt,
-                                   self.gamma, np.ascontiguousarray(grad, np.float64).reshape(-1),
+        lib().orc_lt_init_backstep(self.n, self.invmass, self.nw, self.w, self.geom, self.ns,
+                                   self.sg, self.sr2, self.dt, self.gamma,
+                                   np.ascontiguousarray(grad, np.float64).reshape(-1),
                                    np.ascontiguousarray(x).reshape(-1),
                                    np.ascontiguousarray(v).reshape(-1), dprev.reshape(-1),
                                    self.scratch)
@@ -362,11 +378,21 @@ class Langevin:
 
     def step(self, grad, x, dprev, v, step):
         ke = np.zeros(2)
-        lib().orc_lt_step(self.n, self.invmass, self.nw, self.w, self.geom, self.dt, self.gamma,
-                          self.kbt, self.seed, int(step),
+        lib().orc_lt_step(self.n, self.invmass, self.nw, self.w, self.geom, self.ns, self.sg,
+                          self.sr2, self.dt, self.gamma, self.kbt, self.seed, int(step),
                           np.ascontiguousarray(grad, np.float64).reshape(-1), x.reshape(-1),
                           dprev.reshape(-1), v.reshape(-1), ke, self.scratch)
         return ke
+
+
+def shake_groups(groups, r2, invmass, xref, x, tol=1e-15, maxit=1000):
+    """Iterative SHAKE of hydrogen groups (orc_shake_groups) on (n, 3) x in
+    place; returns the sweeps used (-1: not converged)."""
+    g = np.ascontiguousarray(groups, np.int32).reshape(-1)
+    return lib().orc_shake_groups(len(g) // 4, g, np.ascontiguousarray(r2, np.float64),
+                                  np.ascontiguousarray(invmass, np.float64),
+                                  np.ascontiguousarray(xref, np.float64).reshape(-1),
+                                  x.reshape(-1), tol, maxit)
 
 
 def center(groups_ptr, groups_atom, box, x):

@@ -1,5 +1,5 @@
 """C ABI checks that need no GPU: the library loads, exports every entry point
-declared in include/apocharmm_amd.h, validates arguments before touching the
+declared in include/*.h, validates arguments before touching the
 device, and derives the same exclusion topology as the oracle."""
 import ctypes
 import os
@@ -14,7 +14,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_symbols():
-    hdr = open(os.path.join(ROOT, "include", "apocharmm_amd.h")).read()
+    inc = os.path.join(ROOT, "include")
+    hdr = "".join(open(os.path.join(inc, f)).read() for f in sorted(os.listdir(inc))
+                  if f.endswith(".h"))
     return sorted(set(re.findall(r"APO_API\s+[^;(]*?\b(apo_\w+)\s*\(", hdr)))
 
 

@@ -163,3 +163,75 @@ def test_remove_com_and_centering():
     assert np.allclose(shift[3:5], [10, -12, 0])
     assert np.allclose(shift[5], [0, 12, -14])
     assert np.array_equal(O.center(gptr, gatom, box, xc.copy()), xc)
+
+
+def _shake_case(rng, m, stretch=0.08):
+    """A heavy atom with m hydrogens: reference positions on bonds of 1.09 A,
+    then an unconstrained move of up to `stretch` A per coordinate."""
+    ref = np.zeros((m + 1, 3))
+    for k in range(m):
+        d = rng.normal(size=3)
+        ref[k + 1] = ref[0] + 1.09 * d / np.linalg.norm(d)
+    ref += rng.normal(size=3)
+    new = ref + rng.uniform(-stretch, stretch, size=ref.shape)
+    invm = np.array([1 / 12.011] + [1 / 1.008] * m)
+    g = np.array([[0] + list(range(1, m + 1)) + [-1] * (3 - m)], np.int32)
+    return ref, new, invm, g, np.array([1.09 ** 2])
+
+
+def test_shake_single_bond_matches_closed_form(oracle):
+    """One bond: the iterative solver reaches the root of the quadratic that
+    the reference's Shake2Kernel evaluates in closed form
+    (src/CudaHolonomicConstraint.cu:324-360)."""
+    rng = np.random.default_rng(31)
+    for _ in range(20):
+        ref, new, invm, g, r2 = _shake_case(rng, 1)
+        x = new.copy()
+        assert oracle.shake_groups(g, r2, invm, ref, x) >= 0
+        p = new[0] - new[1]
+        r0 = ref[0] - ref[1]
+        rp, rr = p @ r0, r0 @ r0
+        lam = (-rp + np.sqrt(rp * rp + rr * (r2[0] - p @ p))) / (rr * (invm[0] + invm[1]))
+        np.testing.assert_allclose(x[0], new[0] + invm[0] * lam * r0, rtol=0, atol=1e-12)
+        np.testing.assert_allclose(x[1], new[1] - invm[1] * lam * r0, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("m", [2, 3])
+def test_shake_group_invariants(oracle, m):
+    """Two or three bonds on one heavy atom: every bond length is met, the
+    total momentum of the correction is zero, and each hydrogen moves only
+    along its own reference bond (the heavy atom takes the mass-weighted
+    rest), which fixes the solution of the constraint equations."""
+    rng = np.random.default_rng(40 + m)
+    for _ in range(20):
+        ref, new, invm, g, r2 = _shake_case(rng, m)
+        x = new.copy()
+        assert oracle.shake_groups(g, r2, invm, ref, x) >= 0
+        for k in range(1, m + 1):
+            assert abs(np.sum((x[0] - x[k]) ** 2) - r2[0]) <= 1e-13 * r2[0]
+            dk = x[k] - new[k]
+            bond = ref[0] - ref[k]
+            assert np.linalg.norm(np.cross(dk, bond)) <= 1e-12 * max(np.linalg.norm(dk), 1e-30)
+        mom = ((x - new) / invm[:, None]).sum(0)
+        assert np.abs(mom).max() < 1e-12
+
+
+def test_langevin_step_with_shake_keeps_bond_lengths(oracle):
+    """The oracle integrator constrains hydrogen groups with the waters: after
+    the initial constraint, a back-step and five steps every SHAKE bond and
+    every water geometry holds to FP64 rounding."""
+    rng = np.random.default_rng(5)
+    ref, _, invm, g, r2 = _shake_case(rng, 3, 0.0)
+    w = np.array([[0.0, 0.0, 0.0], [0.9572, 0, 0], [-0.24, 0.9266, 0.0]]) + 4.0
+    x = np.vstack([ref, w])
+    invmass = np.concatenate([invm, [1 / 15.9994, 1 / 1.008, 1 / 1.008]])
+    v = rng.normal(size=x.shape) * 0.01
+    lt = oracle.Langevin(invmass, [[4, 5, 6]], 0.002, 300.0, 1.0, 3, shake=(g, r2))
+    lt.init_constrain(x, v)
+    grad = rng.normal(size=x.shape)
+    dprev = lt.init_backstep(grad, x, v)
+    for s in range(5):
+        lt.step(rng.normal(size=x.shape), x, dprev, v, s)
+        for k in range(1, 4):
+            assert abs(np.sum((x[0] - x[k]) ** 2) - r2[0]) <= 1e-12 * r2[0]
+        assert abs(np.linalg.norm(x[4] - x[5]) - 0.9572) < 1e-10
