@@ -381,7 +381,7 @@ void bonded_launch(NbSystem &S, const float4 *x, int64_t *force, int stride, boo
                S.nbond, S.nub, S.nangle, S.ndihe, S.nimdihe, stride};
   const int nt = S.nbond + S.nub + S.nangle + S.ndihe + S.nimdihe;
   auto *f = reinterpret_cast<unsigned long long *>(force);
-  double *part = S.parts.get() + S.part.bond_off;
+  double *part = S.parts_ptr() + S.part.bond_off;
   const int G = S.part.bond_n;
   if (nt == 0) {
     if (E || V) launch_fill_f64(part, (size_t)G * kBondPart, s);
@@ -403,7 +403,7 @@ void pairs14_launch(NbSystem &S, const float4 *x, int64_t *force, int stride, bo
   P14Args a{x, S.vdwtype.get(), S.in14.get(), S.ex14.get(), S.nin14, S.nex14, stride,
             S.ntypes, S.e14fac};
   auto *f = reinterpret_cast<unsigned long long *>(force);
-  double *part = S.parts.get() + S.part.p14_off;
+  double *part = S.parts_ptr() + S.part.p14_off;
   const int G = S.part.p14_n;
   if (S.nin14 + S.nex14 == 0) {
     if (E || V) launch_fill_f64(part, (size_t)G * kP14Part, s);

@@ -29,7 +29,7 @@ void set_box(NbSystem &S, const double *b);
 struct LtSystem;
 LtSystem *create_lt(NbSystem *nb, const apo_lt_params &p, const double *xyz, const double *vel);
 void destroy_lt(LtSystem *L);
-void lt_propagate(LtSystem &L, int nsteps, hipStream_t caller);
+void lt_propagate(LtSystem &L, int nsteps, hipStream_t caller, bool fresh);
 void lt_get_state(LtSystem &L, double *xyz, double *vel, double *dprev, double *grad,
                   double *thermo, int64_t *step);
 void lt_set_state(LtSystem &L, const double *xyz, const double *vel, const double *dprev,
@@ -757,9 +757,10 @@ APO_API apo_status apo_cuda_integrator_subscribe(apo_cuda_integrator *integrator
     I.subs.push_back(subscriber->object);
   });
 }
-// CudaIntegrator::propagate (src/CudaIntegrator.cu:129-212): steps 1..n, each
-// subscriber updated after the steps that are multiples of its frequency; the
-// steps between reports run as one fused propagate call.
+// CudaIntegrator::propagate (src/CudaIntegrator.cu:129-212): one list reset
+// and centering at the start, steps 1..n, each subscriber updated after the
+// steps that are multiples of its frequency; the steps between reports run as
+// one fused propagate piece, and later pieces continue the list cadence.
 APO_API apo_status apo_cuda_integrator_propagate(apo_cuda_integrator *integrator,
                                                  const int num_steps) {
   return guard([&] {
@@ -772,7 +773,7 @@ APO_API apo_status apo_cuda_integrator_propagate(apo_cuda_integrator *integrator
       int next = num_steps;
       for (auto &s : I.subs)
         if (s->freq > 0) next = std::min(next, (done / s->freq + 1) * s->freq);
-      apo::lt_propagate(L, next - done, nullptr);
+      apo::lt_propagate(L, next - done, nullptr, done == 0);
       I.total_steps += (unsigned long long)(next - done);
       done = next;
       I.report(done);

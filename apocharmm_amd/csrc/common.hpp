@@ -195,6 +195,7 @@ struct PairArgs {
   const uint32_t *ent;
   const int *cand_j, *meta;
   const float2 *vdwtab, *vdwsq;
+  const float2 *ljtype;  // per-type (sqrt(eps), Rmin/2) when the table follows the rule
   int ntypes;
   PairConsts<float> P;
   float qfac;

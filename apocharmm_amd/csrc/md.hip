@@ -36,8 +36,8 @@ void batch_add_sort(BatchLaunch<SortArgs> &b, NbSystem &S, const float4 *x, int6
                     size_t nforce);
 void batch_sort(BatchLaunch<SortArgs> &b, hipStream_t s);
 void batch_add_pair(BatchLaunch<PairArgs> &b, NbSystem &S, int blocks);
-void batch_pair(BatchLaunch<PairArgs> &b, bool tab_lds, hipStream_t s);
-bool pair_tab_lds(const NbSystem &S);
+void batch_pair(BatchLaunch<PairArgs> &b, int tab_mode, hipStream_t s);
+int pair_tab_lds(const NbSystem &S);
 struct PmeBatch;
 PmeBatch *pme_batch_create();
 void pme_batch_destroy(PmeBatch *b);
@@ -1034,8 +1034,12 @@ struct LtSystem {
   // CudaIntegrator::propagate (src/CudaIntegrator.cu:140-230) with the
   // thermostat's propagateOneStep (src/CudaLangevinThermostatIntegrator.cu:703-800):
   // whole list cycles as one graph where the schedule allows, else step by step.
-  void propagate(int nsteps, hipStream_t s) {
-    begin(s);
+  // fresh = false continues the previous call's list cadence (no list reset
+  // or centering at the start), for callers that split one propagate into
+  // pieces (subscriber reports between them)
+  void propagate(int nsteps, hipStream_t s, bool fresh = true) {
+    if (fresh) begin(s);
+    else if (!initialized) initialize(s);
     for (int k = 0; k < nsteps;) {
       if (nsteps - k >= nbfreq && steps_since_rebuild % nbfreq == 0 && cycle_ok()) {
         if (trial_pending()) trial_cycle(s);
@@ -1344,9 +1348,12 @@ void lt_initialize(LtSystem &L, hipStream_t caller) {
   on_stream(L, caller, [&](hipStream_t s) { L.initialize(s); });
 }
 
-void lt_propagate(LtSystem &L, int nsteps, hipStream_t caller) {
+void lt_propagate(LtSystem &L, int nsteps, hipStream_t caller, bool fresh) {
   if (nsteps < 0) throw InvalidArgument("nsteps must be >= 0");
-  on_stream(L, caller, [&](hipStream_t s) { L.propagate(nsteps, s); });
+  on_stream(L, caller, [&](hipStream_t s) { L.propagate(nsteps, s, fresh); });
+}
+void lt_propagate(LtSystem &L, int nsteps, hipStream_t caller) {
+  lt_propagate(L, nsteps, caller, true);
 }
 
 // Many independent systems advanced together (no counterpart in the
@@ -1378,7 +1385,7 @@ struct LtBatch {
   BatchLaunch<LtFinArgs> lfin;
   PmeBatch *pme = nullptr;
   BondedBatch *bond = nullptr;
-  bool tab_lds = true;
+  int tab_mode = 1;  // vdW table layout of the fused pair launch (pair.hip TAB_*)
   std::vector<unsigned long long> tab_gens;
   // launch shapes (the autotuner's choices): pair-kernel blocks per CU
   // shared out by atom count, FFT threads, row pairs and columns per block
@@ -1418,7 +1425,8 @@ struct LtBatch {
     lfin.clear();
     pme_batch_clear(*pme);
     bonded_batch_clear(*bond);
-    tab_lds = true;
+    tab_mode = 1;
+    int i_sys = 0;
     double ntot = 0;
     for (LtSystem *L : sys) ntot += L->n;
     const int ncu = sys[0]->nb->ncu;
@@ -1429,7 +1437,17 @@ struct LtBatch {
       batch_add_sort(sort, S, x, g, 3 * (size_t)L->gstride);
       const int pb = std::max(8, (int)std::lround(pair_bpc * ncu * (L->n / ntot)));
       batch_add_pair(pair, S, pb);
-      tab_lds = tab_lds && pair_tab_lds(S);
+      // one layout for all: square LDS (1) only if every system allows it,
+      // global (0) if any system needs it, else triangular LDS (2)
+      const int m = pair_tab_lds(S);
+      if (i_sys == 0) tab_mode = m;
+      else if (m != tab_mode) {
+        // mixed layouts: the triangular table fits every LDS-capable system;
+        // any system beyond it sends the launch to the global table
+        const bool lds_ok = (m == 1 || m == 2) && (tab_mode == 1 || tab_mode == 2);
+        tab_mode = lds_ok ? 2 : 0;
+      }
+      i_sys++;
       pme_batch_add(*pme, S, g, L->gstride, fft_rows, fft_cols, fft_threads);
       bonded_batch_add(*bond, S, x, g, L->gstride);
       lstep.add(LtStepArgs{L->state(true), L->C, L->counters.get(), L->ke_part.get(),
@@ -1473,7 +1491,7 @@ struct LtBatch {
     APO_HIP(hipStreamWaitEvent(side[1], sort_ev, 0));
     bonded_batch_launch(*bond, side[1]);
     APO_HIP(hipEventRecord(side_ev[1], side[1]));
-    batch_pair(pair, tab_lds, s);
+    batch_pair(pair, tab_mode, s);
     APO_HIP(hipStreamWaitEvent(s, side_ev[0], 0));
     APO_HIP(hipStreamWaitEvent(s, side_ev[1], 0));
     // the sorted pair gradient is added by the step kernel (kb_lt_step)

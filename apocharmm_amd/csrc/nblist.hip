@@ -553,11 +553,14 @@ void nblist_build(NbSystem &S, const float4 *x, hipStream_t s, bool sizing) {
   const float rl2 = (float)(S.cutnb * S.cutnb);
   int *meta = S.meta.get();
   sizing = sizing || S.cand_cap == 0 || S.ent_cap == 0 || S.item_cap == 0;
+  if (sizing && !S.may_allocate)
+    throw InvalidArgument("the neighbor list has no capacity yet: build it once with an "
+                          "allocating call (apo_nb_reset_list or apo_nb_calc_force)");
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   APO_HIP(hipStreamIsCapturing(s, &cap));
   if (sizing && cap != hipStreamCaptureStatusNone)
     throw RuntimeError("the first neighbor-list build cannot be captured in a graph");
-  if (!sizing && cap == hipStreamCaptureStatusNone) list_export_check(S, s, true);
+  if (!sizing && cap == hipStreamCaptureStatusNone) list_export_check(S, s, S.may_allocate);
 
   launch_fill_i32(meta, 16, s);
   launch_fill_i32(S.col_count.get(), S.ncol + 1, s);

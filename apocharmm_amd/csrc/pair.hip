@@ -80,6 +80,27 @@ template <typename CT> __device__ __forceinline__ void group_reduce(CT (&v)[3 * 
 }
 
 constexpr int kMaxTypesLds = 32;  // square vdW table in LDS up to 32 types
+// beyond that the triangular table goes to LDS while it fits this many bytes
+constexpr size_t kMaxTriLdsBytes = 32768;
+
+// Where the pair kernel reads the (6 c6, 12 c12) table: TAB_GLOBAL the
+// triangular table in global memory, TAB_SQUARE an ntypes x ntypes copy in
+// LDS (row = type * ntypes, no index arithmetic per pair), TAB_TRI the
+// triangular table copied to LDS.
+// TAB_COMBINE (mixed precision, more types than the triangular LDS table
+// holds, every entry following the combination rule): 6 c6 = 12 e R^6 and
+// 12 c12 = 12 e R^12 from per-type (sqrt(eps), Rmin/2), e = se_i se_j,
+// R = rh_i + rh_j.
+enum { TAB_GLOBAL = 0, TAB_SQUARE = 1, TAB_TRI = 2, TAB_COMBINE = 3 };
+template <typename T2> static int pair_tab_mode(int ntypes, bool combine) {
+  if (ntypes <= kMaxTypesLds) return TAB_SQUARE;
+  if ((size_t)ntypes * (ntypes + 1) / 2 * sizeof(T2) <= kMaxTriLdsBytes) return TAB_TRI;
+  return combine && sizeof(T2) == sizeof(float2) ? TAB_COMBINE : TAB_GLOBAL;
+}
+template <typename T2> static size_t pair_tab_lds_bytes(int mode, int ntypes) {
+  return mode == TAB_SQUARE ? (size_t)ntypes * ntypes * sizeof(T2)
+                            : (mode == TAB_TRI ? (size_t)ntypes * (ntypes + 1) / 2 * sizeof(T2) : 0);
+}
 
 // One wave64 = one work item: an i-group of kGroup consecutive sorted slots
 // and up to kItemSteps steps of 64 packed j entries (candidate tile of the
@@ -88,21 +109,24 @@ constexpr int kMaxTypesLds = 32;  // square vdW table in LDS up to 32 types
 // visited in an unrolled loop, skipped with a scalar branch when no lane has
 // a live pair with them.  i-forces accumulate in 3*kGroup registers and are
 // reduced once per item; j-forces are flushed per step.
-// TAB_LDS: vdW table square (ntypes x ntypes) in LDS; otherwise the
-// triangular table is read from global memory.
-template <typename CT, bool CALC_E, bool CALC_V, bool TAB_LDS>
+// TAB: where the vdW table is read (TAB_GLOBAL / TAB_SQUARE / TAB_TRI).
+template <typename CT, bool CALC_E, bool CALC_V, int TAB>
 __device__ __forceinline__ void pair_body(
     int bid, int nb, const int4 *__restrict__ posq, const int *__restrict__ types,
     const int4 *__restrict__ items, const uint32_t *__restrict__ ent,
     const int *__restrict__ cand_j, const int *__restrict__ meta,
     const typename Vec2<CT>::T *__restrict__ vdwtab, const typename Vec2<CT>::T *__restrict__ vdwsq,
-    int ntypes, const PairConsts<CT> &P, CT qfac, unsigned long long *__restrict__ fsort, int ss,
-    double *__restrict__ part, const PairConsts<double> &PE, const double2 *__restrict__ vdwtab_e) {
+    const float2 *__restrict__ ljtype, int ntypes, const PairConsts<CT> &P, CT qfac,
+    unsigned long long *__restrict__ fsort, int ss, double *__restrict__ part,
+    const PairConsts<double> &PE, const double2 *__restrict__ vdwtab_e) {
   using T2 = typename Vec2<CT>::T;
   extern __shared__ char smem_pair[];
-  T2 *sh_tab = reinterpret_cast<T2 *>(smem_pair);  // ntypes^2 entries when TAB_LDS
-  if (TAB_LDS) {
+  T2 *sh_tab = reinterpret_cast<T2 *>(smem_pair);
+  if (TAB == TAB_SQUARE) {
     for (int k = threadIdx.x; k < ntypes * ntypes; k += 64) sh_tab[k] = vdwsq[k];
+    __syncthreads();
+  } else if (TAB == TAB_TRI) {
+    for (int k = threadIdx.x; k < ntypes * (ntypes + 1) / 2; k += 64) sh_tab[k] = vdwtab[k];
     __syncthreads();
   }
   const int lane = threadIdx.x;
@@ -119,6 +143,9 @@ __device__ __forceinline__ void pair_body(
     const int til = types[ibase + (lane & (kGroup - 1))];
     int pix[kGroup], piy[kGroup], piz[kGroup], rowi[kGroup];
     CT qi[kGroup];
+    float sei[kGroup], rhi[kGroup];  // TAB_COMBINE only
+    float2 ljl = make_float2(0.f, 0.f);
+    if (TAB == TAB_COMBINE) ljl = ljtype[til];
 #pragma unroll
     for (int k = 0; k < kGroup; k++) {
       pix[k] = __builtin_amdgcn_readlane(pil.x, k);
@@ -126,7 +153,11 @@ __device__ __forceinline__ void pair_body(
       piz[k] = __builtin_amdgcn_readlane(pil.z, k);
       qi[k] = charge_of<CT>(__builtin_amdgcn_readlane(pil.w, k));
       const int t = __builtin_amdgcn_readlane(til, k);
-      rowi[k] = TAB_LDS ? t * ntypes : t;
+      rowi[k] = TAB == TAB_SQUARE ? t * ntypes : t;
+      if (TAB == TAB_COMBINE) {
+        sei[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ljl.x), k));
+        rhi[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ljl.y), k));
+      }
     }
     CT fi[3 * kGroup];
 #pragma unroll
@@ -156,6 +187,8 @@ __device__ __forceinline__ void pair_body(
       for (int off = 32; off > 0; off >>= 1) act |= __shfl_xor(act, off);
       act = __builtin_amdgcn_readfirstlane(act);
       const CT qj = charge_of<CT>(pj.w) * qfac;
+      float2 ljj = make_float2(0.f, 0.f);
+      if (TAB == TAB_COMBINE) ljj = ljtype[tj];
       CT fjx = 0, fjy = 0, fjz = 0;
       // energies accumulate per pair in FP64 (large cancelling sums); the
       // virial per step in CT
@@ -175,12 +208,17 @@ __device__ __forceinline__ void pair_body(
           CT u_el;
           const CT f_el = ewald_pair(r2, r, rinv, qq, P, u_el);
           T2 c;
-          if (TAB_LDS) {
+          if (TAB == TAB_SQUARE) {
             c = sh_tab[rowi[k] + tj];
+          } else if (TAB == TAB_COMBINE) {
+            const float R = rhi[k] + ljj.y, R2 = R * R, R6 = R2 * R2 * R2;
+            const float c6 = 12.0f * sei[k] * ljj.x * R6;
+            c.x = (CT)c6;
+            c.y = (CT)(c6 * R6);
           } else {
             const int ti = rowi[k];
             const int aa = ti > tj ? ti : tj, bb = ti > tj ? tj : ti;
-            c = vdwtab[aa * (aa + 1) / 2 + bb];
+            c = TAB == TAB_TRI ? sh_tab[aa * (aa + 1) / 2 + bb] : vdwtab[aa * (aa + 1) / 2 + bb];
           }
           const CT rinv2 = rinv * rinv;
           const CT rinv3 = rinv * rinv2;
@@ -216,7 +254,7 @@ __device__ __forceinline__ void pair_body(
             const double er = sqrt(er2), erinv = 1.0 / er;
             el += (double)qi[k] * (double)charge_of<CT>(pj.w) * kCcelec * erfc(PE.kappa * er) *
                   erinv;
-            const int ti = TAB_LDS ? rowi[k] / ntypes : rowi[k];
+            const int ti = TAB == TAB_SQUARE ? rowi[k] / ntypes : rowi[k];
             const int aa = ti > tj ? ti : tj, bb = ti > tj ? tj : ti;
             const double2 ce = vdwtab_e[aa * (aa + 1) / 2 + bb];
             const double er3 = erinv * erinv * erinv, er6 = er3 * er3;
@@ -279,30 +317,32 @@ __device__ __forceinline__ void pair_body(
   }
 }
 
-template <typename CT, bool CALC_E, bool CALC_V, bool TAB_LDS>
+template <typename CT, bool CALC_E, bool CALC_V, int TAB>
 __global__ __launch_bounds__(64) void k_pair(
     const int4 *__restrict__ posq, const int *__restrict__ types, const int4 *__restrict__ items,
     const uint32_t *__restrict__ ent, const int *__restrict__ cand_j, const int *__restrict__ meta,
     const typename Vec2<CT>::T *__restrict__ vdwtab, const typename Vec2<CT>::T *__restrict__ vdwsq,
-    int ntypes, const PairConsts<CT> P, CT qfac, unsigned long long *__restrict__ fsort, int ss,
-    double *__restrict__ part, const PairConsts<double> PE, const double2 *__restrict__ vdwtab_e) {
-  pair_body<CT, CALC_E, CALC_V, TAB_LDS>(blockIdx.x, gridDim.x, posq, types, items, ent, cand_j,
-                                         meta, vdwtab, vdwsq, ntypes, P, qfac, fsort, ss, part, PE,
+    const float2 *__restrict__ ljtype, int ntypes, const PairConsts<CT> P, CT qfac,
+    unsigned long long *__restrict__ fsort, int ss, double *__restrict__ part,
+    const PairConsts<double> PE, const double2 *__restrict__ vdwtab_e) {
+  pair_body<CT, CALC_E, CALC_V, TAB>(blockIdx.x, gridDim.x, posq, types, items, ent, cand_j,
+                                         meta, vdwtab, vdwsq, ljtype, ntypes, P, qfac, fsort, ss,
+                                         part, PE,
                                          vdwtab_e);
 }
 
 // Fused launch over several systems: mixed precision, no energy/virial (the
 // MD step).
-template <bool TAB_LDS>
+template <int TAB>
 __global__ __launch_bounds__(64) void kb_pair(const PairArgs *__restrict__ tab,
                                               const int *__restrict__ bstart, int nsys) {
   const int s = batch_system(bstart, nsys);
   const PairArgs &a = tab[s];
   const PairConsts<double> no_energy{};
-  pair_body<float, false, false, TAB_LDS>(blockIdx.x - bstart[s], bstart[s + 1] - bstart[s],
+  pair_body<float, false, false, TAB>(blockIdx.x - bstart[s], bstart[s + 1] - bstart[s],
                                           a.posq, a.types, a.items, a.ent, a.cand_j, a.meta,
-                                          a.vdwtab, a.vdwsq, a.ntypes, a.P, a.qfac, a.fsort, a.ss,
-                                          nullptr, no_energy, nullptr);
+                                          a.vdwtab, a.vdwsq, a.ljtype, a.ntypes, a.P, a.qfac,
+                                          a.fsort, a.ss, nullptr, no_energy, nullptr);
 }
 
 template <typename CT>
@@ -310,20 +350,24 @@ static void launch_pair_t(NbSystem &S, const typename Vec2<CT>::T *tab,
                           const typename Vec2<CT>::T *sq, const PairConsts<CT> &P, CT qfac, bool E,
                           bool V, hipStream_t s, bool pruned) {
   auto *f = reinterpret_cast<unsigned long long *>(S.fsort.get());
-  double *part = S.parts.get() + S.part.pair_off;
+  double *part = S.parts_ptr() + S.part.pair_off;
   const int G = S.pair_blocks;
-  const bool lds = S.ntypes <= kMaxTypesLds;
-  const size_t lds_bytes = (size_t)S.ntypes * S.ntypes * sizeof(typename Vec2<CT>::T);
+  const int mode = pair_tab_mode<typename Vec2<CT>::T>(S.ntypes, S.lj_combine);
+  const size_t lds_bytes = pair_tab_lds_bytes<typename Vec2<CT>::T>(mode, S.ntypes);
   const int4 *items = pruned ? S.items_p.get() : S.items.get();
   const uint32_t *ent = pruned ? S.ent_p.get() : S.ent.get();
 #define APO_PAIR_ARGS                                                                   \
-  S.posq.get(), S.type_s.get(), items, ent, S.cand_j.get(), S.meta.get(), tab, sq, S.ntypes, P, \
-      qfac, f, S.ssort, part, S.pc_d, S.vdwtab_d.get()
+  S.posq.get(), S.type_s.get(), items, ent, S.cand_j.get(), S.meta.get(), tab, sq,           \
+      S.ljtype.get(), S.ntypes, P, qfac, f, S.ssort, part, S.pc_d, S.vdwtab_d.get()
 #define APO_PAIR_GO(EE, VV)                                                             \
-  if (lds)                                                                              \
-    k_pair<CT, EE, VV, true><<<G, 64, lds_bytes, s>>>(APO_PAIR_ARGS);                  \
+  if (mode == TAB_SQUARE)                                                               \
+    k_pair<CT, EE, VV, TAB_SQUARE><<<G, 64, lds_bytes, s>>>(APO_PAIR_ARGS);            \
+  else if (mode == TAB_TRI)                                                             \
+    k_pair<CT, EE, VV, TAB_TRI><<<G, 64, lds_bytes, s>>>(APO_PAIR_ARGS);               \
+  else if (mode == TAB_COMBINE)                                                         \
+    k_pair<CT, EE, VV, TAB_COMBINE><<<G, 64, 0, s>>>(APO_PAIR_ARGS);                   \
   else                                                                                  \
-    k_pair<CT, EE, VV, false><<<G, 64, 0, s>>>(APO_PAIR_ARGS);
+    k_pair<CT, EE, VV, TAB_GLOBAL><<<G, 64, 0, s>>>(APO_PAIR_ARGS);
   if (E && V) {
     APO_PAIR_GO(true, true)
   } else if (E) {
@@ -376,18 +420,26 @@ void direct_finalize(NbSystem &S, int64_t *force, int stride, hipStream_t s) {
 void batch_add_pair(BatchLaunch<PairArgs> &b, NbSystem &S, int blocks) {
   if (S.precision != 0) throw InvalidArgument("fused batches run the mixed-precision path");
   PairArgs a{S.posq.get(), S.type_s.get(), S.items.get(), S.ent.get(), S.cand_j.get(),
-             S.meta.get(), S.vdwtab.get(), S.vdwsq.get(), S.ntypes, S.pc_f, (float)kCcelec,
+             S.meta.get(), S.vdwtab.get(), S.vdwsq.get(), S.ljtype.get(), S.ntypes, S.pc_f,
+             (float)kCcelec,
              reinterpret_cast<unsigned long long *>(S.fsort.get()), S.ssort};
-  b.add(a, blocks, (size_t)S.ntypes * S.ntypes * sizeof(float2));
+  // LDS of both LDS layouts: the launch uses the one every system allows
+  b.add(a, blocks, std::max(pair_tab_lds_bytes<float2>(TAB_SQUARE, S.ntypes),
+                            pair_tab_lds_bytes<float2>(TAB_TRI, S.ntypes)));
 }
-void batch_pair(BatchLaunch<PairArgs> &b, bool tab_lds, hipStream_t s) {
+// mode: the table layout every system of the batch allows (pair_tab_mode)
+void batch_pair(BatchLaunch<PairArgs> &b, int mode, hipStream_t s) {
   if (!b.blocks()) return;
-  if (tab_lds)
-    kb_pair<true><<<b.blocks(), 64, b.lds, s>>>(b.d_args.get(), b.d_start.get(), b.nsys());
+  if (mode == TAB_SQUARE)
+    kb_pair<TAB_SQUARE><<<b.blocks(), 64, b.lds, s>>>(b.d_args.get(), b.d_start.get(), b.nsys());
+  else if (mode == TAB_TRI)
+    kb_pair<TAB_TRI><<<b.blocks(), 64, b.lds, s>>>(b.d_args.get(), b.d_start.get(), b.nsys());
+  else if (mode == TAB_COMBINE)
+    kb_pair<TAB_COMBINE><<<b.blocks(), 64, 0, s>>>(b.d_args.get(), b.d_start.get(), b.nsys());
   else
-    kb_pair<false><<<b.blocks(), 64, 0, s>>>(b.d_args.get(), b.d_start.get(), b.nsys());
+    kb_pair<TAB_GLOBAL><<<b.blocks(), 64, 0, s>>>(b.d_args.get(), b.d_start.get(), b.nsys());
   APO_LAUNCH_CHECK();
 }
-bool pair_tab_lds(const NbSystem &S) { return S.ntypes <= kMaxTypesLds; }
+int pair_tab_lds(const NbSystem &S) { return pair_tab_mode<float2>(S.ntypes, S.lj_combine); }
 
 }  // namespace apo

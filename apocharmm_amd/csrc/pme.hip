@@ -790,7 +790,7 @@ static void recip_t(NbSystem &S, const float4 *x, int64_t *force, int stride, bo
   k_ffty<CT, PRIO><<<K2 * P.ncy, P.ft, P.shy, s>>>(cgrid, H, K1, P.wy, plan_of(S.fy), twy, 0);
   APO_LAUNCH_CHECK();
   if (K1 * P.ncz > S.part.recip_n) throw RuntimeError("reciprocal partial buffer too small");
-  double *part = S.parts.get() + S.part.recip_off;
+  double *part = S.parts_ptr() + S.part.recip_off;
   if (EV) {
     launch_fill_f64(part, (size_t)S.part.recip_n * kRecipPart, s);
     k_fftz_conv<CT, true, PRIO><<<K1 * P.ncz, P.ft, P.shz, s>>>(cgrid, P.c, plan_of(S.fz), twz,

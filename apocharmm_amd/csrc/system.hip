@@ -310,6 +310,40 @@ NbSystem *create_system(const apo_nb_params &p, const apo_nb_topology &t) {
       S->vdwsq.upload(sq, 0);
       S->vdwsq_d.upload(sqd, 0);
     }
+    // does the (c6, c12) table follow eps_ij = sqrt(eps_i eps_j), Rmin_ij =
+    // Rmin_i/2 + Rmin_j/2 (c6 = 2 eps Rmin^6, c12 = eps Rmin^12)?  Per-type
+    // values from the diagonal: Rmin_ii^6 = 2 c12/c6, eps_i = c6^2 / (4 c12).
+    {
+      const int nt = t.ntypes;
+      std::vector<double> se(nt), rh(nt);
+      bool ok = true;
+      for (int a = 0; a < nt && ok; a++) {
+        const double c6 = t.vdwtab[2 * (a * (a + 1) / 2 + a)];
+        const double c12 = t.vdwtab[2 * (a * (a + 1) / 2 + a) + 1];
+        if (c6 == 0.0 && c12 == 0.0) {
+          se[a] = 0.0;
+          rh[a] = 0.0;
+        } else if (c6 > 0.0 && c12 > 0.0) {
+          se[a] = std::sqrt(c6 * c6 / (4.0 * c12));
+          rh[a] = 0.5 * std::pow(2.0 * c12 / c6, 1.0 / 6.0);
+        } else {
+          ok = false;
+        }
+      }
+      for (int a = 0; a < nt && ok; a++)
+        for (int b = 0; b <= a && ok; b++) {
+          const double e = se[a] * se[b], R = rh[a] + rh[b], R6 = std::pow(R, 6);
+          const double c6 = 2.0 * e * R6, c12 = e * R6 * R6;
+          const double t6 = t.vdwtab[2 * (a * (a + 1) / 2 + b)],
+                       t12 = t.vdwtab[2 * (a * (a + 1) / 2 + b) + 1];
+          ok = std::fabs(c6 - t6) <= 1e-6 * std::fabs(t6) + 1e-30 &&
+               std::fabs(c12 - t12) <= 1e-6 * std::fabs(t12) + 1e-30;
+        }
+      S->lj_combine = ok;
+      std::vector<float2> lt(nt);
+      for (int a = 0; a < nt; a++) lt[a] = make_float2((float)se[a], (float)rh[a]);
+      S->ljtype.upload(lt, 0);
+    }
     S->vdwtab.upload(tab, 0);
     S->vdwtab14.upload(tab14, 0);
     S->vdwtab_d.upload(tabd, 0);

@@ -41,6 +41,11 @@ struct NbSystem {
   DevBuf<double2> vdwtab14_d;
   DevBuf<float2> vdwsq;       // square (ntypes x ntypes) copy for <= 32 types
   DevBuf<double2> vdwsq_d;
+  // per-type (sqrt(eps), Rmin/2) when every table entry follows the CHARMM
+  // combination rule (no NBFIX among the types present): the pair kernel
+  // then combines per pair instead of reading a table too large for LDS
+  bool lj_combine = false;
+  DevBuf<float2> ljtype;
   DevBuf<int> excl_ptr, excl_idx;
   int nin14 = 0, nex14 = 0;
   DevBuf<int2> in14, ex14;
@@ -121,6 +126,12 @@ struct NbSystem {
   // ---- energies / virial --------------------------------------------------------
   PartLayout part;
   DevBuf<double> parts;
+  // when set, the partials live here instead (the no-allocation entry point
+  // lends its static device array for the length of one call)
+  double *parts_ext = nullptr;
+  double *parts_ptr() const { return parts_ext ? parts_ext : parts.get(); }
+  // false: list builds must fit the current capacities (no allocation)
+  bool may_allocate = true;
 
   // calc_force runs PME and the bonded/1-4 terms on two side streams that
   // fork from and join back into the caller's stream

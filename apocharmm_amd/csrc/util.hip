@@ -139,7 +139,7 @@ __global__ __launch_bounds__(256) void k_self(const float4 *xyzq, int n, double 
 }
 
 void self_launch(NbSystem &S, const float4 *x, hipStream_t s) {
-  k_self<<<S.part.self_n, 256, 0, s>>>(x, S.natom, S.parts.get() + S.part.self_off);
+  k_self<<<S.part.self_n, 256, 0, s>>>(x, S.natom, S.parts_ptr() + S.part.self_off);
   APO_LAUNCH_CHECK();
 }
 
@@ -215,7 +215,7 @@ void reduce_energy_launch(NbSystem &S, int terms, bool E, bool V, double *energy
   a.V = V;
   a.acc = accumulate ? 1 : 0;
   a.self_fac = -S.kappa * kCcelec / std::sqrt(kPi);
-  k_reduce_energy<<<1, 256, 0, s>>>(S.parts.get(), a, energy, virial);
+  k_reduce_energy<<<1, 256, 0, s>>>(S.parts_ptr(), a, energy, virial);
   APO_LAUNCH_CHECK();
 }
 

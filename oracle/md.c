@@ -216,11 +216,12 @@ int orc_shake_water(int nw, const int *w, const double *geom, const double *xref
  * squared length r2[k], displacements along the reference bonds weighted by
  * the inverse masses.  Classic iterative SHAKE (Ryckaert, Ciccotti and
  * Berendsen 1977): bonds corrected one after another until every
- * |r^2 - d^2| <= tol d^2.  Returns the most iterations a group used, -1 if
- * one did not converge in maxit sweeps. */
+ * |r^2 - d^2| <= tol d^2 (FP64 rounding of box-scale coordinates limits tol to
+ * ~1e-14).  Every group is processed; returns the most iterations a group
+ * used, -1 if any did not converge in maxit sweeps. */
 int orc_shake_groups(int ng, const int *g, const double *r2, const double *invmass,
                      const double *xref, double *x, double tol, int maxit) {
-  int worst = 0;
+  int worst = 0, failed = 0;
   for (int k = 0; k < ng; k++) {
     const int h0 = g[4 * k];
     int it = 0;
@@ -243,10 +244,10 @@ int orc_shake_groups(int ng, const int *g, const double *r2, const double *invma
       }
       if (done) break;
     }
-    if (it == maxit) return -1;
+    if (it == maxit) failed = 1;
     if (it > worst) worst = it;
   }
-  return worst;
+  return failed ? -1 : worst;
 }
 
 /* SETTLE of the waters and SHAKE of the hydrogen groups, to FP64 rounding. */
@@ -254,7 +255,7 @@ static void constrain_all(int nw, const int *w, const double *geom, int ns, cons
                           const double *sr2, const double *invmass, const double *xref,
                           double *x) {
   orc_settle(nw, w, geom, xref, x);
-  if (ns > 0) orc_shake_groups(ns, sg, sr2, invmass, xref, x, 1e-15, 1000);
+  if (ns > 0) orc_shake_groups(ns, sg, sr2, invmass, xref, x, 1e-13, 1000);
 }
 
 /* First half of CudaLangevinThermostatIntegrator::initialize

@@ -385,7 +385,7 @@ class Langevin:
         return ke
 
 
-def shake_groups(groups, r2, invmass, xref, x, tol=1e-15, maxit=1000):
+def shake_groups(groups, r2, invmass, xref, x, tol=1e-13, maxit=1000):
     """Iterative SHAKE of hydrogen groups (orc_shake_groups) on (n, 3) x in
     place; returns the sweeps used (-1: not converged)."""
     g = np.ascontiguousarray(groups, np.int32).reshape(-1)

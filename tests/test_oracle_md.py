@@ -235,3 +235,29 @@ def test_langevin_step_with_shake_keeps_bond_lengths(oracle):
         for k in range(1, 4):
             assert abs(np.sum((x[0] - x[k]) ** 2) - r2[0]) <= 1e-12 * r2[0]
         assert abs(np.linalg.norm(x[4] - x[5]) - 0.9572) < 1e-10
+
+
+def test_shake_processes_every_group_when_one_misses_tol(oracle):
+    """A tolerance below FP64 rounding makes the first group miss it; the
+    solver still constrains every later group (it returns -1 and carries on,
+    rather than leaving them as they were)."""
+    rng = np.random.default_rng(11)
+    cases = [_shake_case(rng, m) for m in (3, 1, 2)]
+    n0 = 0
+    refs, news, invms, groups = [], [], [], []
+    for ref, new, invm, g, r2 in cases:
+        m = len(ref) - 1
+        refs.append(ref + 40.0)  # box-scale coordinates: rounding ~1e-14 of r^2
+        news.append(new + 40.0)
+        invms.append(invm)
+        groups.append([n0 + k for k in range(m + 1)] + [-1] * (3 - m))
+        n0 += m + 1
+    ref, new, invm = np.vstack(refs), np.vstack(news), np.concatenate(invms)
+    g = np.array(groups, np.int32)
+    r2 = np.full(len(g), 1.09 ** 2)
+    x = new.copy()
+    assert oracle.shake_groups(g, r2, invm, ref, x, tol=1e-17, maxit=50) == -1
+    for grp in g:
+        for k in range(1, 4):
+            if grp[k] >= 0:
+                assert abs(np.sum((x[grp[0]] - x[grp[k]]) ** 2) / r2[0] - 1) < 1e-12
