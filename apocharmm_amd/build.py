@@ -20,7 +20,7 @@ OBJ = os.path.join(HERE, "_build")
 CPP_CALLER = os.path.join(OBJ, "fm_plugin")
 
 SOURCES = ["system.hip", "util.hip", "nblist.hip", "pair.hip", "bonded.hip", "pme.hip", "md.hip",
-           "capi.cpp", "charmm_files.cpp", "capi_charmm.cpp"]
+           "capi.cpp", "charmm_files.cpp", "capi_charmm.cpp", "noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

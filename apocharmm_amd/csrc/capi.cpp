@@ -15,6 +15,8 @@ NbSystem *create_system(const apo_nb_params &p, const apo_nb_topology &t);
 void set_box(NbSystem &S, const double *b);
 void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int stride,
                 double *energy, double *virial, hipStream_t s);
+void calc_force_noalloc(NbSystem &S, const float *xyzq, int flags, int64_t *force, int stride,
+                        double *energy, double *virial, hipStream_t s);
 }  // namespace apo
 
 namespace apo {
@@ -120,6 +122,18 @@ APO_API apo_status apo_nb_calc_force(apo_nb_system *sys, const float *d_xyzq, in
     if (flags & ~known) throw apo::InvalidArgument("unknown calc_force flag bits");
     apo::calc_force(get(sys), d_xyzq, flags, d_force, stride, d_energy, d_virial,
                     reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+APO_API apo_status apo_nb_calc_force_noalloc(apo_nb_system *sys, const float *d_xyzq,
+                                             int32_t flags, int64_t *d_force, int32_t stride,
+                                             double *d_energy, double *d_virial, void *stream) {
+  return guard([&] {
+    constexpr int32_t known =
+        APO_CALC_ENERGY | APO_CALC_VIRIAL | APO_RESET_LIST | APO_ACCUMULATE | APO_TERM_ALL;
+    if (flags & ~known) throw apo::InvalidArgument("unknown calc_force flag bits");
+    apo::calc_force_noalloc(get(sys), d_xyzq, flags, d_force, stride, d_energy, d_virial,
+                            reinterpret_cast<hipStream_t>(stream));
   });
 }
 

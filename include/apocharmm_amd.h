@@ -193,6 +193,21 @@ APO_API apo_status apo_nb_calc_force(apo_nb_system *sys, const float *d_xyzq,
                                      int32_t stride, double *d_energy,
                                      double *d_virial, void *stream);
 
+/* apo_nb_calc_force for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops); no counterpart in the
+ * reference.  Same arguments and the same results, bit for bit.  The
+ * energy/virial partial sums go to a static device array of 1 MiB per GPU
+ * (131,072 doubles, held until the process ends) instead of the system's own
+ * buffer, and a list rebuild never grows the list buffers (a rebuild that
+ * would need more room is reported at the next call, as by
+ * apo_nb_calc_force).  APO_INVALID_ARGUMENT when the system's partials need
+ * more than the array, when its list was never built (build it once with
+ * apo_nb_reset_list or apo_nb_calc_force) or with kernel timers on.  Calls
+ * that overlap on one GPU share the array and race. */
+APO_API apo_status apo_nb_calc_force_noalloc(apo_nb_system *sys, const float *d_xyzq,
+                                             int32_t flags, int64_t *d_force, int32_t stride,
+                                             double *d_energy, double *d_virial, void *stream);
+
 /* Plugin forces (ForceManager::subscribe / ForceView, include/ForceManager.h:
  * 37-105, 427-444): the manager adds each subscribed force's int64 gradient
  * into the total, d_dst[k] += d_src[k] for the three strided rows of natom
