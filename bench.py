@@ -193,7 +193,11 @@ def check_list(lt, fm, spec, prune_every):
             "pairs_inside_roff": inside,
             "pairs_inside_roff_beyond_list_cutoff_at_build": missed,
             "strict_displacement_bound": bool(info["max_displacement"] <= info["half_buffer"]),
-            "valid": bool(missed == 0 and rel <= 2e-6)}
+            # the reference's fixed 20-step cadence can miss a pair whose atoms
+            # closed in by more than the 2 A buffer; list_policy 1 (the
+            # displacement check) rebuilds before that happens
+            "no_missed_pairs": bool(missed == 0),
+            "grad_within_2e-6_of_fresh_list": bool(rel <= 2e-6)}
 
 
 def missed_pairs(L, x_eval, x_build, r_off, r_list):
