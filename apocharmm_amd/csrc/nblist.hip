@@ -426,10 +426,18 @@ __global__ __launch_bounds__(64 * kGroupsPerCell) void k_prune(
   }
   const unsigned long long below = (1ull << lane) - 1ull;
   int run = 0;
+  // the next chunk's entry and j coordinates load while this one is tested
+  auto jpos = [&](uint32_t e) { return posq[tiles[(e & 0xffffu) >> 5] * kCell + (int)(e & 31u)]; };
+  uint32_t e_n = st < en ? ent[st + lane] : 0u;
+  int4 pj_n = st < en ? jpos(e_n) : make_int4(0, 0, 0, 0);
   for (int p0 = st; p0 < en; p0 += 64) {  // en - st is a multiple of 64
-    const uint32_t e = ent[p0 + lane];
+    const uint32_t e = e_n;
+    const int4 pj = pj_n;
+    if (p0 + 64 < en) {
+      e_n = ent[p0 + 64 + lane];
+      pj_n = jpos(e_n);
+    }
     uint32_t m = e >> 16;
-    const int4 pj = posq[tiles[(e & 0xffffu) >> 5] * kCell + (int)(e & 31u)];
 #pragma unroll
     for (int k = 0; k < kGroup; k++) {
       const float dx = (float)(int)(pix[k] - (unsigned)pj.x) * unit.x;
