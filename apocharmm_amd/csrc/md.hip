@@ -20,6 +20,7 @@
 // list cycle per graph where the schedule allows, else one step per graph.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "apocharmm_amd.h"
@@ -163,32 +164,43 @@ __device__ inline void settle(const SettleGeom &G, double3 A0, double3 B0, doubl
   C1 = frame_to_lab(com, ex, ey, ez, -Xb2 * cth - Yc2 * sth, -Xb2 * sth + Yc2 * cth, Zc1);
 }
 
-// SHAKE for one group: a heavy atom (slot 0) and m = 1..3 hydrogens bonded to
+// SHAKE for one group: a heavy atom (slot 0) and M = 1..3 hydrogens bonded to
 // it, every bond held at squared length r2 (the reference's Shake2/3/4Kernel,
 // src/CudaHolonomicConstraint.cu:324-520, solve the same constraints).  The
 // constraint displacements lie along the reference bonds d_k = ref0 - refk:
-// x0 += im0 sum_k l_k d_k, xk -= imk l_k d_k.  The m equations
+// x0 += im0 sum_k l_k d_k, xk -= imk l_k d_k.  The M equations
 // |x0 - xk|^2 = r2 are solved for l by Newton's method in FP64 (Jacobian
 // J_kj = 2 p_k . (im0 d_j + [k == j] imk d_k), p_k = x0 - xk), to a relative
-// residual of 1e-14 or 30 iterations.  Momentum is conserved by construction.
-__device__ inline void shake_group(int m, double r2, const double im[4], const double3 ref[4],
-                                   double3 x[4]) {
-  double3 d[3];
-  for (int k = 0; k < m; k++) d[k] = ref[0] - ref[k + 1];
-  double lam[3] = {0.0, 0.0, 0.0};
+// residual of 1e-14 or 30 iterations; each Newton system by Gaussian
+// elimination with partial pivoting (row swaps).  M is a template argument so
+// that every array index is known at compile time and the arrays stay in
+// registers.  Momentum is conserved by construction.
+template <int M>
+__device__ inline void shake_group(double r2, const double (&im)[M + 1],
+                                   const double3 (&ref)[M + 1], double3 (&x)[M + 1]) {
+  double3 d[M];
+#pragma unroll
+  for (int k = 0; k < M; k++) d[k] = ref[0] - ref[k + 1];
+  double lam[M];
+#pragma unroll
+  for (int k = 0; k < M; k++) lam[k] = 0.0;
   const double3 x0 = x[0];
-  double3 xs[4] = {x[0], x[1], x[2], x[3]};
+  double3 xs[M + 1];
+#pragma unroll
+  for (int k = 0; k <= M; k++) xs[k] = x[k];
   for (int it = 0; it < 30; it++) {
     // current positions from the multipliers
     double3 c0 = x0;
-    for (int k = 0; k < m; k++) {
+#pragma unroll
+    for (int k = 0; k < M; k++) {
       c0.x += im[0] * lam[k] * d[k].x;
       c0.y += im[0] * lam[k] * d[k].y;
       c0.z += im[0] * lam[k] * d[k].z;
     }
-    double3 p[3];
-    double f[3], worst = 0.0;
-    for (int k = 0; k < m; k++) {
+    double3 p[M];
+    double f[M], worst = 0.0;
+#pragma unroll
+    for (int k = 0; k < M; k++) {
       const double3 ck = make_double3(x[k + 1].x - im[k + 1] * lam[k] * d[k].x,
                                       x[k + 1].y - im[k + 1] * lam[k] * d[k].y,
                                       x[k + 1].z - im[k + 1] * lam[k] * d[k].z);
@@ -199,9 +211,12 @@ __device__ inline void shake_group(int m, double r2, const double im[4], const d
     }
     xs[0] = c0;
     if (worst <= 1e-14 * r2) break;
-    double J[3][3];
-    for (int k = 0; k < m; k++)
-      for (int j = 0; j < m; j++) {
+    double J[M][M], b[M];
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+      b[k] = -f[k];
+#pragma unroll
+      for (int j = 0; j < M; j++) {
         double3 t = make_double3(im[0] * d[j].x, im[0] * d[j].y, im[0] * d[j].z);
         if (k == j) {
           t.x += im[k + 1] * d[k].x;
@@ -210,31 +225,53 @@ __device__ inline void shake_group(int m, double r2, const double im[4], const d
         }
         J[k][j] = 2.0 * dot3(p[k], t);
       }
-    // solve J dl = -f (Gaussian elimination with partial pivoting, m <= 3)
-    double b[3] = {-f[0], -f[1], -f[2]};
-    int piv[3] = {0, 1, 2};
-    for (int c = 0; c < m; c++) {
-      int best = c;
-      for (int r = c + 1; r < m; r++)
-        if (fabs(J[piv[r]][c]) > fabs(J[piv[best]][c])) best = r;
-      const int t = piv[c];
-      piv[c] = piv[best];
-      piv[best] = t;
-      for (int r = c + 1; r < m; r++) {
-        const double g = J[piv[r]][c] / J[piv[c]][c];
-        for (int j = c; j < m; j++) J[piv[r]][j] -= g * J[piv[c]][j];
-        b[piv[r]] -= g * b[piv[c]];
+    }
+    // J dl = -f: for each column the row with the largest magnitude is swapped
+    // into the pivot position, then the rows below are eliminated
+#pragma unroll
+    for (int c = 0; c < M; c++) {
+#pragma unroll
+      for (int r = c + 1; r < M; r++)
+        if (fabs(J[r][c]) > fabs(J[c][c])) {
+#pragma unroll
+          for (int j = 0; j < M; j++) {
+            const double t = J[r][j];
+            J[r][j] = J[c][j];
+            J[c][j] = t;
+          }
+          const double t = b[r];
+          b[r] = b[c];
+          b[c] = t;
+        }
+#pragma unroll
+      for (int r = c + 1; r < M; r++) {
+        const double g = J[r][c] / J[c][c];
+#pragma unroll
+        for (int j = c; j < M; j++) J[r][j] -= g * J[c][j];
+        b[r] -= g * b[c];
       }
     }
-    double dl[3] = {0.0, 0.0, 0.0};
-    for (int c = m - 1; c >= 0; c--) {
-      double s = b[piv[c]];
-      for (int j = c + 1; j < m; j++) s -= J[piv[c]][j] * dl[j];
-      dl[c] = s / J[piv[c]][c];
+    double dl[M];
+#pragma unroll
+    for (int c = M - 1; c >= 0; c--) {
+      double sum = b[c];
+#pragma unroll
+      for (int j = c + 1; j < M; j++) sum -= J[c][j] * dl[j];
+      dl[c] = sum / J[c][c];
     }
-    for (int k = 0; k < m; k++) lam[k] += dl[k];
+#pragma unroll
+    for (int k = 0; k < M; k++) lam[k] += dl[k];
   }
-  for (int k = 0; k <= m; k++) x[k] = xs[k];
+#pragma unroll
+  for (int k = 0; k <= M; k++) x[k] = xs[k];
+}
+
+// Calls f(std::integral_constant<int, M>{}) with M the hydrogen count of
+// SHAKE group g (heavy, h1, h2 or -1, h3 or -1).
+template <typename F> __device__ __forceinline__ void with_group_size(const int4 &g, F &&f) {
+  if (g.w >= 0) f(std::integral_constant<int, 3>{});
+  else if (g.z >= 0) f(std::integral_constant<int, 2>{});
+  else f(std::integral_constant<int, 1>{});
 }
 
 struct LtState {
@@ -369,10 +406,10 @@ __device__ __forceinline__ void lt_step_body(int bid, const LtState &S, const Lt
   const int u = bid * kStepBlock + lane;
   const unsigned long long step = stepc[0];
   const bool water = u < S.nwater;
-  const int ng = S.nwater + S.nshake;
-  const bool group = !water && k == 0 && u < ng;
-  const bool single = !water && k == 0 && u >= ng && u < ng + S.nsingle;
-  const int i = water ? S.waters[3 * u + k] : (single ? S.singles[u - ng] : -1);
+  // units: the waters, then the atoms in no water and no SHAKE group (the
+  // SHAKE groups are stepped by k_lt_shake)
+  const bool single = !water && k == 0 && u < S.nwater + S.nsingle;
+  const int i = water ? S.waters[3 * u + k] : (single ? S.singles[u - S.nwater] : -1);
   double3 r = make_double3(0, 0, 0), p = r, xn = r;
   if (i >= 0) {
     r = ld3(S.x, S.n, i);
@@ -381,24 +418,6 @@ __device__ __forceinline__ void lt_step_body(int bid, const LtState &S, const Lt
   }
   double ke0 = 0.0, ke1 = 0.0;
   float d2 = 0.f, s2 = 0.f;
-  if (group) {  // a SHAKE group: one lane steps, constrains and finishes its atoms
-    const int4 g = S.shake[u - S.nwater];
-    const int at[4] = {g.x, g.y, g.z, g.w};
-    const int m = g.w >= 0 ? 3 : (g.z >= 0 ? 2 : 1);
-    double im[4] = {0, 0, 0, 0};
-    double3 ref[4], dp4[4], nw[4];
-    for (int a = 0; a <= m; a++) {
-      im[a] = S.invmass[at[a]];
-      ref[a] = ld3(S.x, S.n, at[a]);
-      dp4[a] = ld3(S.dp, S.n, at[a]);
-      nw[a] = lt_position(S, C, step, at[a], ref[a], dp4[a]);
-    }
-    shake_group(m, S.shake_r2[u - S.nwater], im, ref, nw);
-    for (int a = 0; a <= m; a++) {
-      lt_finish(S, C, at[a], ref[a], nw[a], dp4[a], ke0, ke1);
-      disp_track(S, at[a], ref[a], nw[a], d2, s2);
-    }
-  }
   if (water) {
     old_x[k][0][lane] = r.x;
     old_x[k][1][lane] = r.y;
@@ -470,6 +489,66 @@ __global__ __launch_bounds__(kStepThreads) void kb_lt_step(const LtStepArgs *__r
   lt_step_body(blockIdx.x - bstart[s], a.S, a.C, a.stepc, a.part, a.dpart);
 }
 
+
+// The SHAKE groups of a step: one lane steps, constrains and finishes each
+// group's atoms (heavy atom and 1-3 hydrogens).  Its own launch (one wave per
+// block), so that the registers of the group solve do not limit the
+// occupancy of the water kernel; the blocks' kinetic-energy and displacement
+// partials follow the water kernel's (k_lt_finalize sums both).
+constexpr int kShakeBlock = 64;
+__device__ __forceinline__ void lt_shake_body(int bid, const LtState &S, const LtCoef &C,
+                                              const unsigned long long *__restrict__ stepc,
+                                              double2 *__restrict__ part,
+                                              float2 *__restrict__ dpart) {
+  const int u = bid * kShakeBlock + threadIdx.x;
+  const unsigned long long step = stepc[0];
+  double ke0 = 0.0, ke1 = 0.0;
+  float d2 = 0.f, s2 = 0.f;
+  if (u < S.nshake) {
+    const int4 g = S.shake[u];
+    const double r2 = S.shake_r2[u];
+    with_group_size(g, [&](auto mc) {
+      constexpr int M = decltype(mc)::value;
+      const int at[4] = {g.x, g.y, g.z, g.w};
+      double im[M + 1];
+      double3 ref[M + 1], dp4[M + 1], nw[M + 1];
+#pragma unroll
+      for (int a = 0; a <= M; a++) {
+        im[a] = S.invmass[at[a]];
+        ref[a] = ld3(S.x, S.n, at[a]);
+        dp4[a] = ld3(S.dp, S.n, at[a]);
+        nw[a] = lt_position(S, C, step, at[a], ref[a], dp4[a]);
+      }
+      shake_group<M>(r2, im, ref, nw);
+#pragma unroll
+      for (int a = 0; a <= M; a++) {
+        lt_finish(S, C, at[a], ref[a], nw[a], dp4[a], ke0, ke1);
+        disp_track(S, at[a], ref[a], nw[a], d2, s2);
+      }
+    });
+  }
+  ke0 = wave_sum(ke0);
+  ke1 = wave_sum(ke1);
+  d2 = wave_max(d2);
+  s2 = wave_max(s2);
+  if (threadIdx.x == 0) {
+    part[bid] = make_double2(ke0, ke1);
+    dpart[bid] = make_float2(d2, s2);
+  }
+}
+__global__ __launch_bounds__(kShakeBlock) void k_lt_shake(LtState S, LtCoef C,
+                                                          const unsigned long long *__restrict__ stepc,
+                                                          double2 *__restrict__ part,
+                                                          float2 *__restrict__ dpart) {
+  lt_shake_body(blockIdx.x, S, C, stepc, part, dpart);
+}
+__global__ __launch_bounds__(kShakeBlock) void kb_lt_shake(const LtStepArgs *__restrict__ tab,
+                                                           const int *__restrict__ bstart,
+                                                           int nsys) {
+  const int s = batch_system(bstart, nsys);
+  const LtStepArgs &a = tab[s];
+  lt_shake_body(blockIdx.x - bstart[s], a.S, a.C, a.stepc, a.part, a.dpart);
+}
 
 // Fixed-order kinetic energy sum, temperatures and their running means
 // (src/CudaLangevinThermostatIntegrator.cu:684-701), step counter + 1.
@@ -584,24 +663,29 @@ __global__ __launch_bounds__(kStepBlock) void k_lt_init_constrain(LtState S, LtC
     }
   } else if (u < S.nwater + S.nshake) {
     const int4 g = S.shake[u - S.nwater];
-    const int at[4] = {g.x, g.y, g.z, g.w};
-    const int m = g.w >= 0 ? 3 : (g.z >= 0 ? 2 : 1);
-    double im[4] = {0, 0, 0, 0};
-    double3 o[4], nn[4];
-    for (int a = 0; a <= m; a++) {
-      im[a] = S.invmass[at[a]];
-      o[a] = nn[a] = ld3(S.x, S.n, at[a]);
-    }
-    shake_group(m, S.shake_r2[u - S.nwater], im, o, nn);
-    for (int a = 0; a <= m; a++) {
-      double3 v = ld3(S.v, S.n, at[a]);
-      v.x += (nn[a].x - o[a].x) / C.dt;
-      v.y += (nn[a].y - o[a].y) / C.dt;
-      v.z += (nn[a].z - o[a].z) / C.dt;
-      st3(S.v, S.n, at[a], v);
-      st3(S.x, S.n, at[a], nn[a]);
-      store_xyzq(S, at[a], nn[a]);
-    }
+    const double r2 = S.shake_r2[u - S.nwater];
+    with_group_size(g, [&](auto mc) {
+      constexpr int M = decltype(mc)::value;
+      const int at[4] = {g.x, g.y, g.z, g.w};
+      double im[M + 1];
+      double3 o[M + 1], nn[M + 1];
+#pragma unroll
+      for (int a = 0; a <= M; a++) {
+        im[a] = S.invmass[at[a]];
+        o[a] = nn[a] = ld3(S.x, S.n, at[a]);
+      }
+      shake_group<M>(r2, im, o, nn);
+#pragma unroll
+      for (int a = 0; a <= M; a++) {
+        double3 v = ld3(S.v, S.n, at[a]);
+        v.x += (nn[a].x - o[a].x) / C.dt;
+        v.y += (nn[a].y - o[a].y) / C.dt;
+        v.z += (nn[a].z - o[a].z) / C.dt;
+        st3(S.v, S.n, at[a], v);
+        st3(S.x, S.n, at[a], nn[a]);
+        store_xyzq(S, at[a], nn[a]);
+      }
+    });
   } else if (u < S.nwater + S.nshake + S.nsingle) {
     const int i = S.singles[u - S.nwater - S.nshake];
     store_xyzq(S, i, ld3(S.x, S.n, i));
@@ -631,17 +715,22 @@ __global__ __launch_bounds__(kStepBlock) void k_lt_init_backstep(LtState S, LtCo
     st3(S.dp, S.n, ic, rc - pc);
   } else if (u < S.nwater + S.nshake) {
     const int4 g = S.shake[u - S.nwater];
-    const int at[4] = {g.x, g.y, g.z, g.w};
-    const int m = g.w >= 0 ? 3 : (g.z >= 0 ? 2 : 1);
-    double im[4] = {0, 0, 0, 0};
-    double3 r[4], pb[4];
-    for (int a = 0; a <= m; a++) {
-      im[a] = S.invmass[at[a]];
-      r[a] = ld3(S.x, S.n, at[a]);
-      pb[a] = back(at[a], r[a]);
-    }
-    shake_group(m, S.shake_r2[u - S.nwater], im, r, pb);
-    for (int a = 0; a <= m; a++) st3(S.dp, S.n, at[a], r[a] - pb[a]);
+    const double r2 = S.shake_r2[u - S.nwater];
+    with_group_size(g, [&](auto mc) {
+      constexpr int M = decltype(mc)::value;
+      const int at[4] = {g.x, g.y, g.z, g.w};
+      double im[M + 1];
+      double3 r[M + 1], pb[M + 1];
+#pragma unroll
+      for (int a = 0; a <= M; a++) {
+        im[a] = S.invmass[at[a]];
+        r[a] = ld3(S.x, S.n, at[a]);
+        pb[a] = back(at[a], r[a]);
+      }
+      shake_group<M>(r2, im, r, pb);
+#pragma unroll
+      for (int a = 0; a <= M; a++) st3(S.dp, S.n, at[a], r[a] - pb[a]);
+    });
   } else if (u < S.nwater + S.nshake + S.nsingle) {
     const int i = S.singles[u - S.nwater - S.nshake];
     const double3 r = ld3(S.x, S.n, i);
@@ -755,6 +844,8 @@ struct LtSystem {
   bool use_graph = true;
   double ndof = 0;
   int nwater = 0, nshake = 0, nsingle = 0, nunit_blocks = 0;
+  int nstep_blocks = 0;   // k_lt_step blocks (waters and free atoms)
+  int nshake_blocks = 0;  // k_lt_shake blocks; their partials follow k_lt_step's
   DevBuf<int4> shake;
   DevBuf<double> shake_r2;
   DevBuf<double> x, dp, v, invmass;
@@ -875,9 +966,13 @@ struct LtSystem {
   void update(hipStream_t s) {
     hipEvent_t t0 = nullptr;
     nb->tic(8, s, t0);
-    k_lt_step<<<nunit_blocks, kStepThreads, 0, s>>>(state(true), C, counters.get(), ke_part.get(),
+    k_lt_step<<<nstep_blocks, kStepThreads, 0, s>>>(state(true), C, counters.get(), ke_part.get(),
                                                   dpart.get());
-    k_lt_finalize<<<1, 256, 0, s>>>(ke_part.get(), nunit_blocks, ndof, thermo.get(),
+    if (nshake_blocks)
+      k_lt_shake<<<nshake_blocks, kShakeBlock, 0, s>>>(state(true), C, counters.get(),
+                                                      ke_part.get() + nstep_blocks,
+                                                      dpart.get() + nstep_blocks);
+    k_lt_finalize<<<1, 256, 0, s>>>(ke_part.get(), nstep_blocks + nshake_blocks, ndof, thermo.get(),
                                     counters.get(), dpart.get(), nb->meta.get(), h_disp);
     APO_LAUNCH_CHECK();
     nb->toc(8, s, t0);
@@ -1231,6 +1326,23 @@ LtSystem *create_lt(NbSystem *nb, const apo_lt_params &p, const double *xyz, con
       if (!(sr2[g] > 0)) throw InvalidArgument("SHAKE bond lengths must be positive");
       sg[g] = make_int4(a[0], a[1], a[2], a[3]);
     }
+    // groups ordered by hydrogen count (stable), so that the lanes of a wave
+    // of k_lt_shake mostly take the same solver instantiation
+    {
+      auto nh = [](const int4 &q) { return q.w >= 0 ? 3 : (q.z >= 0 ? 2 : 1); };
+      std::vector<int> order(p.nshake);
+      for (int g = 0; g < p.nshake; g++) order[g] = g;
+      std::stable_sort(order.begin(), order.end(),
+                       [&](int x, int y) { return nh(sg[x]) < nh(sg[y]); });
+      std::vector<int4> sg2(p.nshake);
+      std::vector<double> sr22(p.nshake);
+      for (int g = 0; g < p.nshake; g++) {
+        sg2[g] = sg[order[g]];
+        sr22[g] = sr2[order[g]];
+      }
+      sg.swap(sg2);
+      sr2.swap(sr22);
+    }
     std::vector<int> singles;
     for (int i = 0; i < n; i++)
       if (!in_water[i]) singles.push_back(i);
@@ -1238,6 +1350,8 @@ LtSystem *create_lt(NbSystem *nb, const apo_lt_params &p, const double *xyz, con
     L->nshake = p.nshake;
     L->nsingle = (int)singles.size();
     L->nunit_blocks = nblk(L->nwater + L->nshake + L->nsingle, kStepBlock);
+    L->nstep_blocks = std::max(1, nblk(L->nwater + L->nsingle, kStepBlock));
+    L->nshake_blocks = nblk(L->nshake, kShakeBlock);
     L->com_blocks = nblk(n, kStepBlock);
     // P1: 3N - 3, less 3 per SETTLE water and 1 per SHAKE bond
     // (CharmmContext::useHolonomicConstraints, src/CharmmContext.cu:919-945)
@@ -1288,13 +1402,13 @@ LtSystem *create_lt(NbSystem *nb, const apo_lt_params &p, const double *xyz, con
     L->shake_r2.upload(sr2, s);
     L->gptr.upload(gptr, s);
     L->gatom.upload(gatom, s);
-    L->ke_part.reserve(L->nunit_blocks);
+    L->ke_part.reserve(L->nstep_blocks + L->nshake_blocks);
     L->com_part.reserve(L->com_blocks + 1);
     L->thermo.reserve(8);
     launch_fill_f64(L->thermo.get(), 8, s);
     std::vector<unsigned long long> c0 = {0ull, 0ull, ~0ull};
     L->counters.upload(c0, s);
-    L->dpart.reserve(L->nunit_blocks);
+    L->dpart.reserve(L->nstep_blocks + L->nshake_blocks);
     APO_HIP(hipHostMalloc(reinterpret_cast<void **>(&L->h_disp), 4 * sizeof(double)));
     for (int k = 0; k < 4; k++) L->h_disp[k] = 0.0;
     L->list_policy = p.list_policy;
@@ -1381,7 +1495,7 @@ struct LtBatch {
   // fused step: one launch per kernel type for all systems
   BatchLaunch<SortArgs> sort;
   BatchLaunch<PairArgs> pair;
-  BatchLaunch<LtStepArgs> lstep;
+  BatchLaunch<LtStepArgs> lstep, lshake;
   BatchLaunch<LtFinArgs> lfin;
   PmeBatch *pme = nullptr;
   BondedBatch *bond = nullptr;
@@ -1422,6 +1536,7 @@ struct LtBatch {
     sort.clear();
     pair.clear();
     lstep.clear();
+    lshake.clear();
     lfin.clear();
     pme_batch_clear(*pme);
     bonded_batch_clear(*bond);
@@ -1452,14 +1567,19 @@ struct LtBatch {
       bonded_batch_add(*bond, S, x, g, L->gstride);
       lstep.add(LtStepArgs{L->state(true), L->C, L->counters.get(), L->ke_part.get(),
                            L->dpart.get()},
-                L->nunit_blocks);
-      lfin.add(LtFinArgs{L->ke_part.get(), L->nunit_blocks, L->ndof, L->thermo.get(),
+                L->nstep_blocks);
+      if (L->nshake_blocks)
+        lshake.add(LtStepArgs{L->state(true), L->C, L->counters.get(),
+                              L->ke_part.get() + L->nstep_blocks, L->dpart.get() + L->nstep_blocks},
+                   L->nshake_blocks);
+      lfin.add(LtFinArgs{L->ke_part.get(), L->nstep_blocks + L->nshake_blocks, L->ndof, L->thermo.get(),
                          L->counters.get(), L->dpart.get(), S.meta.get(), L->h_disp},
                1);
     }
     sort.upload(m);
     pair.upload(m);
     lstep.upload(m);
+    lshake.upload(m);
     lfin.upload(m);
     pme_batch_upload(*pme, m);
     bonded_batch_upload(*bond, m);
@@ -1468,7 +1588,8 @@ struct LtBatch {
     for (size_t i = 0; i < sys.size(); i++) tab_gens[i] = sys[i]->nb->generation;
   }
 
-  // force path of every system + integrator update: 12 launches in all.
+  // force path of every system + integrator update: 12 launches in all (13
+  // with SHAKE groups).
   // After the sort, the PME chain and the bonded terms fork onto side
   // streams and the pair kernel runs on s; all three write int64 sums, so
   // their overlap changes no bit.
@@ -1476,6 +1597,9 @@ struct LtBatch {
     force_step(s);
     kb_lt_step<<<lstep.blocks(), kStepThreads, 0, s>>>(lstep.d_args.get(), lstep.d_start.get(),
                                                      lstep.nsys());
+    if (lshake.nsys())
+      kb_lt_shake<<<lshake.blocks(), kShakeBlock, 0, s>>>(lshake.d_args.get(),
+                                                         lshake.d_start.get(), lshake.nsys());
     kb_lt_finalize<<<lfin.nsys(), 256, 0, s>>>(lfin.d_args.get());
     APO_LAUNCH_CHECK();
   }

@@ -166,20 +166,23 @@ __device__ __forceinline__ void pair_body(
     // the next step's j data is loaded while the current one computes
     const int *tiles = cand_j + it.w;
     auto slot_of = [&](uint32_t e) { return tiles[(e & 0xffffu) >> 5] * kCell + (int)(e & 31u); };
+    // two-deep pipeline: the entry two steps ahead, the coordinates one step ahead
     uint32_t e_n = ent[it.y + lane];
     int jl_n = slot_of(e_n);
     int4 pj_n = posq[jl_n];
     int tj_n = types[jl_n];
+    uint32_t e_nn = it.y + 64 < it.z ? ent[it.y + 64 + lane] : 0u;
     for (int e0 = it.y; e0 < it.z; e0 += 64) {
       const uint32_t e = e_n;
       const int jl = jl_n;
       const int4 pj = pj_n;
       const int tj = tj_n;
       if (e0 + 64 < it.z) {
-        e_n = ent[e0 + 64 + lane];
+        e_n = e_nn;
         jl_n = slot_of(e_n);
         pj_n = posq[jl_n];
         tj_n = types[jl_n];
+        if (e0 + 128 < it.z) e_nn = ent[e0 + 128 + lane];
       }
       const uint32_t m = e >> 16;
       // i-atoms with no live pair in this step are skipped with a scalar branch
