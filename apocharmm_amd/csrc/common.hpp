@@ -22,10 +22,17 @@ constexpr double kPi = 3.14159265358979323846;
 constexpr int kCell = 32;
 constexpr int kGroup = 16;
 constexpr int kGroupsPerCell = kCell / kGroup;
-constexpr int kItemSteps = 16;
+#ifndef APO_ITEM_STEPS
+#define APO_ITEM_STEPS 16
+#endif
+constexpr int kItemSteps = APO_ITEM_STEPS;
 // a j entry is (candidate tile index within its i-cell << 5 | slot in the
 // j-cell) in the low 16 bits and the 16-bit skip mask in the high bits
 constexpr int kMaxCellTiles = 2048;
+// Types up to which the pair kernel reads the switched vdW coefficients
+// (P, Q, A12) of both regions from an LDS table: 2 x 11^2 float4 = 3.9 KB,
+// small enough for 32 one-wave blocks per CU.
+constexpr int kMaxTypesPqa = 11;
 // Pair-kernel grid: blocks (one wave each) per CU.  The kernel loops over
 // work items, so this only sets how many waves each SIMD can interleave.
 #ifndef APO_PAIR_BLOCKS_PER_CU
@@ -182,6 +189,9 @@ template <typename A> struct BatchLaunch {
 // Switching constants follow src/CudaPMEDirectForce.cu:321-331.
 template <typename CT> struct PairConsts {
   CT sx, sy, sz;          // box / 2^32: fixed-point coordinate unit
+  // cubic boxes: the pair kernel forms r^2 in fixed-point units and converts
+  // once (s2 = sx^2); roff2u = roff^2 in those units
+  CT s2, roff2u;
   CT roff2, ron2, rlist2;
   CT k6, k12, dv6, dv12, roffinv3, roffinv6;
   CT kappa, kappa2, k2log2e, two_kappa_sqrtpi;
@@ -196,6 +206,7 @@ struct PairArgs {
   const int *cand_j, *meta;
   const float2 *vdwtab, *vdwsq;
   const float2 *ljtype;  // per-type (sqrt(eps), Rmin/2) when the table follows the rule
+  const float4 *vdwpqa;  // switched vdW coefficients (P, Q, A12) per type pair, few types
   int ntypes;
   PairConsts<float> P;
   float qfac;

@@ -37,8 +37,9 @@ void batch_add_sort(BatchLaunch<SortArgs> &b, NbSystem &S, const float4 *x, int6
                     size_t nforce);
 void batch_sort(BatchLaunch<SortArgs> &b, hipStream_t s);
 void batch_add_pair(BatchLaunch<PairArgs> &b, NbSystem &S, int blocks);
-void batch_pair(BatchLaunch<PairArgs> &b, int tab_mode, hipStream_t s);
+void batch_pair(BatchLaunch<PairArgs> &b, int tab_mode, bool iso, hipStream_t s);
 int pair_tab_lds(const NbSystem &S);
+bool pair_tab_iso(const NbSystem &S);
 struct PmeBatch;
 PmeBatch *pme_batch_create();
 void pme_batch_destroy(PmeBatch *b);
@@ -1500,6 +1501,7 @@ struct LtBatch {
   PmeBatch *pme = nullptr;
   BondedBatch *bond = nullptr;
   int tab_mode = 1;  // vdW table layout of the fused pair launch (pair.hip TAB_*)
+  bool tab_iso = false;  // every system's box is cubic (fixed-point-unit distances)
   std::vector<unsigned long long> tab_gens;
   // launch shapes (the autotuner's choices): pair-kernel blocks per CU
   // shared out by atom count, FFT threads, row pairs and columns per block
@@ -1552,16 +1554,19 @@ struct LtBatch {
       batch_add_sort(sort, S, x, g, 3 * (size_t)L->gstride);
       const int pb = std::max(8, (int)std::lround(pair_bpc * ncu * (L->n / ntot)));
       batch_add_pair(pair, S, pb);
-      // one layout for all: square LDS (1) only if every system allows it,
-      // global (0) if any system needs it, else triangular LDS (2)
+      // one layout for all: a layout every system allows (pair.hip TAB_*:
+      // 4 switched-coefficient LDS table, 1 square LDS, 2 triangular LDS,
+      // 0 global), the global table if any system needs it
       const int m = pair_tab_lds(S);
       if (i_sys == 0) tab_mode = m;
       else if (m != tab_mode) {
         // mixed layouts: the triangular table fits every LDS-capable system;
         // any system beyond it sends the launch to the global table
-        const bool lds_ok = (m == 1 || m == 2) && (tab_mode == 1 || tab_mode == 2);
+        const bool lds_ok = (m == 1 || m == 2 || m == 4) && (tab_mode == 1 || tab_mode == 2 ||
+                                                             tab_mode == 4);
         tab_mode = lds_ok ? 2 : 0;
       }
+      tab_iso = (i_sys == 0 || tab_iso) && pair_tab_iso(S);
       i_sys++;
       pme_batch_add(*pme, S, g, L->gstride, fft_rows, fft_cols, fft_threads);
       bonded_batch_add(*bond, S, x, g, L->gstride);
@@ -1615,7 +1620,7 @@ struct LtBatch {
     APO_HIP(hipStreamWaitEvent(side[1], sort_ev, 0));
     bonded_batch_launch(*bond, side[1]);
     APO_HIP(hipEventRecord(side_ev[1], side[1]));
-    batch_pair(pair, tab_mode, s);
+    batch_pair(pair, tab_mode, tab_iso, s);
     APO_HIP(hipStreamWaitEvent(s, side_ev[0], 0));
     APO_HIP(hipStreamWaitEvent(s, side_ev[1], 0));
     // the sorted pair gradient is added by the step kernel (kb_lt_step)

@@ -91,13 +91,19 @@ constexpr size_t kMaxTriLdsBytes = 32768;
 // holds, every entry following the combination rule): 6 c6 = 12 e R^6 and
 // 12 c12 = 12 e R^12 from per-type (sqrt(eps), Rmin/2), e = se_i se_j,
 // R = rh_i + rh_j.
-enum { TAB_GLOBAL = 0, TAB_SQUARE = 1, TAB_TRI = 2, TAB_COMBINE = 3 };
+// TAB_PQA (mixed precision, at most kMaxTypesPqa types): the switched force
+// coefficients (P, Q, A12) of the inner and outer regions per type pair
+// (NbSystem::vdwpqa) in LDS, so one table read and three multiply-adds give
+// the vdW force in either region.
+enum { TAB_GLOBAL = 0, TAB_SQUARE = 1, TAB_TRI = 2, TAB_COMBINE = 3, TAB_PQA = 4 };
 template <typename T2> static int pair_tab_mode(int ntypes, bool combine) {
+  if (sizeof(T2) == sizeof(float2) && ntypes <= kMaxTypesPqa) return TAB_PQA;
   if (ntypes <= kMaxTypesLds) return TAB_SQUARE;
   if ((size_t)ntypes * (ntypes + 1) / 2 * sizeof(T2) <= kMaxTriLdsBytes) return TAB_TRI;
   return combine && sizeof(T2) == sizeof(float2) ? TAB_COMBINE : TAB_GLOBAL;
 }
 template <typename T2> static size_t pair_tab_lds_bytes(int mode, int ntypes) {
+  if (mode == TAB_PQA) return 2 * (size_t)ntypes * ntypes * sizeof(float4);
   return mode == TAB_SQUARE ? (size_t)ntypes * ntypes * sizeof(T2)
                             : (mode == TAB_TRI ? (size_t)ntypes * (ntypes + 1) / 2 * sizeof(T2) : 0);
 }
@@ -109,20 +115,33 @@ template <typename T2> static size_t pair_tab_lds_bytes(int mode, int ntypes) {
 // visited in an unrolled loop, skipped with a scalar branch when no lane has
 // a live pair with them.  i-forces accumulate in 3*kGroup registers and are
 // reduced once per item; j-forces are flushed per step.
-// TAB: where the vdW table is read (TAB_GLOBAL / TAB_SQUARE / TAB_TRI).
-template <typename CT, bool CALC_E, bool CALC_V, int TAB>
+// TAB: where the vdW table is read (TAB_GLOBAL / TAB_SQUARE / TAB_TRI /
+// TAB_COMBINE / TAB_PQA).
+// ISO (cubic box, mixed precision): the coordinate differences stay in
+// fixed-point units, r^2 is compared with the cutoff in those units and
+// converted once per pair, and the forces accumulate per unit and are scaled
+// to A when they are flushed.
+template <typename CT, bool CALC_E, bool CALC_V, int TAB, bool ISO>
 __device__ __forceinline__ void pair_body(
     int bid, int nb, const int4 *__restrict__ posq, const int *__restrict__ types,
     const int4 *__restrict__ items, const uint32_t *__restrict__ ent,
     const int *__restrict__ cand_j, const int *__restrict__ meta,
     const typename Vec2<CT>::T *__restrict__ vdwtab, const typename Vec2<CT>::T *__restrict__ vdwsq,
-    const float2 *__restrict__ ljtype, int ntypes, const PairConsts<CT> &P, CT qfac,
-    unsigned long long *__restrict__ fsort, int ss, double *__restrict__ part,
-    const PairConsts<double> &PE, const double2 *__restrict__ vdwtab_e) {
+    const float2 *__restrict__ ljtype, const float4 *__restrict__ vdwpqa, int ntypes,
+    const PairConsts<CT> &P, CT qfac, unsigned long long *__restrict__ fsort, int ss,
+    double *__restrict__ part, const PairConsts<double> &PE,
+    const double2 *__restrict__ vdwtab_e) {
+  static_assert(!ISO || sizeof(CT) == sizeof(float), "fixed-point-unit distances are FP32 only");
+  static_assert(TAB != TAB_PQA || sizeof(CT) == sizeof(float), "the PQA table is FP32 only");
   using T2 = typename Vec2<CT>::T;
   extern __shared__ char smem_pair[];
   T2 *sh_tab = reinterpret_cast<T2 *>(smem_pair);
-  if (TAB == TAB_SQUARE) {
+  float4 *sh_pqa = reinterpret_cast<float4 *>(smem_pair);
+  const int nt2 = ntypes * ntypes;
+  if (TAB == TAB_PQA) {
+    for (int k = threadIdx.x; k < 2 * nt2; k += 64) sh_pqa[k] = vdwpqa[k];
+    __syncthreads();
+  } else if (TAB == TAB_SQUARE) {
     for (int k = threadIdx.x; k < ntypes * ntypes; k += 64) sh_tab[k] = vdwsq[k];
     __syncthreads();
   } else if (TAB == TAB_TRI) {
@@ -153,7 +172,8 @@ __device__ __forceinline__ void pair_body(
       piz[k] = __builtin_amdgcn_readlane(pil.z, k);
       qi[k] = charge_of<CT>(__builtin_amdgcn_readlane(pil.w, k));
       const int t = __builtin_amdgcn_readlane(til, k);
-      rowi[k] = TAB == TAB_SQUARE ? t * ntypes : t;
+      // TAB_PQA: byte offset of the type's row of (inner, outer) float4 pairs
+      rowi[k] = TAB == TAB_SQUARE ? t * ntypes : (TAB == TAB_PQA ? t * ntypes * 32 : t);
       if (TAB == TAB_COMBINE) {
         sei[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ljl.x), k));
         rhi[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ljl.y), k));
@@ -190,6 +210,7 @@ __device__ __forceinline__ void pair_body(
       for (int off = 32; off > 0; off >>= 1) act |= __shfl_xor(act, off);
       act = __builtin_amdgcn_readfirstlane(act);
       const CT qj = charge_of<CT>(pj.w) * qfac;
+      const int tjb = tj * 32;  // TAB_PQA: byte offset of column tj
       float2 ljj = make_float2(0.f, 0.f);
       if (TAB == TAB_COMBINE) ljj = ljtype[tj];
       CT fjx = 0, fjy = 0, fjz = 0;
@@ -200,56 +221,83 @@ __device__ __forceinline__ void pair_body(
 #pragma unroll
       for (int k = 0; k < kGroup; k++) {
         if (!(act & (1u << k))) continue;
-        const CT dx = (CT)(int)(pix[k] - pj.x) * P.sx;
-        const CT dy = (CT)(int)(piy[k] - pj.y) * P.sy;
-        const CT dz = (CT)(int)(piz[k] - pj.z) * P.sz;
-        const CT r2 = dx * dx + dy * dy + dz * dz;
-        if (!(m & (1u << k)) && r2 < P.roff2) {
+        // ISO: differences in fixed-point units (dx = unit * dxu)
+        const CT dxu = (CT)(int)(pix[k] - pj.x);
+        const CT dyu = (CT)(int)(piy[k] - pj.y);
+        const CT dzu = (CT)(int)(piz[k] - pj.z);
+        const CT dx = ISO ? dxu : dxu * P.sx;
+        const CT dy = ISO ? dyu : dyu * P.sy;
+        const CT dz = ISO ? dzu : dzu * P.sz;
+        const CT r2t = dx * dx + dy * dy + dz * dz;
+        if (!(m & (1u << k)) && r2t < (ISO ? P.roff2u : P.roff2)) {
+          const CT r2 = ISO ? r2t * P.s2 : r2t;
           const CT rinv = rsqrt_ct(r2);
-          const CT r = r2 * rinv;
-          const CT qq = qi[k] * qj;
-          CT u_el;
-          const CT f_el = ewald_pair(r2, r, rinv, qq, P, u_el);
-          T2 c;
-          if (TAB == TAB_SQUARE) {
-            c = sh_tab[rowi[k] + tj];
-          } else if (TAB == TAB_COMBINE) {
-            const float R = rhi[k] + ljj.y, R2 = R * R, R6 = R2 * R2 * R2;
-            const float c6 = 12.0f * sei[k] * ljj.x * R6;
-            c.x = (CT)c6;
-            c.y = (CT)(c6 * R6);
-          } else {
-            const int ti = rowi[k];
-            const int aa = ti > tj ? ti : tj, bb = ti > tj ? tj : ti;
-            c = TAB == TAB_TRI ? sh_tab[aa * (aa + 1) / 2 + bb] : vdwtab[aa * (aa + 1) / 2 + bb];
-          }
           const CT rinv2 = rinv * rinv;
-          const CT rinv3 = rinv * rinv2;
-          const CT rinv6 = rinv3 * rinv3;
-          // force switch: in the switched region (r > r_on) the c6/c12
-          // terms are k6 c6 r^-3 (r^-3 - r_off^-3) and k12 c12 r^-6 (r^-6 -
-          // r_off^-6); selects between per-lane values and an fma with a -1/0
-          // factor avoid copying the switch constants into registers
-          const bool outer = r2 > P.ron2;
-          const CT mo = outer ? (CT)-1 : (CT)0;
-          const CT t6 = fma(mo, P.roffinv3, rinv3), t12 = fma(mo, P.roffinv6, rinv6);
-          const CT c6k = c.x * P.k6, c12k = c.y * P.k12;
-          const CT c6 = outer ? c6k : c.x, c12 = outer ? c12k : c.y;
-          const CT f_vdw = c6 * t6 * rinv3 - c12 * t12 * rinv6;
-          const CT fij = (f_vdw + f_el) * rinv2;
-          const CT gx = fij * dx, gy = fij * dy, gz = fij * dz;
-          fjx -= gx;
-          fjy -= gy;
-          fjz -= gz;
-          fi[3 * k] += gx;
-          fi[3 * k + 1] += gy;
-          fi[3 * k + 2] += gz;
+          const CT qq = qi[k] * qj;
+          CT fij;
+          if constexpr (TAB == TAB_PQA) {
+            // Coulomb: r dU/dr = -qq exp(-k^2 r^2) (2 k / sqrt(pi) + P(t) / r);
+            // vdW: P r^-6 - Q r^-3 - A12 r^-12 with the region's coefficients
+            const CT r = r2 * rinv;
+            const CT e2 = __builtin_amdgcn_exp2f(-P.k2log2e * r2);
+            const CT t = __builtin_amdgcn_rcpf(fmaf(P.erfc_p, r, 1.0f));
+            CT p = P.erfc_c[8];
+#pragma unroll
+            for (int c = 7; c >= 0; c--) p = fmaf(p, t, P.erfc_c[c]);
+            const CT w = fmaf(p, rinv, P.two_kappa_sqrtpi);
+            const CT rinv3 = rinv2 * rinv;
+            const CT rinv6 = rinv3 * rinv3;
+            const CT rinv12 = rinv6 * rinv6;
+            const float4 c = *reinterpret_cast<const float4 *>(
+                reinterpret_cast<const char *>(sh_pqa) + rowi[k] + tjb + (r2 > P.ron2 ? 16 : 0));
+            CT fv = c.x * rinv6;
+            fv = fmaf(-c.y, rinv3, fv);
+            fv = fmaf(-c.z, rinv12, fv);
+            fij = fmaf(-(qq * e2), w, fv) * rinv2;
+          } else {
+            const CT r = r2 * rinv;
+            CT u_el;
+            const CT f_el = ewald_pair(r2, r, rinv, qq, P, u_el);
+            (void)u_el;
+            T2 c;
+            if (TAB == TAB_SQUARE) {
+              c = sh_tab[rowi[k] + tj];
+            } else if (TAB == TAB_COMBINE) {
+              const float R = rhi[k] + ljj.y, R2 = R * R, R6 = R2 * R2 * R2;
+              const float c6 = 12.0f * sei[k] * ljj.x * R6;
+              c.x = (CT)c6;
+              c.y = (CT)(c6 * R6);
+            } else {
+              const int ti = rowi[k];
+              const int aa = ti > tj ? ti : tj, bb = ti > tj ? tj : ti;
+              c = TAB == TAB_TRI ? sh_tab[aa * (aa + 1) / 2 + bb] : vdwtab[aa * (aa + 1) / 2 + bb];
+            }
+            const CT rinv3 = rinv * rinv2;
+            const CT rinv6 = rinv3 * rinv3;
+            // force switch: in the switched region (r > r_on) the c6/c12
+            // terms are k6 c6 r^-3 (r^-3 - r_off^-3) and k12 c12 r^-6 (r^-6 -
+            // r_off^-6); selects between per-lane values and an fma with a -1/0
+            // factor avoid copying the switch constants into registers
+            const bool outer = r2 > P.ron2;
+            const CT mo = outer ? (CT)-1 : (CT)0;
+            const CT t6 = fma(mo, P.roffinv3, rinv3), t12 = fma(mo, P.roffinv6, rinv6);
+            const CT c6k = c.x * P.k6, c12k = c.y * P.k12;
+            const CT c6 = outer ? c6k : c.x, c12 = outer ? c12k : c.y;
+            const CT f_vdw = c6 * t6 * rinv3 - c12 * t12 * rinv6;
+            fij = (f_vdw + f_el) * rinv2;
+          }
+          // ISO: gradients in A^-1 x unit, scaled when flushed
+          fjx = fma(-fij, dx, fjx);
+          fjy = fma(-fij, dy, fjy);
+          fjz = fma(-fij, dz, fjz);
+          fi[3 * k] = fma(fij, dx, fi[3 * k]);
+          fi[3 * k + 1] = fma(fij, dy, fi[3 * k + 1]);
+          fi[3 * k + 2] = fma(fij, dz, fi[3 * k + 2]);
           if (CALC_E) {
             // energies of energy steps in FP64 from the same fixed-point
             // geometry: a pair energy is ~1e2 larger than the sums it
             // cancels into, and FP32 pair arithmetic (~2e-7 relative) would
             // dominate the energy error
-            (void)u_el;
             const double ex = (double)(int)(pix[k] - pj.x) * PE.sx;
             const double ey = (double)(int)(piy[k] - pj.y) * PE.sy;
             const double ez = (double)(int)(piz[k] - pj.z) * PE.sz;
@@ -257,7 +305,8 @@ __device__ __forceinline__ void pair_body(
             const double er = sqrt(er2), erinv = 1.0 / er;
             el += (double)qi[k] * (double)charge_of<CT>(pj.w) * kCcelec * erfc(PE.kappa * er) *
                   erinv;
-            const int ti = TAB == TAB_SQUARE ? rowi[k] / ntypes : rowi[k];
+            const int ti = TAB == TAB_SQUARE ? rowi[k] / ntypes
+                                             : (TAB == TAB_PQA ? rowi[k] / (ntypes * 32) : rowi[k]);
             const int aa = ti > tj ? ti : tj, bb = ti > tj ? tj : ti;
             const double2 ce = vdwtab_e[aa * (aa + 1) / 2 + bb];
             const double er3 = erinv * erinv * erinv, er6 = er3 * er3;
@@ -269,6 +318,7 @@ __device__ __forceinline__ void pair_body(
                   ce.x * (1.0 / 6.0) * (A6 * e6 * e6 + C6);
           }
           if (CALC_V) {
+            const CT gx = fij * dx, gy = fij * dy, gz = fij * dz;
             v0 -= gx * dx;
             v1 -= gx * dy;
             v2 -= gx * dz;
@@ -280,30 +330,32 @@ __device__ __forceinline__ void pair_body(
       }
       // j-forces that stayed zero (no pair inside r_off) cost no memory traffic
       if (fjx != (CT)0 || fjy != (CT)0 || fjz != (CT)0) {
-        atomicAdd(&fsort[jl], fx_round(fjx));
-        atomicAdd(&fsort[ss + jl], fx_round(fjy));
-        atomicAdd(&fsort[2 * (size_t)ss + jl], fx_round(fjz));
+        atomicAdd(&fsort[jl], fx_round(ISO ? fjx * P.sx : fjx));
+        atomicAdd(&fsort[ss + jl], fx_round(ISO ? fjy * P.sx : fjy));
+        atomicAdd(&fsort[2 * (size_t)ss + jl], fx_round(ISO ? fjz * P.sx : fjz));
       }
       if (CALC_E) {
         acc[0] += el;
         acc[1] += ev;
       }
       if (CALC_V) {
-        acc[2] += (double)v0;
-        acc[3] += (double)v1;
-        acc[4] += (double)v2;
-        acc[5] += (double)v3;
-        acc[6] += (double)v4;
-        acc[7] += (double)v5;
+        // ISO: the products are in unit^2 x (energy / A^2)
+        const double vs = ISO ? (double)P.s2 : 1.0;
+        acc[2] += (double)v0 * vs;
+        acc[3] += (double)v1 * vs;
+        acc[4] += (double)v2 * vs;
+        acc[5] += (double)v3 * vs;
+        acc[6] += (double)v4 * vs;
+        acc[7] += (double)v5 * vs;
       }
     }
     group_reduce<CT>(fi, lane);
     if ((lane & 3) == 0) {
       const int il = ibase + (lane >> 2);
       if (fi[0] != (CT)0 || fi[1] != (CT)0 || fi[2] != (CT)0) {
-        atomicAdd(&fsort[il], fx_round(fi[0]));
-        atomicAdd(&fsort[ss + il], fx_round(fi[1]));
-        atomicAdd(&fsort[2 * (size_t)ss + il], fx_round(fi[2]));
+        atomicAdd(&fsort[il], fx_round(ISO ? fi[0] * P.sx : fi[0]));
+        atomicAdd(&fsort[ss + il], fx_round(ISO ? fi[1] * P.sx : fi[1]));
+        atomicAdd(&fsort[2 * (size_t)ss + il], fx_round(ISO ? fi[2] * P.sx : fi[2]));
       }
     }
   }
@@ -320,68 +372,116 @@ __device__ __forceinline__ void pair_body(
   }
 }
 
-template <typename CT, bool CALC_E, bool CALC_V, int TAB>
+template <typename CT, bool CALC_E, bool CALC_V, int TAB, bool ISO>
 __global__ __launch_bounds__(64) void k_pair(
     const int4 *__restrict__ posq, const int *__restrict__ types, const int4 *__restrict__ items,
     const uint32_t *__restrict__ ent, const int *__restrict__ cand_j, const int *__restrict__ meta,
     const typename Vec2<CT>::T *__restrict__ vdwtab, const typename Vec2<CT>::T *__restrict__ vdwsq,
-    const float2 *__restrict__ ljtype, int ntypes, const PairConsts<CT> P, CT qfac,
-    unsigned long long *__restrict__ fsort, int ss, double *__restrict__ part,
-    const PairConsts<double> PE, const double2 *__restrict__ vdwtab_e) {
-  pair_body<CT, CALC_E, CALC_V, TAB>(blockIdx.x, gridDim.x, posq, types, items, ent, cand_j,
-                                         meta, vdwtab, vdwsq, ljtype, ntypes, P, qfac, fsort, ss,
-                                         part, PE,
-                                         vdwtab_e);
+    const float2 *__restrict__ ljtype, const float4 *__restrict__ vdwpqa, int ntypes,
+    const PairConsts<CT> P, CT qfac, unsigned long long *__restrict__ fsort, int ss,
+    double *__restrict__ part, const PairConsts<double> PE, const double2 *__restrict__ vdwtab_e) {
+  pair_body<CT, CALC_E, CALC_V, TAB, ISO>(blockIdx.x, gridDim.x, posq, types, items, ent, cand_j,
+                                          meta, vdwtab, vdwsq, ljtype, vdwpqa, ntypes, P, qfac,
+                                          fsort, ss, part, PE, vdwtab_e);
 }
 
 // Fused launch over several systems: mixed precision, no energy/virial (the
 // MD step).
-template <int TAB>
+template <int TAB, bool ISO>
 __global__ __launch_bounds__(64) void kb_pair(const PairArgs *__restrict__ tab,
                                               const int *__restrict__ bstart, int nsys) {
   const int s = batch_system(bstart, nsys);
   const PairArgs &a = tab[s];
   const PairConsts<double> no_energy{};
-  pair_body<float, false, false, TAB>(blockIdx.x - bstart[s], bstart[s + 1] - bstart[s],
-                                          a.posq, a.types, a.items, a.ent, a.cand_j, a.meta,
-                                          a.vdwtab, a.vdwsq, a.ljtype, a.ntypes, a.P, a.qfac,
-                                          a.fsort, a.ss, nullptr, no_energy, nullptr);
+  pair_body<float, false, false, TAB, ISO>(blockIdx.x - bstart[s], bstart[s + 1] - bstart[s],
+                                           a.posq, a.types, a.items, a.ent, a.cand_j, a.meta,
+                                           a.vdwtab, a.vdwsq, a.ljtype, a.vdwpqa, a.ntypes, a.P,
+                                           a.qfac, a.fsort, a.ss, nullptr, no_energy, nullptr);
+}
+
+struct PairLaunch {
+  const int4 *posq;
+  const int *types;
+  const int4 *items;
+  const uint32_t *ent;
+  const int *cand_j, *meta;
+  const float2 *ljtype;
+  const float4 *vdwpqa;
+  int ntypes;
+  unsigned long long *fsort;
+  int ss;
+  double *part;
+  const double2 *vdwtab_e;
+  int blocks;
+  size_t lds;
+  hipStream_t s;
+};
+
+template <typename CT, bool E, bool V, int TAB, bool ISO>
+static void go_pair(const PairLaunch &L, const typename Vec2<CT>::T *tab,
+                    const typename Vec2<CT>::T *sq, const PairConsts<CT> &P, CT qfac,
+                    const PairConsts<double> &PE) {
+  k_pair<CT, E, V, TAB, ISO><<<L.blocks, 64, L.lds, L.s>>>(
+      L.posq, L.types, L.items, L.ent, L.cand_j, L.meta, tab, sq, L.ljtype, L.vdwpqa, L.ntypes, P,
+      qfac, L.fsort, L.ss, L.part, PE, L.vdwtab_e);
+}
+template <typename CT, bool E, bool V, bool ISO>
+static void go_pair_tab(int mode, const PairLaunch &L, const typename Vec2<CT>::T *tab,
+                        const typename Vec2<CT>::T *sq, const PairConsts<CT> &P, CT qfac,
+                        const PairConsts<double> &PE) {
+  if constexpr (sizeof(CT) == sizeof(float)) {
+    if (mode == TAB_PQA) return go_pair<CT, E, V, TAB_PQA, ISO>(L, tab, sq, P, qfac, PE);
+    if (mode == TAB_COMBINE) return go_pair<CT, E, V, TAB_COMBINE, ISO>(L, tab, sq, P, qfac, PE);
+  }
+  if (mode == TAB_SQUARE) return go_pair<CT, E, V, TAB_SQUARE, ISO>(L, tab, sq, P, qfac, PE);
+  if (mode == TAB_TRI) return go_pair<CT, E, V, TAB_TRI, ISO>(L, tab, sq, P, qfac, PE);
+  go_pair<CT, E, V, TAB_GLOBAL, ISO>(L, tab, sq, P, qfac, PE);
+}
+template <typename CT, bool E, bool V>
+static void go_pair_iso(bool iso, int mode, const PairLaunch &L, const typename Vec2<CT>::T *tab,
+                        const typename Vec2<CT>::T *sq, const PairConsts<CT> &P, CT qfac,
+                        const PairConsts<double> &PE) {
+  if constexpr (sizeof(CT) == sizeof(float))
+    if (iso) return go_pair_tab<CT, E, V, true>(mode, L, tab, sq, P, qfac, PE);
+  go_pair_tab<CT, E, V, false>(mode, L, tab, sq, P, qfac, PE);
+}
+
+// Whether the pair kernel may form distances in fixed-point units: a cubic
+// box (one unit for x, y and z), mixed precision.
+static bool pair_iso(const NbSystem &S) {
+  return S.precision == 0 && S.box[0] == S.box[1] && S.box[1] == S.box[2];
 }
 
 template <typename CT>
 static void launch_pair_t(NbSystem &S, const typename Vec2<CT>::T *tab,
                           const typename Vec2<CT>::T *sq, const PairConsts<CT> &P, CT qfac, bool E,
                           bool V, hipStream_t s, bool pruned) {
-  auto *f = reinterpret_cast<unsigned long long *>(S.fsort.get());
-  double *part = S.parts_ptr() + S.part.pair_off;
-  const int G = S.pair_blocks;
   const int mode = pair_tab_mode<typename Vec2<CT>::T>(S.ntypes, S.lj_combine);
-  const size_t lds_bytes = pair_tab_lds_bytes<typename Vec2<CT>::T>(mode, S.ntypes);
-  const int4 *items = pruned ? S.items_p.get() : S.items.get();
-  const uint32_t *ent = pruned ? S.ent_p.get() : S.ent.get();
-#define APO_PAIR_ARGS                                                                   \
-  S.posq.get(), S.type_s.get(), items, ent, S.cand_j.get(), S.meta.get(), tab, sq,           \
-      S.ljtype.get(), S.ntypes, P, qfac, f, S.ssort, part, S.pc_d, S.vdwtab_d.get()
-#define APO_PAIR_GO(EE, VV)                                                             \
-  if (mode == TAB_SQUARE)                                                               \
-    k_pair<CT, EE, VV, TAB_SQUARE><<<G, 64, lds_bytes, s>>>(APO_PAIR_ARGS);            \
-  else if (mode == TAB_TRI)                                                             \
-    k_pair<CT, EE, VV, TAB_TRI><<<G, 64, lds_bytes, s>>>(APO_PAIR_ARGS);               \
-  else if (mode == TAB_COMBINE)                                                         \
-    k_pair<CT, EE, VV, TAB_COMBINE><<<G, 64, 0, s>>>(APO_PAIR_ARGS);                   \
-  else                                                                                  \
-    k_pair<CT, EE, VV, TAB_GLOBAL><<<G, 64, 0, s>>>(APO_PAIR_ARGS);
-  if (E && V) {
-    APO_PAIR_GO(true, true)
-  } else if (E) {
-    APO_PAIR_GO(true, false)
-  } else if (V) {
-    APO_PAIR_GO(false, true)
-  } else {
-    APO_PAIR_GO(false, false)
-  }
-#undef APO_PAIR_GO
-#undef APO_PAIR_ARGS
+  PairLaunch L{S.posq.get(),
+               S.type_s.get(),
+               pruned ? S.items_p.get() : S.items.get(),
+               pruned ? S.ent_p.get() : S.ent.get(),
+               S.cand_j.get(),
+               S.meta.get(),
+               S.ljtype.get(),
+               S.vdwpqa.get(),
+               S.ntypes,
+               reinterpret_cast<unsigned long long *>(S.fsort.get()),
+               S.ssort,
+               S.parts_ptr() + S.part.pair_off,
+               S.vdwtab_d.get(),
+               S.pair_blocks,
+               pair_tab_lds_bytes<typename Vec2<CT>::T>(mode, S.ntypes),
+               s};
+  const bool iso = pair_iso(S);
+  if (E && V)
+    go_pair_iso<CT, true, true>(iso, mode, L, tab, sq, P, qfac, S.pc_d);
+  else if (E)
+    go_pair_iso<CT, true, false>(iso, mode, L, tab, sq, P, qfac, S.pc_d);
+  else if (V)
+    go_pair_iso<CT, false, true>(iso, mode, L, tab, sq, P, qfac, S.pc_d);
+  else
+    go_pair_iso<CT, false, false>(iso, mode, L, tab, sq, P, qfac, S.pc_d);
   APO_LAUNCH_CHECK();
 }
 
@@ -423,26 +523,41 @@ void direct_finalize(NbSystem &S, int64_t *force, int stride, hipStream_t s) {
 void batch_add_pair(BatchLaunch<PairArgs> &b, NbSystem &S, int blocks) {
   if (S.precision != 0) throw InvalidArgument("fused batches run the mixed-precision path");
   PairArgs a{S.posq.get(), S.type_s.get(), S.items.get(), S.ent.get(), S.cand_j.get(),
-             S.meta.get(), S.vdwtab.get(), S.vdwsq.get(), S.ljtype.get(), S.ntypes, S.pc_f,
-             (float)kCcelec,
+             S.meta.get(), S.vdwtab.get(), S.vdwsq.get(), S.ljtype.get(), S.vdwpqa.get(),
+             S.ntypes, S.pc_f, (float)kCcelec,
              reinterpret_cast<unsigned long long *>(S.fsort.get()), S.ssort};
-  // LDS of both LDS layouts: the launch uses the one every system allows
-  b.add(a, blocks, std::max(pair_tab_lds_bytes<float2>(TAB_SQUARE, S.ntypes),
-                            pair_tab_lds_bytes<float2>(TAB_TRI, S.ntypes)));
+  // LDS of every LDS layout: the launch uses the one every system allows
+  b.add(a, blocks, std::max({pair_tab_lds_bytes<float2>(TAB_SQUARE, S.ntypes),
+                             pair_tab_lds_bytes<float2>(TAB_TRI, S.ntypes),
+                             S.ntypes <= kMaxTypesPqa
+                                 ? pair_tab_lds_bytes<float2>(TAB_PQA, S.ntypes)
+                                 : (size_t)0}));
 }
-// mode: the table layout every system of the batch allows (pair_tab_mode)
-void batch_pair(BatchLaunch<PairArgs> &b, int mode, hipStream_t s) {
-  if (!b.blocks()) return;
-  if (mode == TAB_SQUARE)
-    kb_pair<TAB_SQUARE><<<b.blocks(), 64, b.lds, s>>>(b.d_args.get(), b.d_start.get(), b.nsys());
+template <bool ISO>
+static void batch_pair_t(BatchLaunch<PairArgs> &b, int mode, hipStream_t s) {
+  if (mode == TAB_PQA)
+    kb_pair<TAB_PQA, ISO><<<b.blocks(), 64, b.lds, s>>>(b.d_args.get(), b.d_start.get(), b.nsys());
+  else if (mode == TAB_SQUARE)
+    kb_pair<TAB_SQUARE, ISO><<<b.blocks(), 64, b.lds, s>>>(b.d_args.get(), b.d_start.get(),
+                                                           b.nsys());
   else if (mode == TAB_TRI)
-    kb_pair<TAB_TRI><<<b.blocks(), 64, b.lds, s>>>(b.d_args.get(), b.d_start.get(), b.nsys());
+    kb_pair<TAB_TRI, ISO><<<b.blocks(), 64, b.lds, s>>>(b.d_args.get(), b.d_start.get(), b.nsys());
   else if (mode == TAB_COMBINE)
-    kb_pair<TAB_COMBINE><<<b.blocks(), 64, 0, s>>>(b.d_args.get(), b.d_start.get(), b.nsys());
+    kb_pair<TAB_COMBINE, ISO><<<b.blocks(), 64, 0, s>>>(b.d_args.get(), b.d_start.get(), b.nsys());
   else
-    kb_pair<TAB_GLOBAL><<<b.blocks(), 64, 0, s>>>(b.d_args.get(), b.d_start.get(), b.nsys());
+    kb_pair<TAB_GLOBAL, ISO><<<b.blocks(), 64, 0, s>>>(b.d_args.get(), b.d_start.get(), b.nsys());
+}
+// mode: the table layout every system of the batch allows (pair_tab_mode);
+// iso: every system's box is cubic
+void batch_pair(BatchLaunch<PairArgs> &b, int mode, bool iso, hipStream_t s) {
+  if (!b.blocks()) return;
+  if (iso)
+    batch_pair_t<true>(b, mode, s);
+  else
+    batch_pair_t<false>(b, mode, s);
   APO_LAUNCH_CHECK();
 }
 int pair_tab_lds(const NbSystem &S) { return pair_tab_mode<float2>(S.ntypes, S.lj_combine); }
+bool pair_tab_iso(const NbSystem &S) { return pair_iso(S); }
 
 }  // namespace apo

@@ -92,6 +92,9 @@ void NbSystem::make_pair_consts() {
     pc.sx = (T)(box[0] / s32);
     pc.sy = (T)(box[1] / s32);
     pc.sz = (T)(box[2] / s32);
+    const double u = box[0] / s32;
+    pc.s2 = (T)(u * u);
+    pc.roff2u = (T)(roff * roff / (u * u));
     pc.roff2 = (T)(roff * roff);
     pc.ron2 = (T)(ron * ron);
     pc.rlist2 = (T)(cutnb * cutnb);
@@ -413,6 +416,24 @@ NbSystem *create_system(const apo_nb_params &p, const apo_nb_topology &t) {
     S->imdihe.upload(mkd(t.imdihe, t.imdihe_par, t.nimdihe), 0);
 
     S->make_pair_consts();
+    if (t.ntypes <= kMaxTypesPqa) {
+      // (P, Q, A12) of the inner and outer switch regions per type pair,
+      // formed in FP64 from the (6 c6, 12 c12) table
+      const int nt = t.ntypes;
+      const double r3 = 1.0 / (S->ctofnb * S->ctofnb * S->ctofnb), r6 = r3 * r3;
+      const double k6 = S->pc_d.k6, k12 = S->pc_d.k12;
+      std::vector<float4> pqa(2 * (size_t)nt * nt);
+      for (int a = 0; a < nt; a++)
+        for (int b = 0; b < nt; b++) {
+          const int hi = a > b ? a : b, lo = a > b ? b : a;
+          const double c6 = tabd[hi * (hi + 1) / 2 + lo].x, c12 = tabd[hi * (hi + 1) / 2 + lo].y;
+          const size_t k = 2 * ((size_t)a * nt + b);
+          pqa[k] = make_float4((float)c6, 0.f, (float)c12, 0.f);
+          pqa[k + 1] = make_float4((float)(k6 * c6 + k12 * c12 * r6), (float)(k6 * c6 * r3),
+                                   (float)(k12 * c12), 0.f);
+        }
+      S->vdwpqa.upload(pqa, 0);
+    }
     S->setup_columns();
     S->setup_pme();
 

@@ -46,6 +46,11 @@ struct NbSystem {
   // then combines per pair instead of reading a table too large for LDS
   bool lj_combine = false;
   DevBuf<float2> ljtype;
+  // Switched vdW force in one form for both regions, per type pair (few
+  // types): r dU/dr / r^2 contribution = (P r^-6 - Q r^-3 - A12 r^-12) r^-2,
+  // with (P, Q, A12) = (6c6, 0, 12c12) inside r_on and (k6 6c6 + k12 12c12
+  // r_off^-6, k6 6c6 r_off^-3, k12 12c12) beyond it; [ti][tj][inner | outer]
+  DevBuf<float4> vdwpqa;
   DevBuf<int> excl_ptr, excl_idx;
   int nin14 = 0, nex14 = 0;
   DevBuf<int2> in14, ex14;
