@@ -272,7 +272,11 @@ constexpr int kCalcPrune = 1 << 28;
 // integrator's step kernel does, as it reads each atom's gradient).
 constexpr int kCalcNoFinalize = 1 << 26;
 
-void direct_launch(NbSystem &sys, bool energy, bool virial, hipStream_t s, bool pruned = false);
+// pruned: the pair kernel reads the pruned list copy; prune_out: it reads the
+// build's list and writes the pruned copy for the next steps (fused pruning,
+// mixed-precision MD steps)
+void direct_launch(NbSystem &sys, bool energy, bool virial, hipStream_t s, bool pruned = false,
+                   bool prune_out = false);
 void direct_finalize(NbSystem &sys, int64_t *d_force, int stride,
                      hipStream_t s);
 void pairs14_launch(NbSystem &sys, const float4 *d_xyzq, int64_t *d_force,

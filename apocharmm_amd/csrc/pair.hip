@@ -121,7 +121,16 @@ template <typename T2> static size_t pair_tab_lds_bytes(int mode, int ntypes) {
 // fixed-point units, r^2 is compared with the cutoff in those units and
 // converted once per pair, and the forces accumulate per unit and are scaled
 // to A when they are flushed.
-template <typename CT, bool CALC_E, bool CALC_V, int TAB, bool ISO>
+// PRUNE (list pruning fused into an MD step, mixed precision): the kernel
+// runs on the build's list and also writes the pruned copy the next steps
+// read: each entry gains the skip bits of its pairs at or beyond the prune
+// radius (rp2t, in the units r^2 is compared in), entries left with no live
+// pair are dropped, and each item's surviving entries are packed to the front
+// of its own region (null entries after them, so every entry of the copy
+// stays valid) with the item's end shortened to whole steps.  One wave owns an
+// item, so the packing needs no coordination.  (The idea of pruning the list
+// between builds is GROMACS's dynamic pruning; the reference keeps its list.)
+template <typename CT, bool CALC_E, bool CALC_V, int TAB, bool ISO, bool PRUNE>
 __device__ __forceinline__ void pair_body(
     int bid, int nb, const int4 *__restrict__ posq, const int *__restrict__ types,
     const int4 *__restrict__ items, const uint32_t *__restrict__ ent,
@@ -130,8 +139,11 @@ __device__ __forceinline__ void pair_body(
     const float2 *__restrict__ ljtype, const float4 *__restrict__ vdwpqa, int ntypes,
     const PairConsts<CT> &P, CT qfac, unsigned long long *__restrict__ fsort, int ss,
     double *__restrict__ part, const PairConsts<double> &PE,
-    const double2 *__restrict__ vdwtab_e) {
+    const double2 *__restrict__ vdwtab_e, uint32_t *__restrict__ ent_out,
+    int4 *__restrict__ items_out, CT rp2t) {
   static_assert(!ISO || sizeof(CT) == sizeof(float), "fixed-point-unit distances are FP32 only");
+  static_assert(!PRUNE || (sizeof(CT) == sizeof(float) && !CALC_E && !CALC_V),
+                "fused pruning is an MD-step variant");
   static_assert(TAB != TAB_PQA || sizeof(CT) == sizeof(float), "the PQA table is FP32 only");
   using T2 = typename Vec2<CT>::T;
   extern __shared__ char smem_pair[];
@@ -192,6 +204,7 @@ __device__ __forceinline__ void pair_body(
     int4 pj_n = posq[jl_n];
     int tj_n = types[jl_n];
     uint32_t e_nn = it.y + 64 < it.z ? ent[it.y + 64 + lane] : 0u;
+    int wpos = it.y;  // PRUNE: next free slot of the item's pruned entries
     for (int e0 = it.y; e0 < it.z; e0 += 64) {
       const uint32_t e = e_n;
       const int jl = jl_n;
@@ -205,6 +218,7 @@ __device__ __forceinline__ void pair_body(
         if (e0 + 128 < it.z) e_nn = ent[e0 + 128 + lane];
       }
       const uint32_t m = e >> 16;
+      uint32_t mp = m;  // PRUNE: m plus the pairs at or beyond the prune radius
       // i-atoms with no live pair in this step are skipped with a scalar branch
       uint32_t act = ~m & ((1u << kGroup) - 1u);
       for (int off = 32; off > 0; off >>= 1) act |= __shfl_xor(act, off);
@@ -229,6 +243,7 @@ __device__ __forceinline__ void pair_body(
         const CT dy = ISO ? dyu : dyu * P.sy;
         const CT dz = ISO ? dzu : dzu * P.sz;
         const CT r2t = dx * dx + dy * dy + dz * dz;
+        if (PRUNE) mp |= (r2t < rp2t ? 0u : 1u) << k;
         if (!(m & (1u << k)) && r2t < (ISO ? P.roff2u : P.roff2)) {
           const CT r2 = ISO ? r2t * P.s2 : r2t;
           const CT rinv = rsqrt_ct(r2);
@@ -328,6 +343,13 @@ __device__ __forceinline__ void pair_body(
           }
         }
       }
+      if (PRUNE) {
+        const bool live = e0 + lane < it.z && mp != (1u << kGroup) - 1u;
+        const unsigned long long b = __ballot(live);
+        if (live)
+          ent_out[wpos + __popcll(b & ((1ull << lane) - 1ull))] = (e & 0xffffu) | (mp << 16);
+        wpos += __popcll(b);
+      }
       // j-forces that stayed zero (no pair inside r_off) cost no memory traffic
       if (fjx != (CT)0 || fjy != (CT)0 || fjz != (CT)0) {
         atomicAdd(&fsort[jl], fx_round(ISO ? fjx * P.sx : fjx));
@@ -348,6 +370,12 @@ __device__ __forceinline__ void pair_body(
         acc[6] += (double)v4 * vs;
         acc[7] += (double)v5 * vs;
       }
+    }
+    if (PRUNE) {
+      // null entries (every pair skipped) after the survivors, whole steps kept
+      const int end = it.y + (wpos - it.y + 63) / 64 * 64;
+      for (int p = wpos + lane; p < it.z; p += 64) ent_out[p] = ((1u << kGroup) - 1u) << 16;
+      if (lane == 0) items_out[item] = make_int4(it.x, it.y, end, it.w);
     }
     group_reduce<CT>(fi, lane);
     if ((lane & 3) == 0) {
@@ -372,17 +400,19 @@ __device__ __forceinline__ void pair_body(
   }
 }
 
-template <typename CT, bool CALC_E, bool CALC_V, int TAB, bool ISO>
+template <typename CT, bool CALC_E, bool CALC_V, int TAB, bool ISO, bool PRUNE>
 __global__ __launch_bounds__(64) void k_pair(
     const int4 *__restrict__ posq, const int *__restrict__ types, const int4 *__restrict__ items,
     const uint32_t *__restrict__ ent, const int *__restrict__ cand_j, const int *__restrict__ meta,
     const typename Vec2<CT>::T *__restrict__ vdwtab, const typename Vec2<CT>::T *__restrict__ vdwsq,
     const float2 *__restrict__ ljtype, const float4 *__restrict__ vdwpqa, int ntypes,
     const PairConsts<CT> P, CT qfac, unsigned long long *__restrict__ fsort, int ss,
-    double *__restrict__ part, const PairConsts<double> PE, const double2 *__restrict__ vdwtab_e) {
-  pair_body<CT, CALC_E, CALC_V, TAB, ISO>(blockIdx.x, gridDim.x, posq, types, items, ent, cand_j,
-                                          meta, vdwtab, vdwsq, ljtype, vdwpqa, ntypes, P, qfac,
-                                          fsort, ss, part, PE, vdwtab_e);
+    double *__restrict__ part, const PairConsts<double> PE, const double2 *__restrict__ vdwtab_e,
+    uint32_t *__restrict__ ent_out, int4 *__restrict__ items_out, CT rp2t) {
+  pair_body<CT, CALC_E, CALC_V, TAB, ISO, PRUNE>(blockIdx.x, gridDim.x, posq, types, items, ent,
+                                                 cand_j, meta, vdwtab, vdwsq, ljtype, vdwpqa,
+                                                 ntypes, P, qfac, fsort, ss, part, PE, vdwtab_e,
+                                                 ent_out, items_out, rp2t);
 }
 
 // Fused launch over several systems: mixed precision, no energy/virial (the
@@ -393,10 +423,10 @@ __global__ __launch_bounds__(64) void kb_pair(const PairArgs *__restrict__ tab,
   const int s = batch_system(bstart, nsys);
   const PairArgs &a = tab[s];
   const PairConsts<double> no_energy{};
-  pair_body<float, false, false, TAB, ISO>(blockIdx.x - bstart[s], bstart[s + 1] - bstart[s],
-                                           a.posq, a.types, a.items, a.ent, a.cand_j, a.meta,
-                                           a.vdwtab, a.vdwsq, a.ljtype, a.vdwpqa, a.ntypes, a.P,
-                                           a.qfac, a.fsort, a.ss, nullptr, no_energy, nullptr);
+  pair_body<float, false, false, TAB, ISO, false>(
+      blockIdx.x - bstart[s], bstart[s + 1] - bstart[s], a.posq, a.types, a.items, a.ent,
+      a.cand_j, a.meta, a.vdwtab, a.vdwsq, a.ljtype, a.vdwpqa, a.ntypes, a.P, a.qfac, a.fsort,
+      a.ss, nullptr, no_energy, nullptr, nullptr, nullptr, 0.0f);
 }
 
 struct PairLaunch {
@@ -415,15 +445,29 @@ struct PairLaunch {
   int blocks;
   size_t lds;
   hipStream_t s;
+  // fused pruning: the pruned copy written, and the prune radius squared in
+  // the units the kernel compares r^2 in
+  uint32_t *ent_out;
+  int4 *items_out;
+  float rp2t;
 };
 
 template <typename CT, bool E, bool V, int TAB, bool ISO>
 static void go_pair(const PairLaunch &L, const typename Vec2<CT>::T *tab,
                     const typename Vec2<CT>::T *sq, const PairConsts<CT> &P, CT qfac,
                     const PairConsts<double> &PE) {
-  k_pair<CT, E, V, TAB, ISO><<<L.blocks, 64, L.lds, L.s>>>(
+  if constexpr (sizeof(CT) == sizeof(float) && !E && !V) {
+    if (L.ent_out) {
+      k_pair<CT, false, false, TAB, ISO, true><<<L.blocks, 64, L.lds, L.s>>>(
+          L.posq, L.types, L.items, L.ent, L.cand_j, L.meta, tab, sq, L.ljtype, L.vdwpqa,
+          L.ntypes, P, qfac, L.fsort, L.ss, L.part, PE, L.vdwtab_e, L.ent_out, L.items_out,
+          L.rp2t);
+      return;
+    }
+  }
+  k_pair<CT, E, V, TAB, ISO, false><<<L.blocks, 64, L.lds, L.s>>>(
       L.posq, L.types, L.items, L.ent, L.cand_j, L.meta, tab, sq, L.ljtype, L.vdwpqa, L.ntypes, P,
-      qfac, L.fsort, L.ss, L.part, PE, L.vdwtab_e);
+      qfac, L.fsort, L.ss, L.part, PE, L.vdwtab_e, nullptr, nullptr, (CT)0);
 }
 template <typename CT, bool E, bool V, bool ISO>
 static void go_pair_tab(int mode, const PairLaunch &L, const typename Vec2<CT>::T *tab,
@@ -452,11 +496,15 @@ static bool pair_iso(const NbSystem &S) {
   return S.precision == 0 && S.box[0] == S.box[1] && S.box[1] == S.box[2];
 }
 
+// pruned: read the pruned copy; prune_out: read the build's list and write
+// the pruned copy (mixed-precision MD steps only)
 template <typename CT>
 static void launch_pair_t(NbSystem &S, const typename Vec2<CT>::T *tab,
                           const typename Vec2<CT>::T *sq, const PairConsts<CT> &P, CT qfac, bool E,
-                          bool V, hipStream_t s, bool pruned) {
+                          bool V, hipStream_t s, bool pruned, bool prune_out) {
   const int mode = pair_tab_mode<typename Vec2<CT>::T>(S.ntypes, S.lj_combine);
+  pruned = pruned && !prune_out;
+  const bool iso = pair_iso(S);
   PairLaunch L{S.posq.get(),
                S.type_s.get(),
                pruned ? S.items_p.get() : S.items.get(),
@@ -472,8 +520,12 @@ static void launch_pair_t(NbSystem &S, const typename Vec2<CT>::T *tab,
                S.vdwtab_d.get(),
                S.pair_blocks,
                pair_tab_lds_bytes<typename Vec2<CT>::T>(mode, S.ntypes),
-               s};
-  const bool iso = pair_iso(S);
+               s,
+               prune_out ? S.ent_p.get() : nullptr,
+               prune_out ? S.items_p.get() : nullptr,
+               (float)(iso ? (double)S.prune_r2 / ((S.box[0] / 4294967296.0) *
+                                                   (S.box[0] / 4294967296.0))
+                           : (double)S.prune_r2)};
   if (E && V)
     go_pair_iso<CT, true, true>(iso, mode, L, tab, sq, P, qfac, S.pc_d);
   else if (E)
@@ -485,12 +537,15 @@ static void launch_pair_t(NbSystem &S, const typename Vec2<CT>::T *tab,
   APO_LAUNCH_CHECK();
 }
 
-void direct_launch(NbSystem &S, bool E, bool V, hipStream_t s, bool pruned) {
+void direct_launch(NbSystem &S, bool E, bool V, hipStream_t s, bool pruned, bool prune_out) {
+  if (prune_out && (S.precision != 0 || E || V))
+    throw InvalidArgument("fused list pruning runs in mixed-precision MD steps only");
   if (S.precision == 1)
-    launch_pair_t<double>(S, S.vdwtab_d.get(), S.vdwsq_d.get(), S.pc_d, kCcelec, E, V, s, pruned);
+    launch_pair_t<double>(S, S.vdwtab_d.get(), S.vdwsq_d.get(), S.pc_d, kCcelec, E, V, s, pruned,
+                          false);
   else
     launch_pair_t<float>(S, S.vdwtab.get(), S.vdwsq.get(), S.pc_f, (float)kCcelec, E, V, s,
-                         pruned);
+                         pruned, prune_out);
 }
 
 // Sorted -> global order (addSortedForce, src/CudaPMEDirectForce.cu:1375-1386).

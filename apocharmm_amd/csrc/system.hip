@@ -547,11 +547,15 @@ void calc_force(NbSystem &S, const float *xyzq, int flags, int64_t *force, int s
     APO_HIP(hipStreamWaitEvent(sb, S.ev_fork, 0));
   }
   auto pair_branch = [&] {
-    // a pruned list is refreshed after every build and when asked
+    // a pruned list is refreshed after every build and when asked: by the
+    // pair kernel itself in mixed-precision steps without energies (it tests
+    // every listed pair anyway), else by the separate prune kernel first
     const bool pruned = flags & kCalcUsePruned;
-    if (pruned && (rebuilt || (flags & kCalcPrune))) nblist_prune(S, s);
+    const bool refresh = pruned && (rebuilt || (flags & kCalcPrune));
+    const bool fused = refresh && S.precision == 0 && !E && !V;
+    if (refresh && !fused) nblist_prune(S, s);
     S.tic(0, s, t0);
-    direct_launch(S, E, V, s, pruned);
+    direct_launch(S, E, V, s, pruned, fused);
     S.toc(0, s, t0);
   };
   if (dir && S.pair_first) pair_branch();
