@@ -334,7 +334,8 @@ __global__ __launch_bounds__(64 * kGroupsPerCell) void k_grp_count(const int *__
                                                   const uint32_t *__restrict__ cand_mask,
                                                   int *__restrict__ grp_cnt,
                                                   int *__restrict__ grp_nitem,
-                                                  int *__restrict__ meta_w, int cand_cap) {
+                                                  int *__restrict__ meta_w, int cand_cap,
+                                                  int isteps) {
   constexpr uint32_t kFull = (1u << kGroup) - 1u;
   const int icell = blockIdx.x;
   const int lane = threadIdx.x & 63, l = lane & 31, half = lane >> 5;
@@ -353,7 +354,7 @@ __global__ __launch_bounds__(64 * kGroupsPerCell) void k_grp_count(const int *__
   if (lane == 0) {
     const int c = (cnt + 63) / 64 * 64;
     grp_cnt[icell * kGroupsPerCell + h] = c;
-    grp_nitem[icell * kGroupsPerCell + h] = (c + 64 * kItemSteps - 1) / (64 * kItemSteps);
+    grp_nitem[icell * kGroupsPerCell + h] = (c + 64 * isteps - 1) / (64 * isteps);
   }
 }
 
@@ -363,7 +364,8 @@ __global__ __launch_bounds__(64 * kGroupsPerCell) void k_grp_count(const int *__
 __global__ __launch_bounds__(64 * kGroupsPerCell) void k_grp_emit(
     const int *__restrict__ meta, const int *__restrict__ icell_start,
     const uint32_t *__restrict__ cand_mask, const int *__restrict__ grp_start,
-    const int *__restrict__ grp_itemstart, uint32_t *__restrict__ ent, int4 *__restrict__ items) {
+    const int *__restrict__ grp_itemstart, uint32_t *__restrict__ ent, int4 *__restrict__ items,
+    int isteps) {
   constexpr uint32_t kFull = (1u << kGroup) - 1u;
   const int icell = blockIdx.x;
   if (icell >= meta[0]) return;
@@ -388,8 +390,8 @@ __global__ __launch_bounds__(64 * kGroupsPerCell) void k_grp_emit(
   for (int p = st + run + lane; p < en; p += 64) ent[p] = kFull << 16;
   const int i0 = grp_itemstart[g], i1 = grp_itemstart[g + 1];
   for (int it = i0 + lane; it < i1; it += 64) {
-    const int a = st + (it - i0) * 64 * kItemSteps;
-    items[it] = make_int4(g, a, min(a + 64 * kItemSteps, en), s);
+    const int a = st + (it - i0) * 64 * isteps;
+    items[it] = make_int4(g, a, min(a + 64 * isteps, en), s);
   }
 }
 
@@ -611,7 +613,17 @@ void nblist_build(NbSystem &S, const float4 *x, hipStream_t s, bool sizing) {
     S.host_ncell = hmeta[0];
     S.host_ncand = hmeta[1];
     list_reserve(S, hmeta[1], 0, 0);
+    if (S.item_steps == 0) {
+      // the longest items that still give every other pair-kernel wave one
+      // (measured: 98,304 atoms keep 16 steps, 23,558 atoms run faster on 4);
+      // a candidate tile holds ~32 live entries, half a 64-entry step
+      const double steps = 0.5 * hmeta[1];
+      int k = kItemSteps;
+      while (k > 2 && steps / k < 0.5 * S.pair_blocks) k /= 2;
+      S.item_steps = k;
+    }
   }
+  if (S.item_steps == 0) S.item_steps = kItemSteps;
   const int ccap = (int)S.cand_cap;
   k_tiles<<<S.ncell_cap, 64, 0, s>>>(S.cell_bb.get(), meta, Lf, rl2, nullptr, S.icell_start.get(),
                                      S.cand_i.get(), S.cand_j.get(), 1, ccap);
@@ -622,7 +634,8 @@ void nblist_build(NbSystem &S, const float4 *x, hipStream_t s, bool sizing) {
   APO_LAUNCH_CHECK();
   const int ngrp = S.ncell_cap * kGroupsPerCell;
   k_grp_count<<<S.ncell_cap, 64 * kGroupsPerCell, 0, s>>>(
-      meta, S.icell_start.get(), S.cand_mask.get(), S.grp_cnt.get(), S.grp_nitem.get(), meta, ccap);
+      meta, S.icell_start.get(), S.cand_mask.get(), S.grp_cnt.get(), S.grp_nitem.get(), meta, ccap,
+      S.item_steps);
   APO_LAUNCH_CHECK();
   launch_scan(S.grp_cnt.get(), S.grp_start.get(), nullptr, ngrp, s);
   launch_scan(S.grp_nitem.get(), S.grp_itemstart.get(), nullptr, ngrp, s);
@@ -644,7 +657,7 @@ void nblist_build(NbSystem &S, const float4 *x, hipStream_t s, bool sizing) {
   APO_LAUNCH_CHECK();
   k_grp_emit<<<S.ncell_cap, 64 * kGroupsPerCell, 0, s>>>(meta, S.icell_start.get(), S.cand_mask.get(),
                                         S.grp_start.get(), S.grp_itemstart.get(), S.ent.get(),
-                                        S.items.get());
+                                        S.items.get(), S.item_steps);
   APO_LAUNCH_CHECK();
   k_meta_export<<<1, 64, 0, s>>>(meta, S.h_meta);
   APO_LAUNCH_CHECK();

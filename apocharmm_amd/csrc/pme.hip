@@ -221,10 +221,21 @@ constexpr int kSpreadThreads = APO_SPREAD_THREADS;  // threads per slab block
 // banks in each half wave; a stride that is a multiple of 16 puts all 16 rows
 // of a lane group on the same banks.
 __host__ __device__ __forceinline__ int slab_stride(int K0) { return K0 + ((36 - K0 % 32) % 32); }
+// Atoms whose stencils a wave walks are staged as records in LDS, kSpreadRec
+// per wave at a time: the twelve FP64 B-spline weights (the charge folded into
+// the x weights) and the packed stencil base.
+constexpr int kSpreadRec = 32;
+// Register budget of the spread kernels: five waves per SIMD, so a slab
+// block still finds room beside the pair kernel's waves
+#ifndef APO_SPREAD_WAVES_ATTR
+#define APO_SPREAD_WAVES_ATTR __attribute__((amdgpu_waves_per_eu(5)))
+#endif
+constexpr size_t kSpreadRecBytes = (size_t)kSpreadRec * (12 * sizeof(double) + sizeof(int));
 template <typename C2> static size_t slab_lds(int K0, int by, int bz) {
   const size_t rows = (size_t)by * bz;
   return rows * slab_stride(K0) * sizeof(unsigned long long) + (size_t)K0 * sizeof(C2) +
-         rows * K0 * sizeof(C2) + (kSlabCand + 4) * sizeof(int);
+         rows * K0 * sizeof(C2) + (kSlabCand + 4) * sizeof(int) +
+         (kSpreadThreads / 64) * kSpreadRecBytes;
 }
 
 // Cyclic-interval overlap on a circle of length K (iK = 1/K).  The reduced
@@ -300,6 +311,12 @@ __device__ __forceinline__ void spread_fftx_body(int bid, const int4 *__restrict
   int *cand = reinterpret_cast<int *>(b + (nrow / 2) * n);
   int &ncand = cand[kSlabCand];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // this wave's atom records: x weights times charge, y and z weights (four
+  // stencil points each), packed base (ix | (dy + 3) << 16 | (dz + 3) << 24)
+  double *rtx = reinterpret_cast<double *>(cand + kSlabCand + 4) +
+                (size_t)wave * (kSpreadRecBytes / sizeof(double));
+  double *rty = rtx + 4 * kSpreadRec, *rtz = rty + 4 * kSpreadRec;
+  int *rpos = reinterpret_cast<int *>(rtz + 4 * kSpreadRec);
   const int byi = bid % g.nby, bzi = bid / g.nby;
   const int y0 = byi * by, z0 = bzi * bz;
   for (int k = tid; k < nrow * ss; k += kSpreadThreads) slab[k] = 0ull;
@@ -374,31 +391,53 @@ __device__ __forceinline__ void spread_fftx_body(int bid, const int4 *__restrict
           qs = q * (kGridScale / 216.0);
         }
       }
-      unsigned long long hit = __ballot(dy < by && dz < bz);
-      // two atoms per pass: their weight products are independent chains
-      auto contrib = [&](int k, int &at) -> long long {
-        int px = __builtin_amdgcn_readlane(ix, k) + sa;
-        const int py = __builtin_amdgcn_readlane(dy, k) + sb;
-        const int pz = __builtin_amdgcn_readlane(dz, k) + sc;
-        const double awx = readlane_f64(wx, k), awy = readlane_f64(wy, k),
-                     awz = readlane_f64(wz, k), aqs = readlane_f64(qs, k);
+      const bool ishit = dy < by && dz < bz;
+      const unsigned long long hit = __ballot(ishit);
+      const int nhit = __popcll(hit);
+      const int rank = __popcll(hit & ((1ull << lane) - 1ull));
+      // the hits' weights are formed once per atom (by the atom's own lane)
+      // and staged in LDS; each lane then reads the three weights of its
+      // stencil point, so the product is (q wx) wy wz in the same order as a
+      // lane forming it itself
+      auto contrib = [&](int r, int &at) -> long long {
+        const int pos = rpos[r];
+        int px = (pos & 0xffff) + sa;
+        const int py = ((pos >> 16) & 0xff) - 3 + sb;
+        const int pz = (int)((unsigned)pos >> 24) - 3 + sc;
         if (px >= n) px -= n;
         at = -1;
         if (py < 0 || py >= by || pz < 0 || pz >= bz) return 0;
         at = (pz * by + py) * ss + px;
-        return rint_i64(aqs * theta6(cx, awx) * theta6(cy, awy) * theta6(cz, awz));
+        return rint_i64(rtx[4 * r + sa] * rty[4 * r + sb] * rtz[4 * r + sc]);
       };
-      while (hit) {
-        const int k1 = __builtin_ctzll(hit);
-        hit &= hit - 1ull;
-        const bool two = hit != 0ull;
-        const int k2 = two ? __builtin_ctzll(hit) : k1;
-        if (two) hit &= hit - 1ull;
-        int at1, at2;
-        const long long iv1 = contrib(k1, at1);
-        const long long iv2 = two ? contrib(k2, at2) : 0ll;
-        if (iv1) atomicAdd(&slab[at1], (unsigned long long)iv1);
-        if (two && iv2) atomicAdd(&slab[at2], (unsigned long long)iv2);
+      for (int h0 = 0; h0 < nhit; h0 += kSpreadRec) {
+        if (ishit && rank >= h0 && rank < h0 + kSpreadRec) {
+          const int r = rank - h0;
+#pragma unroll
+          for (int t = 0; t < 4; t++) {
+            const Theta6 c = theta6_coef(t);
+            rtx[4 * r + t] = qs * theta6(c, wx);
+            rty[4 * r + t] = theta6(c, wy);
+            rtz[4 * r + t] = theta6(c, wz);
+          }
+          rpos[r] = ix | ((dy + 3) << 16) | ((dz + 3) << 24);
+        }
+        // the records are this wave's own: in-order LDS within a wave, no
+        // block barrier (waves walk different numbers of chunks)
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const int nr = min(kSpreadRec, nhit - h0);
+        // two atoms per pass: their weight products are independent chains
+        for (int r = 0; r < nr; r += 2) {
+          const bool two = r + 1 < nr;
+          int at1, at2 = -1;
+          const long long iv1 = contrib(r, at1);
+          const long long iv2 = two ? contrib(r + 1, at2) : 0ll;
+          if (iv1) atomicAdd(&slab[at1], (unsigned long long)iv1);
+          if (two && iv2) atomicAdd(&slab[at2], (unsigned long long)iv2);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
       }
     }
     __syncthreads();
@@ -435,7 +474,8 @@ __device__ __forceinline__ void spread_fftx_body(int bid, const int4 *__restrict
   }
 }
 template <typename CT, bool PRIO>
-__global__ __launch_bounds__(kSpreadThreads) void k_spread_fftx(const int4 *__restrict__ posq,
+__global__ __launch_bounds__(kSpreadThreads) APO_SPREAD_WAVES_ATTR
+void k_spread_fftx(const int4 *__restrict__ posq,
                                                      const float4 *__restrict__ bb,
                                                      const int *__restrict__ meta, BrickGeom g,
                                                      double qunscale, FftPlan p,
@@ -871,7 +911,8 @@ struct GatherArgs {
   int stride;
 };
 
-__global__ __launch_bounds__(kSpreadThreads) void kb_spread_fftx(const SpreadArgs *__restrict__ tab,
+__global__ __launch_bounds__(kSpreadThreads) APO_SPREAD_WAVES_ATTR
+void kb_spread_fftx(const SpreadArgs *__restrict__ tab,
                                                       const int *__restrict__ bstart, int nsys) {
   const int s = batch_system(bstart, nsys);
   const SpreadArgs &a = tab[s];

@@ -102,6 +102,10 @@ struct NbSystem {
   unsigned long long pruned_build = ~0ull;
   size_t cand_cap = 0, ent_cap = 0, item_cap = 0;
   int host_ncell = 0, host_ncand = 0, host_nent = 0, host_nitem = 0;
+  // 64-entry steps per pair-kernel work item (power of two, 2..kItemSteps),
+  // fixed by the first sizing build so the items roughly fill the pair
+  // kernel's waves (small systems get short items)
+  int item_steps = 0;
   // counts and flags of each build, exported to pinned host memory (slot 15
   // = build number) and checked lazily on the host
   int *h_meta = nullptr;
