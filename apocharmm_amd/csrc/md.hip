@@ -25,6 +25,7 @@
 
 #include "apocharmm_amd.h"
 #include "system.hpp"
+#include "thermo.hpp"
 
 namespace apo {
 
@@ -59,7 +60,6 @@ void bonded_batch_launch(BondedBatch &b, hipStream_t s);
 namespace {
 
 constexpr double kTimfac = 0.0488882129;   // AKMA time unit in ps (src/CudaIntegrator.cu:22)
-constexpr double kBoltz = 1.987191e-03;    // kcal/mol/K (include/Constants.h:17)
 constexpr unsigned kPhiloxM0 = 0xD2511F53u, kPhiloxM1 = 0xCD9E8D57u;
 constexpr unsigned kPhiloxW0 = 0x9E3779B9u, kPhiloxW1 = 0xBB67AE85u;
 constexpr unsigned kLangevinTag = 0x4C414E47u;
@@ -470,11 +470,15 @@ __device__ __forceinline__ void lt_step_body(int bid, const LtState &S, const Lt
                              fmaxf(fmaxf(red_d[1][0], red_d[1][1]), red_d[1][2]));
   }
 }
+// pend: raised for the kinetic-energy reduction of this step, which the next
+// coordinate sort runs (thermo.hpp)
 __global__ __launch_bounds__(kStepThreads) void k_lt_step(LtState S, LtCoef C,
                                                           const unsigned long long *__restrict__ stepc,
                                                           double2 *__restrict__ part,
-                                                          float2 *__restrict__ dpart) {
+                                                          float2 *__restrict__ dpart,
+                                                          unsigned long long *__restrict__ pend) {
   lt_step_body(blockIdx.x, S, C, stepc, part, dpart);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *pend = 1ull;
 }
 struct LtStepArgs {
   LtState S;
@@ -551,93 +555,20 @@ __global__ __launch_bounds__(kShakeBlock) void kb_lt_shake(const LtStepArgs *__r
   lt_shake_body(blockIdx.x - bstart[s], a.S, a.C, a.stepc, a.part, a.dpart);
 }
 
-// Fixed-order kinetic energy sum, temperatures and their running means
-// (src/CudaLangevinThermostatIntegrator.cu:684-701), step counter + 1.
-// thermo: [0] ke (three-estimate mean) [1] ke (on-step) [2] T [3] T on-step
-//         [4] <T> [5] <T on-step> [6] max displacement since the last list
-//         build (A) [7] max displacement of one step since that build (A);
-// counters: [0] step [1] averaging window [2] number of the list build that
-// [6], [7] refer to (the build's meta[16]).  [6], [7], the step count and the
-// build number also go to the host's pinned hdisp[0..3] (step count last).
-__device__ __forceinline__ void lt_finalize_body(const double2 *__restrict__ part, int npart,
-                                                 double ndof, double *__restrict__ thermo,
-                                                 unsigned long long *__restrict__ counters,
-                                                 const float2 *__restrict__ dpart,
-                                                 const int *__restrict__ meta,
-                                                 double *__restrict__ hdisp) {
-  __shared__ double s0[256], s1[256];
-  __shared__ float m0[256], m1[256];
-  double a = 0.0, b = 0.0;
-  float d2 = 0.f, s2 = 0.f;
-  for (int k = threadIdx.x; k < npart; k += 256) {
-    a += part[k].x;
-    b += part[k].y;
-    d2 = fmaxf(d2, dpart[k].x);
-    s2 = fmaxf(s2, dpart[k].y);
-  }
-  s0[threadIdx.x] = a;
-  s1[threadIdx.x] = b;
-  m0[threadIdx.x] = d2;
-  m1[threadIdx.x] = s2;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      s0[threadIdx.x] += s0[threadIdx.x + w];
-      s1[threadIdx.x] += s1[threadIdx.x + w];
-      m0[threadIdx.x] = fmaxf(m0[threadIdx.x], m0[threadIdx.x + w]);
-      m1[threadIdx.x] = fmaxf(m1[threadIdx.x], m1[threadIdx.x + w]);
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const unsigned long long seq = (unsigned long long)meta[16];
-    if (counters[2] != seq) {
-      counters[2] = seq;
-      thermo[6] = 0.0;
-      thermo[7] = 0.0;
-    }
-    thermo[6] = fmax(thermo[6], sqrt((double)m0[0]));
-    thermo[7] = fmax(thermo[7], sqrt((double)m1[0]));
-    hdisp[0] = thermo[6];
-    hdisp[1] = thermo[7];
-    hdisp[3] = (double)seq;
-    __threadfence_system();
-    hdisp[2] = (double)(counters[0] + 1);
-    __threadfence_system();
-    const double t0 = s0[0] / (0.5 * ndof * kBoltz), t1 = s1[0] / (0.5 * ndof * kBoltz);
-    const double cnt = (double)(counters[1] + 1);
-    thermo[0] = s0[0];
-    thermo[1] = s1[0];
-    thermo[2] = t0;
-    thermo[3] = t1;
-    thermo[4] += (t0 - thermo[4]) / cnt;
-    thermo[5] += (t1 - thermo[5]) / cnt;
-    counters[0] += 1;
-    counters[1] += 1;
-  }
-}
 __global__ __launch_bounds__(256) void k_lt_finalize(const double2 *__restrict__ part, int npart,
                                                      double ndof, double *__restrict__ thermo,
                                                      unsigned long long *__restrict__ counters,
                                                      const float2 *__restrict__ dpart,
                                                      const int *__restrict__ meta,
-                                                     double *__restrict__ hdisp) {
-  lt_finalize_body(part, npart, ndof, thermo, counters, dpart, meta, hdisp);
+                                                     double *__restrict__ hdisp,
+                                                     unsigned long long *__restrict__ pend) {
+  lt_finalize_body(part, npart, ndof, thermo, counters, dpart, meta, hdisp, pend);
 }
-struct LtFinArgs {
-  const double2 *part;
-  int npart;
-  double ndof;
-  double *thermo;
-  unsigned long long *counters;
-  const float2 *dpart;
-  const int *meta;
-  double *hdisp;
-};
 // one block per system
 __global__ __launch_bounds__(256) void kb_lt_finalize(const LtFinArgs *__restrict__ tab) {
   const LtFinArgs &a = tab[blockIdx.x];
-  lt_finalize_body(a.part, a.npart, a.ndof, a.thermo, a.counters, a.dpart, a.meta, a.hdisp);
+  lt_finalize_body(a.part, a.npart, a.ndof, a.thermo, a.counters, a.dpart, a.meta, a.hdisp,
+                   a.pend);
 }
 
 
@@ -860,7 +791,9 @@ struct LtSystem {
   DevBuf<double4> com_part;
   int com_blocks = 0;
   DevBuf<double> thermo;                 // 8 values, see k_lt_finalize
-  DevBuf<unsigned long long> counters;   // [0] step [1] averaging window [2] build number
+  // [0] step [1] averaging window [2] build number [3] a step's kinetic-energy
+  // reduction is pending (thermo.hpp)
+  DevBuf<unsigned long long> counters;
   DevBuf<float2> dpart;                  // per-block displacement maxima (squared)
   double *h_disp = nullptr;              // pinned: see k_lt_finalize
   // list rebuild policy: 0 = every nbfreq steps (the reference's cadence);
@@ -940,6 +873,14 @@ struct LtSystem {
   void force(hipStream_t s, bool reset, bool prune = false, bool fused = false) {
     int flags = (reset ? APO_RESET_LIST : 0) | (fused ? kCalcNoFinalize : 0);
     if (prune_every > 0) flags |= kCalcUsePruned | (prune ? kCalcPrune : 0);
+    // fixed cadence: the coordinate sort also runs the previous step's
+    // kinetic-energy reduction (one launch less per step)
+    LtFinArgs fin = fin_args();
+    struct Reset {
+      NbSystem *nb;
+      ~Reset() { nb->sort_fin = nullptr; }
+    } reset_fin{nb};
+    nb->sort_fin = list_policy == 0 ? &fin : nullptr;
     calc_force(*nb, reinterpret_cast<const float *>(xyzq.get()), flags, grad.get(), gstride,
                nullptr, nullptr, s);
   }
@@ -968,15 +909,28 @@ struct LtSystem {
     hipEvent_t t0 = nullptr;
     nb->tic(8, s, t0);
     k_lt_step<<<nstep_blocks, kStepThreads, 0, s>>>(state(true), C, counters.get(), ke_part.get(),
-                                                  dpart.get());
+                                                  dpart.get(), counters.get() + 3);
     if (nshake_blocks)
       k_lt_shake<<<nshake_blocks, kShakeBlock, 0, s>>>(state(true), C, counters.get(),
                                                       ke_part.get() + nstep_blocks,
                                                       dpart.get() + nstep_blocks);
-    k_lt_finalize<<<1, 256, 0, s>>>(ke_part.get(), nstep_blocks + nshake_blocks, ndof, thermo.get(),
-                                    counters.get(), dpart.get(), nb->meta.get(), h_disp);
+    // with the fixed cadence the next force call's coordinate sort reduces
+    // this step's kinetic energy (fin_args), and propagate() closes with a
+    // reduction of the last step; the displacement check reads every step's
+    if (list_policy != 0) finalize(s);
     APO_LAUNCH_CHECK();
     nb->toc(8, s, t0);
+  }
+  // the step's kinetic-energy reduction if one is pending (thermo.hpp)
+  void finalize(hipStream_t s) {
+    k_lt_finalize<<<1, 256, 0, s>>>(ke_part.get(), nstep_blocks + nshake_blocks, ndof, thermo.get(),
+                                    counters.get(), dpart.get(), nb->meta.get(), h_disp,
+                                    counters.get() + 3);
+    APO_LAUNCH_CHECK();
+  }
+  LtFinArgs fin_args() {
+    return LtFinArgs{ke_part.get(), nstep_blocks + nshake_blocks, ndof, thermo.get(), counters.get(),
+                     dpart.get(), nb->meta.get(), h_disp, counters.get() + 3};
   }
 
   // force + integrator update + kinetic energy (after a list prune)
@@ -1146,6 +1100,9 @@ struct LtSystem {
         k++;
       }
     }
+    // the last step's kinetic energy (its reduction would otherwise wait for
+    // the next force call)
+    if (list_policy == 0 && nsteps > 0) finalize(s);
   }
 
   void begin(hipStream_t s) {
@@ -1407,7 +1364,7 @@ LtSystem *create_lt(NbSystem *nb, const apo_lt_params &p, const double *xyz, con
     L->com_part.reserve(L->com_blocks + 1);
     L->thermo.reserve(8);
     launch_fill_f64(L->thermo.get(), 8, s);
-    std::vector<unsigned long long> c0 = {0ull, 0ull, ~0ull};
+    std::vector<unsigned long long> c0 = {0ull, 0ull, ~0ull, 0ull};
     L->counters.upload(c0, s);
     L->dpart.reserve(L->nstep_blocks + L->nshake_blocks);
     APO_HIP(hipHostMalloc(reinterpret_cast<void **>(&L->h_disp), 4 * sizeof(double)));
@@ -1578,7 +1535,7 @@ struct LtBatch {
                               L->ke_part.get() + L->nstep_blocks, L->dpart.get() + L->nstep_blocks},
                    L->nshake_blocks);
       lfin.add(LtFinArgs{L->ke_part.get(), L->nstep_blocks + L->nshake_blocks, L->ndof, L->thermo.get(),
-                         L->counters.get(), L->dpart.get(), S.meta.get(), L->h_disp},
+                         L->counters.get(), L->dpart.get(), S.meta.get(), L->h_disp, nullptr},
                1);
     }
     sort.upload(m);

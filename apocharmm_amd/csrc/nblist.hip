@@ -17,6 +17,7 @@
 #include <cmath>
 
 #include "system.hpp"
+#include "thermo.hpp"
 
 namespace apo {
 
@@ -210,6 +211,8 @@ __device__ __forceinline__ void sort_coords_body(int bid, const float4 *__restri
     bb[2 * cell + 1] = make_float4(0.5f * (mxx - mnx), 0.5f * (mxy - mny), 0.5f * (mxz - mnz), 0.f);
   }
 }
+// with_fin: the last block runs an integrator's pending kinetic-energy
+// reduction instead (thermo.hpp)
 __global__ __launch_bounds__(256) void k_sort_coords(const float4 *__restrict__ xyzq,
                                                      const int *__restrict__ vdwtype,
                                                      const int *__restrict__ loc2glo,
@@ -219,9 +222,16 @@ __global__ __launch_bounds__(256) void k_sort_coords(const float4 *__restrict__ 
                                                      int *__restrict__ type_s,
                                                      int64_t *__restrict__ fsort, int ss,
                                                      float4 *__restrict__ bb,
-                                                     int64_t *__restrict__ force, size_t nforce) {
+                                                     int64_t *__restrict__ force, size_t nforce,
+                                                     LtFinArgs fin, int with_fin) {
+  const int nsort = gridDim.x - with_fin;
+  if ((int)blockIdx.x >= nsort) {
+    lt_finalize_body(fin.part, fin.npart, fin.ndof, fin.thermo, fin.counters, fin.dpart, fin.meta,
+                     fin.hdisp, fin.pend);
+    return;
+  }
   sort_coords_body(blockIdx.x, xyzq, vdwtype, loc2glo, meta, inv_len, qscale, unit, posq, type_s,
-                   fsort, ss, bb, force, nforce, gridDim.x);
+                   fsort, ss, bb, force, nforce, nsort);
 }
 __global__ __launch_bounds__(256) void kb_sort_coords(const SortArgs *__restrict__ tab,
                                                       const int *__restrict__ bstart, int nsys) {
@@ -671,10 +681,11 @@ void nblist_sort_coords(NbSystem &S, const float4 *x, hipStream_t s, int64_t *fo
   const float qscale = 1.0f;  // raw charge; ccelec is applied per tile in the pair kernel
   const float3 unit = make_float3((float)(S.box[0] / 4294967296.0), (float)(S.box[1] / 4294967296.0),
                                   (float)(S.box[2] / 4294967296.0));
-  k_sort_coords<<<nblk(S.ssort, 256), 256, 0, s>>>(x, S.vdwtype.get(), S.loc2glo.get(),
-                                                    S.meta.get(), inv, qscale, unit, S.posq.get(),
-                                                    S.type_s.get(), S.fsort.get(), S.ssort,
-                                                    S.cell_bb.get(), force, nforce);
+  const int with_fin = S.sort_fin ? 1 : 0;
+  const LtFinArgs fin = S.sort_fin ? *S.sort_fin : LtFinArgs{};
+  k_sort_coords<<<nblk(S.ssort, 256) + with_fin, 256, 0, s>>>(
+      x, S.vdwtype.get(), S.loc2glo.get(), S.meta.get(), inv, qscale, unit, S.posq.get(),
+      S.type_s.get(), S.fsort.get(), S.ssort, S.cell_bb.get(), force, nforce, fin, with_fin);
   APO_LAUNCH_CHECK();
 }
 

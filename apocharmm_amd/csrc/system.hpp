@@ -8,6 +8,7 @@
 
 namespace apo {
 
+struct LtFinArgs;
 struct BondT { int i, j; float b0, kb; };
 struct AngleT { int i, j, k; float t0, kt; };
 struct DiheT { int i, j, k, l; float n, kc, sd, cd; float delta; int pad[3]; };
@@ -106,6 +107,9 @@ struct NbSystem {
   // fixed by the first sizing build so the items roughly fill the pair
   // kernel's waves (small systems get short items)
   int item_steps = 0;
+  // set by an integrator around its force calls: the coordinate sort runs
+  // that integrator's pending kinetic-energy reduction in one extra block
+  const LtFinArgs *sort_fin = nullptr;
   // counts and flags of each build, exported to pinned host memory (slot 15
   // = build number) and checked lazily on the host
   int *h_meta = nullptr;
