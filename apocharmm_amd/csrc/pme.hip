@@ -769,10 +769,16 @@ template <typename CT> struct PmeShapes {
 // rows: x-inverse row pairs per block; cols: y/z-pass columns per block (both
 // capped by 64 KiB of LDS); threads: block size of those passes (64..256).
 // The spread slab is kSlabBy x kSlabBz rows, halved (z first, then y down to
-// 2) until its LDS fits.
+// 2) while the grid gives fewer than two slab blocks per CU (small grids: a
+// 64^3 grid has 256 4x4 slabs, one per CU) or the LDS does not fit.  Any slab
+// shape sums every grid point exactly and pairs the same rows for the x FFT,
+// so the grid is bitwise the same.
 template <typename CT>
+// batched: the system is one of a fused batch, whose launch is filled by the
+// other systems, so its slabs are not narrowed for parallelism
 static PmeShapes<CT> pme_shapes(const NbSystem &S, int rows = kFftRowsPerBlock,
-                                int cols = kFftColsPerBlock, int threads = kFftThreads) {
+                                int cols = kFftColsPerBlock, int threads = kFftThreads,
+                                bool batched = false) {
   using C2 = typename Cx<CT>::T;
   PmeShapes<CT> P;
   if (threads < 64 || threads > 256 || threads % 64)
@@ -782,9 +788,11 @@ static PmeShapes<CT> pme_shapes(const NbSystem &S, int rows = kFftRowsPerBlock,
   P.g = GridGeom{K0, K1, K2, 1.0 / S.box[0], 1.0 / S.box[1], 1.0 / S.box[2]};
   const size_t lds_cap = 64 * 1024;
   int by = std::max(2, kSlabBy & ~1), bz = std::max(1, kSlabBz);
-  while (slab_lds<C2>(K0, by, bz) > lds_cap - 2048 && (bz > 1 || by > 2)) {
-    if (bz > 1) bz /= 2;
-    else by /= 2;
+  auto slabs = [&](int y, int z) { return ((K1 + y - 1) / y) * ((K2 + z - 1) / z); };
+  while ((slab_lds<C2>(K0, by, bz) > lds_cap - 2048 || (!batched && slabs(by, bz) < 2 * S.ncu)) &&
+         (bz > 1 || by > 2)) {
+    if (bz >= by && bz > 1) bz /= 2;
+    else by = std::max(2, (by / 2) & ~1);
   }
   if (slab_lds<C2>(K0, by, bz) > lds_cap - 2048)
     throw InvalidArgument("PME grid x dimension too large for the spread slab");
@@ -967,7 +975,7 @@ void pme_batch_clear(PmeBatch &b) {
 void pme_batch_add(PmeBatch &b, NbSystem &S, int64_t *force, int stride, int fft_rows,
                    int fft_cols, int fft_threads) {
   if (S.precision != 0) throw InvalidArgument("fused batches run the mixed-precision path");
-  const PmeShapes<float> P = pme_shapes<float>(S, fft_rows, fft_cols, fft_threads);
+  const PmeShapes<float> P = pme_shapes<float>(S, fft_rows, fft_cols, fft_threads, true);
   b.ft = P.ft;
   if (S.K[1] * P.ncz > S.part.recip_n) throw RuntimeError("reciprocal partial buffer too small");
   const int H = S.K[0] / 2 + 1;
