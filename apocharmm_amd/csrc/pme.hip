@@ -147,6 +147,29 @@ __device__ __forceinline__ C2 *lds_fft(C2 *a, C2 *b, const FftPlan &p, int nrow,
   return a;
 }
 
+// Block-wide copy of `total` elements, global -> LDS (or any element map):
+// each thread issues up to kStage loads before it writes any of them, so
+// their latencies overlap instead of each load waiting for the previous
+// element's store.
+constexpr int kStage = 8;
+template <typename T, typename Load, typename Store>
+__device__ __forceinline__ void staged_copy(int total, int tid, int nthreads, Load &&ld,
+                                            Store &&st) {
+  for (int e0 = tid; e0 < total; e0 += kStage * nthreads) {
+    T v[kStage];
+#pragma unroll
+    for (int u = 0; u < kStage; u++) {
+      const int e = e0 + u * nthreads;
+      if (e < total) v[u] = ld(e);
+    }
+#pragma unroll
+    for (int u = 0; u < kStage; u++) {
+      const int e = e0 + u * nthreads;
+      if (e < total) st(e, v[u]);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // B-splines and fractional coordinates
 // ---------------------------------------------------------------------------
@@ -320,7 +343,8 @@ __device__ __forceinline__ void spread_fftx_body(int bid, const int4 *__restrict
   const int byi = bid % g.nby, bzi = bid / g.nby;
   const int y0 = byi * by, z0 = bzi * bz;
   for (int k = tid; k < nrow * ss; k += kSpreadThreads) slab[k] = 0ull;
-  for (int e = tid; e < n; e += kSpreadThreads) tw[e] = twg[e];
+  staged_copy<C2>(n, tid, kSpreadThreads, [&](int e) { return twg[e]; },
+                  [&](int e, C2 v) { tw[e] = v; });
   const int ncell = meta[0];
   // atoms whose stencil base lies in [p0 - 3, p0 + B) reach the slab in y and z
   const float by0 = (float)(y0 - 3), bz0 = (float)(z0 - 3);
@@ -567,29 +591,43 @@ __device__ __forceinline__ void fftx_inv_body(int bid, const typename Cx<CT>::T 
   C2 *tw = reinterpret_cast<C2 *>(smem);
   C2 *a = tw + n;
   C2 *b = a + rb * n;
-  for (int e = threadIdx.x; e < n; e += blockDim.x) tw[e] = twg[e];
+  staged_copy<C2>(n, threadIdx.x, blockDim.x, [&](int e) { return twg[e]; },
+                  [&](int e, C2 v) { tw[e] = v; });
   const int r0 = bid * rb;
-  for (int e = threadIdx.x; e < rb * n; e += blockDim.x) {
-    const int r = e / n, k = e - r * n;
-    const int ra = 2 * (r0 + r), rbw = ra + 1;
-    const bool mirror = k > n / 2;
-    const int kk = mirror ? n - k : k;
-    C2 A = {0, 0}, B = {0, 0};
-    if (ra < nrows) A = in[(size_t)ra * H + kk];
-    if (rbw < nrows) B = in[(size_t)rbw * H + kk];
-    if (kk == 0 || 2 * kk == n) {
-      A.y = 0;
-      B.y = 0;
-    }
-    if (mirror) {
-      A.y = -A.y;
-      B.y = -B.y;
-    }
-    C2 Z;
-    Z.x = A.x - B.y;
-    Z.y = A.y + B.x;
-    a[e] = Z;
-  }
+  struct Pair2 {
+    C2 A, B;
+  };
+  staged_copy<Pair2>(
+      rb * n, threadIdx.x, blockDim.x,
+      [&](int e) {
+        const int r = e / n, k = e - r * n;
+        const int ra = 2 * (r0 + r), rbw = ra + 1;
+        const int kk = k > n / 2 ? n - k : k;
+        Pair2 q;
+        q.A = C2{0, 0};
+        q.B = C2{0, 0};
+        if (ra < nrows) q.A = in[(size_t)ra * H + kk];
+        if (rbw < nrows) q.B = in[(size_t)rbw * H + kk];
+        return q;
+      },
+      [&](int e, Pair2 q) {
+        const int k = e - (e / n) * n;
+        const bool mirror = k > n / 2;
+        const int kk = mirror ? n - k : k;
+        C2 A = q.A, B = q.B;
+        if (kk == 0 || 2 * kk == n) {
+          A.y = 0;
+          B.y = 0;
+        }
+        if (mirror) {
+          A.y = -A.y;
+          B.y = -B.y;
+        }
+        C2 Z;
+        Z.x = A.x - B.y;
+        Z.y = A.y + B.x;
+        a[e] = Z;
+      });
   __syncthreads();
   C2 *res = lds_fft<C2>(a, b, p, rb, tw, true);
   for (int e = threadIdx.x; e < rb * n; e += blockDim.x) {
@@ -620,14 +658,21 @@ __device__ __forceinline__ void ffty_body(int bid, typename Cx<CT>::T *__restric
   C2 *tw = reinterpret_cast<C2 *>(smem);
   C2 *a = tw + n;
   C2 *b = a + W * n;
-  for (int e = threadIdx.x; e < n; e += blockDim.x) tw[e] = twg[e];
+  staged_copy<C2>(n, threadIdx.x, blockDim.x, [&](int e) { return twg[e]; },
+                  [&](int e, C2 v) { tw[e] = v; });
   const int nchunk = (H + W - 1) / W;
   const int z = bid / nchunk, kx0 = (bid % nchunk) * W;
   const int wc = min(W, H - kx0);
-  for (int e = threadIdx.x; e < W * n; e += blockDim.x) {
-    const int y = e / W, c = e - y * W;
-    if (c < wc) a[c * n + y] = data[((size_t)z * K1 + y) * H + kx0 + c];
-  }
+  staged_copy<C2>(
+      W * n, threadIdx.x, blockDim.x,
+      [&](int e) {
+        const int y = e / W, c = e - y * W;
+        return c < wc ? data[((size_t)z * K1 + y) * H + kx0 + c] : C2{0, 0};
+      },
+      [&](int e, C2 v) {
+        const int y = e / W, c = e - y * W;
+        if (c < wc) a[c * n + y] = v;
+      });
   __syncthreads();
   C2 *res = lds_fft<C2>(a, b, p, wc, tw, inv != 0);
   for (int e = threadIdx.x; e < W * n; e += blockDim.x) {
@@ -671,16 +716,24 @@ __device__ __forceinline__ void fftz_conv_body(int bid, typename Cx<CT>::T *__re
   C2 *b = a + W * n;
   CT *pre = reinterpret_cast<CT *>(b + W * n);
   __shared__ double red[4 * 8];
-  for (int e = threadIdx.x; e < n; e += blockDim.x) tw[e] = twg[e];
-  for (int e = threadIdx.x; e < c.K0 + c.K1 + c.K2; e += blockDim.x) pre[e] = prefac[e];
+  staged_copy<C2>(n, threadIdx.x, blockDim.x, [&](int e) { return twg[e]; },
+                  [&](int e, C2 v) { tw[e] = v; });
+  staged_copy<CT>(c.K0 + c.K1 + c.K2, threadIdx.x, blockDim.x, [&](int e) { return prefac[e]; },
+                  [&](int e, CT v) { pre[e] = v; });
   const int nchunk = (H + W - 1) / W;
   const int y = bid / nchunk, kx0 = (bid % nchunk) * W;
   const int wc = min(W, H - kx0);
   const size_t zs = (size_t)c.K1 * H;
-  for (int e = threadIdx.x; e < W * n; e += blockDim.x) {
-    const int z = e / W, col = e - z * W;
-    if (col < wc) a[col * n + z] = data[(size_t)z * zs + (size_t)y * H + kx0 + col];
-  }
+  staged_copy<C2>(
+      W * n, threadIdx.x, blockDim.x,
+      [&](int e) {
+        const int z = e / W, col = e - z * W;
+        return col < wc ? data[(size_t)z * zs + (size_t)y * H + kx0 + col] : C2{0, 0};
+      },
+      [&](int e, C2 v) {
+        const int z = e / W, col = e - z * W;
+        if (col < wc) a[col * n + z] = v;
+      });
   __syncthreads();
   C2 *res = lds_fft<C2>(a, b, p, wc, tw, false);
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
