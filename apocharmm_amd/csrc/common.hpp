@@ -245,8 +245,13 @@ struct SortArgs {
 };
 void launch_fill_f64(double *p, size_t n, hipStream_t s);
 void launch_fill_i32(int *p, size_t n, hipStream_t s);
-void launch_scan(const int *in, int *out, const int *n_dev, int n_max,
-                 hipStream_t s);
+// exclusive scans, out[n] = total (also to *total_dst when given); scan2 runs
+// two independent scans in one launch
+void launch_scan(const int *in, int *out, const int *n_dev, int n_max, hipStream_t s,
+                 int *total_dst = nullptr);
+void launch_scan2(const int *in0, int *out0, const int *n0_dev, int n0_max, int *total0,
+                  const int *in1, int *out1, const int *n1_dev, int n1_max, int *total1,
+                  hipStream_t s);
 void launch_force_to_double(const int64_t *f, int natom, int stride,
                             double *out, hipStream_t s);
 void launch_add_force(const int64_t *src, int natom, int stride, int64_t *dst, hipStream_t s);

@@ -82,11 +82,6 @@ __global__ void k_col_prep(const int *__restrict__ col_count, int ncol, int *__r
   atomicMax(&meta[4], cnt);
 }
 
-__global__ void k_copy_total(const int *__restrict__ scan, const int *__restrict__ nptr, int nfix,
-                             int *__restrict__ dst) {
-  if (threadIdx.x == 0) *dst = scan[nptr ? *nptr : nfix];
-}
-
 // 3. scatter atoms into column order (slot order is arbitrary here; step 4
 // makes it deterministic).
 __global__ void k_scatter(int n, const int *__restrict__ atom_col, const int *__restrict__ atom_slot,
@@ -273,7 +268,7 @@ __global__ __launch_bounds__(64) void k_tiles(const float4 *__restrict__ bb,
     unsigned long long m = __ballot(hit);
     if (emit && hit) {
       int pos = base + count + __popcll(m & ((1ull << lane) - 1ull));
-      if (pos < cand_cap) {  // overflow is flagged by k_list_guard
+      if (pos < cand_cap) {  // overflow is flagged by k_list_guard_export
         cand_i[pos] = icell;
         cand_j[pos] = j;
       }
@@ -288,13 +283,16 @@ __global__ __launch_bounds__(64) void k_tiles(const float4 *__restrict__ bb,
 // or farther than the list cutoff at build time.  Four waves per i-cell walk
 // its candidate tiles two at a time (lanes 0-31 and 32-63); each wave loads
 // the i-cell's atoms once and reads them as wave-uniform values.
+// ISO (cubic box): the differences stay in fixed-point units and r^2 is
+// compared with rl2 in those units (rl2u), with no per-axis multiplies.
 constexpr int kMaskWaves = 4;
+template <bool ISO>
 __global__ __launch_bounds__(64 * kMaskWaves) void k_tile_mask(
     const int *__restrict__ meta, const int *__restrict__ icell_start,
     const int *__restrict__ cand_j, const int4 *__restrict__ posq,
     const int *__restrict__ loc2glo, const int *__restrict__ glo2loc,
     const int *__restrict__ excl_ptr, const int *__restrict__ excl_idx, float3 unit, float rl2,
-    uint32_t *__restrict__ cand_mask, int cand_cap) {
+    float rl2u, uint32_t *__restrict__ cand_mask, int cand_cap) {
   const int ic = blockIdx.x;
   if (ic >= meta[0]) return;
   const int lane = threadIdx.x & 63, l = lane & 31, half = lane >> 5;
@@ -321,10 +319,10 @@ __global__ __launch_bounds__(64 * kMaskWaves) void k_tile_mask(
       const int px = __builtin_amdgcn_readlane(pil.x, t);
       const int py = __builtin_amdgcn_readlane(pil.y, t);
       const int pz = __builtin_amdgcn_readlane(pil.z, t);
-      const float dx = (float)(int)(px - pj.x) * unit.x;
-      const float dy = (float)(int)(py - pj.y) * unit.y;
-      const float dz = (float)(int)(pz - pj.z) * unit.z;
-      if (dx * dx + dy * dy + dz * dz > rl2) m |= 1u << t;
+      const float dx = ISO ? (float)(int)(px - pj.x) : (float)(int)(px - pj.x) * unit.x;
+      const float dy = ISO ? (float)(int)(py - pj.y) : (float)(int)(py - pj.y) * unit.y;
+      const float dz = ISO ? (float)(int)(pz - pj.z) : (float)(int)(pz - pj.z) * unit.z;
+      if (dx * dx + dy * dy + dz * dz > (ISO ? rl2u : rl2)) m |= 1u << t;
     }
     if (tv && gj >= 0)
       for (int x = excl_ptr[gj]; x < excl_ptr[gj + 1]; x++) {
@@ -381,7 +379,7 @@ __global__ __launch_bounds__(64 * kGroupsPerCell) void k_grp_emit(
   if (icell >= meta[0]) return;
   const int lane = threadIdx.x & 63, l = lane & 31, half = lane >> 5;
   const int h = threadIdx.x >> 6;  // one wave per group
-  if (meta[8]) return;  // the list overflowed a capacity: k_list_guard emptied it
+  if (meta[8]) return;  // the list overflowed a capacity: k_list_guard_export emptied it
   const int s = icell_start[icell], e = icell_start[icell + 1];
   if (e - s > kMaxCellTiles) return;  // flagged in k_grp_count; the host raises
   const int g = icell * kGroupsPerCell + h;
@@ -483,27 +481,23 @@ void nblist_prune(NbSystem &S, hipStream_t s) {
 
 // 10. After the counts are known: any capacity overflow or input error
 // leaves the list empty (no pair kernel work, no out-of-range write) and sets
-// meta[8]; the host learns of it from the exported counts.
-__global__ void k_list_guard(int *__restrict__ meta, int cand_cap, int ent_cap, int item_cap) {
+// meta[8]; then the build's counts and flags go to pinned host memory with
+// the running build number in slot 15 (meta[16] survives the per-build
+// clear), where the host learns of errors and growth.
+__global__ void k_list_guard_export(int *__restrict__ meta, int cand_cap, int ent_cap,
+                                    int item_cap, int *__restrict__ host) {
   if (threadIdx.x != 0) return;
   const bool over = meta[1] > cand_cap || meta[2] > ent_cap || meta[3] > item_cap;
   if (over || meta[5] || meta[6] || meta[7]) {
     meta[8] = over ? 1 : 2;
     meta[3] = 0;
   }
-}
-
-// 11. Copies the build's counts and flags to pinned host memory, with the
-// running build number in slot 15 (meta[16] survives the per-build clear).
-__global__ void k_meta_export(int *__restrict__ meta, int *__restrict__ host) {
-  if (threadIdx.x == 0) {
-    const int seq = meta[16] + 1;
-    meta[16] = seq;
-    for (int k = 0; k < 15; k++) host[k] = meta[k];
-    __threadfence_system();
-    host[15] = seq;
-    __threadfence_system();
-  }
+  const int seq = meta[16] + 1;
+  meta[16] = seq;
+  for (int k = 0; k < 15; k++) host[k] = meta[k];
+  __threadfence_system();
+  host[15] = seq;
+  __threadfence_system();
 }
 
 static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
@@ -588,12 +582,11 @@ void nblist_build(NbSystem &S, const float4 *x, hipStream_t s, bool sizing) {
                                          S.atom_col.get(), S.atom_slot.get(), S.zkey.get(), meta,
                                          S.xref.get());
   APO_LAUNCH_CHECK();
-  launch_scan(S.col_count.get(), S.col_start.get(), nullptr, S.ncol, s);
   k_col_prep<<<nblk(S.ncol, 256), 256, 0, s>>>(S.col_count.get(), S.ncol, S.col_ncell.get(), meta);
   APO_LAUNCH_CHECK();
-  launch_scan(S.col_ncell.get(), S.col_cellstart.get(), nullptr, S.ncol, s);
-  k_copy_total<<<1, 64, 0, s>>>(S.col_cellstart.get(), nullptr, S.ncol, meta + 0);
-  APO_LAUNCH_CHECK();
+  // column starts of the atoms and of the cells; the cell total is meta[0]
+  launch_scan2(S.col_count.get(), S.col_start.get(), nullptr, S.ncol, nullptr, S.col_ncell.get(),
+               S.col_cellstart.get(), nullptr, S.ncol, meta + 0, s);
   unsigned long long *kb = S.colkeys.get();
   k_scatter<<<nblk(n, 256), 256, 0, s>>>(n, S.atom_col.get(), S.atom_slot.get(), S.col_start.get(),
                                           S.zkey.get(), kb);
@@ -605,9 +598,7 @@ void nblist_build(NbSystem &S, const float4 *x, hipStream_t s, bool sizing) {
   k_tiles<<<S.ncell_cap, 64, 0, s>>>(S.cell_bb.get(), meta, Lf, rl2, S.icell_cnt.get(), nullptr,
                                      nullptr, nullptr, 0, 0);
   APO_LAUNCH_CHECK();
-  launch_scan(S.icell_cnt.get(), S.icell_start.get(), meta + 0, S.ncell_cap, s);
-  k_copy_total<<<1, 64, 0, s>>>(S.icell_start.get(), meta + 0, 0, meta + 1);
-  APO_LAUNCH_CHECK();
+  launch_scan(S.icell_cnt.get(), S.icell_start.get(), meta + 0, S.ncell_cap, s, meta + 1);
   int hmeta[8];
   if (sizing) {
     APO_HIP(hipMemcpyAsync(hmeta, meta, sizeof(hmeta), hipMemcpyDeviceToHost, s));
@@ -638,20 +629,30 @@ void nblist_build(NbSystem &S, const float4 *x, hipStream_t s, bool sizing) {
   k_tiles<<<S.ncell_cap, 64, 0, s>>>(S.cell_bb.get(), meta, Lf, rl2, nullptr, S.icell_start.get(),
                                      S.cand_i.get(), S.cand_j.get(), 1, ccap);
   APO_LAUNCH_CHECK();
-  k_tile_mask<<<S.ncell_cap, 64 * kMaskWaves, 0, s>>>(
-      meta, S.icell_start.get(), S.cand_j.get(), S.posq.get(), S.loc2glo.get(), S.glo2loc.get(),
-      S.excl_ptr.get(), S.excl_idx.get(), unit, rl2, S.cand_mask.get(), ccap);
+  {
+    // cubic boxes compare in fixed-point units (the pair kernel's form)
+    const bool iso = S.box[0] == S.box[1] && S.box[1] == S.box[2];
+    const double u = S.box[0] / 4294967296.0;
+    const float rl2u = (float)(S.cutnb * S.cutnb / (u * u));
+    if (iso)
+      k_tile_mask<true><<<S.ncell_cap, 64 * kMaskWaves, 0, s>>>(
+          meta, S.icell_start.get(), S.cand_j.get(), S.posq.get(), S.loc2glo.get(),
+          S.glo2loc.get(), S.excl_ptr.get(), S.excl_idx.get(), unit, rl2, rl2u, S.cand_mask.get(),
+          ccap);
+    else
+      k_tile_mask<false><<<S.ncell_cap, 64 * kMaskWaves, 0, s>>>(
+          meta, S.icell_start.get(), S.cand_j.get(), S.posq.get(), S.loc2glo.get(),
+          S.glo2loc.get(), S.excl_ptr.get(), S.excl_idx.get(), unit, rl2, rl2u, S.cand_mask.get(),
+          ccap);
+  }
   APO_LAUNCH_CHECK();
   const int ngrp = S.ncell_cap * kGroupsPerCell;
   k_grp_count<<<S.ncell_cap, 64 * kGroupsPerCell, 0, s>>>(
       meta, S.icell_start.get(), S.cand_mask.get(), S.grp_cnt.get(), S.grp_nitem.get(), meta, ccap,
       S.item_steps);
   APO_LAUNCH_CHECK();
-  launch_scan(S.grp_cnt.get(), S.grp_start.get(), nullptr, ngrp, s);
-  launch_scan(S.grp_nitem.get(), S.grp_itemstart.get(), nullptr, ngrp, s);
-  k_copy_total<<<1, 64, 0, s>>>(S.grp_start.get(), nullptr, ngrp, meta + 2);
-  k_copy_total<<<1, 64, 0, s>>>(S.grp_itemstart.get(), nullptr, ngrp, meta + 3);
-  APO_LAUNCH_CHECK();
+  launch_scan2(S.grp_cnt.get(), S.grp_start.get(), nullptr, ngrp, meta + 2, S.grp_nitem.get(),
+               S.grp_itemstart.get(), nullptr, ngrp, meta + 3, s);
   if (sizing) {
     APO_HIP(hipMemcpyAsync(hmeta, meta, sizeof(hmeta), hipMemcpyDeviceToHost, s));
     APO_HIP(hipStreamSynchronize(s));
@@ -663,13 +664,11 @@ void nblist_build(NbSystem &S, const float4 *x, hipStream_t s, bool sizing) {
     S.host_nitem = hmeta[3];
     list_reserve(S, 0, hmeta[2], hmeta[3]);
   }
-  k_list_guard<<<1, 64, 0, s>>>(meta, ccap, (int)S.ent_cap, (int)S.item_cap);
+  k_list_guard_export<<<1, 64, 0, s>>>(meta, ccap, (int)S.ent_cap, (int)S.item_cap, S.h_meta);
   APO_LAUNCH_CHECK();
   k_grp_emit<<<S.ncell_cap, 64 * kGroupsPerCell, 0, s>>>(meta, S.icell_start.get(), S.cand_mask.get(),
                                         S.grp_start.get(), S.grp_itemstart.get(), S.ent.get(),
                                         S.items.get(), S.item_steps);
-  APO_LAUNCH_CHECK();
-  k_meta_export<<<1, 64, 0, s>>>(meta, S.h_meta);
   APO_LAUNCH_CHECK();
   S.builds_queued++;
   S.list_valid = true;

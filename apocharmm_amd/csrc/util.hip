@@ -46,38 +46,78 @@ void launch_fill_f64(double *p, size_t n, hipStream_t s) {
   APO_LAUNCH_CHECK();
 }
 
-// Exclusive scan of in[0..n) into out[0..n], out[n] = total, one block.
-// n = *n_dev (clamped to n_max) when n_dev is given, else n_max.
-__global__ __launch_bounds__(1024) void k_scan(const int *in, int *out, const int *n_dev,
-                                               int n_max) {
-  __shared__ int sh[1024];
-  __shared__ int carry;
-  int n = n_dev ? min(*n_dev, n_max) : n_max;
-  int tid = threadIdx.x;
-  if (tid == 0) carry = 0;
-  __syncthreads();
-  for (int base = 0; base < n; base += 1024) {
-    int i = base + tid;
-    int v = i < n ? in[i] : 0;
-    sh[tid] = v;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      int t = tid >= off ? sh[tid - off] : 0;
-      __syncthreads();
-      sh[tid] += t;
-      __syncthreads();
+// Exclusive scan of in[0..n) into out[0..n], out[n] = total, one block per
+// array; the total also goes to *total_dst when given.  n = *n_dev (clamped
+// to n_max) when n_dev is given, else n_max.  Each thread scans 8
+// consecutive elements in registers, the waves' sums are scanned across the
+// block, and chunks of 8192 elements follow each other with a carry.
+struct ScanArgs {
+  const int *in;
+  int *out;
+  const int *n_dev;
+  int n_max;
+  int *total_dst;
+};
+constexpr int kScanThreads = 1024, kScanPer = 8;
+__global__ __launch_bounds__(kScanThreads) void k_scan(ScanArgs a0, ScanArgs a1) {
+  const ScanArgs &a = blockIdx.x == 0 ? a0 : a1;
+  __shared__ int wsum[kScanThreads / 64];
+  const int n = a.n_dev ? min(*a.n_dev, a.n_max) : a.n_max;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int carry = 0;
+  for (int base = 0; base < n; base += kScanThreads * kScanPer) {
+    const int i0 = base + tid * kScanPer;
+    int v[kScanPer];
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; k++) {
+      v[k] = i0 + k < n ? a.in[i0 + k] : 0;
+      sum += v[k];
     }
-    int c = carry;
-    if (i < n) out[i] = c + sh[tid] - v;
+    int x = sum;  // inclusive scan of the thread sums within the wave
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(x, off, 64);
+      if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
     __syncthreads();
-    if (tid == 1023) carry = c + sh[1023];
+    if (wave == 0) {
+      int w = lane < kScanThreads / 64 ? wsum[lane] : 0;
+#pragma unroll
+      for (int off = 1; off < kScanThreads / 64; off <<= 1) {
+        const int y = __shfl_up(w, off, 64);
+        if (lane >= off) w += y;
+      }
+      if (lane < kScanThreads / 64) wsum[lane] = w;
+    }
     __syncthreads();
+    int e = carry + x - sum + (wave > 0 ? wsum[wave - 1] : 0);
+#pragma unroll
+    for (int k = 0; k < kScanPer; k++) {
+      if (i0 + k < n) a.out[i0 + k] = e;
+      e += v[k];
+    }
+    carry += wsum[kScanThreads / 64 - 1];
+    __syncthreads();  // wsum is rewritten by the next chunk
   }
-  if (tid == 0) out[n] = carry;
+  if (tid == 0) {
+    a.out[n] = carry;
+    if (a.total_dst) *a.total_dst = carry;
+  }
 }
 
-void launch_scan(const int *in, int *out, const int *n_dev, int n_max, hipStream_t s) {
-  k_scan<<<1, 1024, 0, s>>>(in, out, n_dev, n_max);
+void launch_scan(const int *in, int *out, const int *n_dev, int n_max, hipStream_t s,
+                 int *total_dst) {
+  const ScanArgs a{in, out, n_dev, n_max, total_dst};
+  k_scan<<<1, kScanThreads, 0, s>>>(a, a);
+  APO_LAUNCH_CHECK();
+}
+void launch_scan2(const int *in0, int *out0, const int *n0_dev, int n0_max, int *total0,
+                  const int *in1, int *out1, const int *n1_dev, int n1_max, int *total1,
+                  hipStream_t s) {
+  k_scan<<<2, kScanThreads, 0, s>>>(ScanArgs{in0, out0, n0_dev, n0_max, total0},
+                                    ScanArgs{in1, out1, n1_dev, n1_max, total1});
   APO_LAUNCH_CHECK();
 }
 

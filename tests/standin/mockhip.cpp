@@ -4,7 +4,7 @@
 // Model: every operation runs at once on the calling thread.  Device memory
 // is host memory (calloc), so host code that touches freed device or pinned
 // memory is caught by ASAN.  Kernels are no-ops, except the list build's
-// count export (k_meta_export), which is emulated so that the host sees
+// count export (k_list_guard_export), which is emulated so that the host sees
 // builds complete; with APO_STANDIN_GROW=1 the exported counts grow at every
 // export, which drives the capacity-growth path (list_export_check ->
 // list_reserve -> new generation -> graph recapture).  Stream capture records
@@ -76,7 +76,8 @@ bool grow_on() {
   return g && g[0] == '1';
 }
 
-// k_meta_export(int *meta, int *host): the build's counts and its number to
+// k_list_guard_export(int *meta, int cand_cap, int ent_cap, int item_cap,
+// int *host): the build's counts and its number to
 // the pinned host copy (host[15] last); with growth on, the counts ncand,
 // nent, nitem (host[1..3]) report more than the last build used
 void meta_export(int *meta, int *host) {
@@ -150,10 +151,10 @@ hipError_t hipLaunchKernel(const void *f, dim3, dim3, void **args, size_t, hipSt
   std::lock_guard<std::mutex> lk(mu);
   auto it = kernel_names.find(f);
   if (it == kernel_names.end()) fail("launch of an unregistered kernel", f);
-  if (it->second.find("k_meta_export") != std::string::npos) {
+  if (it->second.find("k_list_guard_export") != std::string::npos) {
     Op op{Op::kMetaExport};
     op.dst = *static_cast<int **>(args[0]);
-    op.src = *static_cast<int **>(args[1]);
+    op.src = *static_cast<int **>(args[4]);
     submit(s, op);
   } else {
     (void)as_stream(s);
