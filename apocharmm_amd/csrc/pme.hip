@@ -27,16 +27,41 @@ template <typename CT> struct Cx;
 template <> struct Cx<float> { using T = float2; };
 template <> struct Cx<double> { using T = double2; };
 
+// Division by a small positive divisor d as a high multiply: with
+// m = ceil(2^32 / d), floor(x m / 2^32) = floor(x / d) for 0 <= x < 2^32 / d
+// (the rounding excess x (m d - 2^32) / (d 2^32) stays below 1/d, and
+// x / d sits at least 1/d below the next integer).  m = 0 stands for d = 1.
+// Every index divided here stays below 2^20 and every divisor below 2^12.
+static inline unsigned magic_of(int d) { return d > 1 ? 0xffffffffu / (unsigned)d + 1u : 0u; }
+__device__ __forceinline__ int fdiv(int x, unsigned m) {
+  return m ? (int)__umulhi((unsigned)x, m) : x;
+}
+__device__ __forceinline__ unsigned magic_dev(int d) {
+  return d > 1 ? 0xffffffffu / (unsigned)d + 1u : 0u;
+}
+
+// Radix plan of one transform length, with the divisors each Stockham stage
+// needs (butterflies per row, and the span Ns of the stage) as magic numbers.
 struct FftPlan {
   int n, nrad;
   int rad[16];
+  unsigned mbf[16], mns[16];
 };
 
 static FftPlan plan_of(const FftDim &d) {
   FftPlan p;
   p.n = d.n;
   p.nrad = d.nrad;
-  for (int i = 0; i < 16; i++) p.rad[i] = d.rad[i];
+  int ns = 1;
+  for (int i = 0; i < 16; i++) {
+    p.rad[i] = d.rad[i];
+    p.mbf[i] = p.mns[i] = 0;
+    if (i < d.nrad) {
+      p.mbf[i] = magic_of(d.n / d.rad[i]);
+      p.mns[i] = magic_of(ns);
+      ns *= d.rad[i];
+    }
+  }
   return p;
 }
 
@@ -97,23 +122,24 @@ __device__ __forceinline__ void small_dft(C2 (&v)[R], const C2 *tw, int n, bool 
 }
 
 // One Stockham autosort stage of radix R over nrow rows of length n.
+// mbf, mns: magic numbers of n / R and Ns (fdiv).
 template <typename C2, int R>
 __device__ __forceinline__ void fft_stage(const C2 *in, C2 *out, int n, int nrow, int Ns,
-                                          const C2 *tw, bool inv) {
+                                          unsigned mbf, unsigned mns, const C2 *tw, bool inv) {
   const int nbf = n / R;
   const int step = n / (Ns * R);
   for (int w = threadIdx.x; w < nbf * nrow; w += blockDim.x) {
-    const int row = w / nbf, j = w - row * nbf;
+    const int row = fdiv(w, mbf), j = w - row * nbf;
     const C2 *ri = in + row * n;
     C2 *ro = out + row * n;
-    const int k = j % Ns;
+    const int jq = fdiv(j, mns), k = j - jq * Ns;
     C2 v[R];
 #pragma unroll
     for (int r = 0; r < R; r++) v[r] = ri[j + r * nbf];
 #pragma unroll
     for (int r = 1; r < R; r++) v[r] = cmul(v[r], twid(tw, r * k * step, inv));
     small_dft<C2, R>(v, tw, n, inv);
-    const int idxD = (j / Ns) * Ns * R + k;
+    const int idxD = jq * Ns * R + k;
 #pragma unroll
     for (int r = 0; r < R; r++) ro[idxD + r * Ns] = v[r];
   }
@@ -129,14 +155,15 @@ __device__ __forceinline__ C2 *lds_fft(C2 *a, C2 *b, const FftPlan &p, int nrow,
   int Ns = 1;
   for (int s = 0; s < p.nrad; s++) {
     const int R = p.rad[s];
+    const unsigned mb = p.mbf[s], mn = p.mns[s];
     switch (R) {
-      case 2: fft_stage<C2, 2>(a, b, p.n, nrow, Ns, tw, inv); break;
-      case 3: fft_stage<C2, 3>(a, b, p.n, nrow, Ns, tw, inv); break;
-      case 4: fft_stage<C2, 4>(a, b, p.n, nrow, Ns, tw, inv); break;
-      case 5: fft_stage<C2, 5>(a, b, p.n, nrow, Ns, tw, inv); break;
-      case 7: fft_stage<C2, 7>(a, b, p.n, nrow, Ns, tw, inv); break;
-      case 11: fft_stage<C2, 11>(a, b, p.n, nrow, Ns, tw, inv); break;
-      default: fft_stage<C2, 13>(a, b, p.n, nrow, Ns, tw, inv); break;
+      case 2: fft_stage<C2, 2>(a, b, p.n, nrow, Ns, mb, mn, tw, inv); break;
+      case 3: fft_stage<C2, 3>(a, b, p.n, nrow, Ns, mb, mn, tw, inv); break;
+      case 4: fft_stage<C2, 4>(a, b, p.n, nrow, Ns, mb, mn, tw, inv); break;
+      case 5: fft_stage<C2, 5>(a, b, p.n, nrow, Ns, mb, mn, tw, inv); break;
+      case 7: fft_stage<C2, 7>(a, b, p.n, nrow, Ns, mb, mn, tw, inv); break;
+      case 11: fft_stage<C2, 11>(a, b, p.n, nrow, Ns, mb, mn, tw, inv); break;
+      default: fft_stage<C2, 13>(a, b, p.n, nrow, Ns, mb, mn, tw, inv); break;
     }
     __syncthreads();
     C2 *t = a;
@@ -469,9 +496,10 @@ __device__ __forceinline__ void spread_fftx_body(int bid, const int4 *__restrict
   // rows (y, y + 1) of one z plane -> one complex row; rows past the grid
   // (edge slabs) hold only wrapped duplicates and enter as zero
   const int hb = by >> 1, ncx = nrow >> 1, H = n / 2 + 1;
+  const unsigned mn = magic_dev(n), mh = magic_dev(H), mhb = magic_dev(hb);
   for (int e = tid; e < ncx * n; e += kSpreadThreads) {
-    const int r = e / n, x = e - r * n;
-    const int pz = r / hb, py = 2 * (r - pz * hb);
+    const int r = fdiv(e, mn), x = e - r * n;
+    const int pz = fdiv(r, mhb), py = 2 * (r - pz * hb);
     const int y = y0 + py, z = z0 + pz;
     const unsigned long long *row = slab + (size_t)(pz * by + py) * ss;
     C2 v;
@@ -483,10 +511,10 @@ __device__ __forceinline__ void spread_fftx_body(int bid, const int4 *__restrict
   __syncthreads();
   C2 *res = lds_fft<C2>(a, b, p, ncx, tw, false);
   for (int e = tid; e < ncx * H; e += kSpreadThreads) {
-    const int r = e / H, k = e - r * H;
-    const int pz = r / hb, py = 2 * (r - pz * hb);
+    const int r = fdiv(e, mh), k = e - r * H;
+    const int pz = fdiv(r, mhb), py = 2 * (r - pz * hb);
     const int y = y0 + py, z = z0 + pz;
-    const C2 Z = res[r * n + k], Zm = res[r * n + (n - k) % n];
+    const C2 Z = res[r * n + k], Zm = res[r * n + (k ? n - k : 0)];
     C2 A, B;
     A.x = (CT)0.5 * (Z.x + Zm.x);
     A.y = (CT)0.5 * (Z.y - Zm.y);
@@ -594,13 +622,14 @@ __device__ __forceinline__ void fftx_inv_body(int bid, const typename Cx<CT>::T 
   staged_copy<C2>(n, threadIdx.x, blockDim.x, [&](int e) { return twg[e]; },
                   [&](int e, C2 v) { tw[e] = v; });
   const int r0 = bid * rb;
+  const unsigned mn = magic_dev(n);
   struct Pair2 {
     C2 A, B;
   };
   staged_copy<Pair2>(
       rb * n, threadIdx.x, blockDim.x,
       [&](int e) {
-        const int r = e / n, k = e - r * n;
+        const int r = fdiv(e, mn), k = e - r * n;
         const int ra = 2 * (r0 + r), rbw = ra + 1;
         const int kk = k > n / 2 ? n - k : k;
         Pair2 q;
@@ -611,7 +640,7 @@ __device__ __forceinline__ void fftx_inv_body(int bid, const typename Cx<CT>::T 
         return q;
       },
       [&](int e, Pair2 q) {
-        const int k = e - (e / n) * n;
+        const int k = e - fdiv(e, mn) * n;
         const bool mirror = k > n / 2;
         const int kk = mirror ? n - k : k;
         C2 A = q.A, B = q.B;
@@ -631,7 +660,7 @@ __device__ __forceinline__ void fftx_inv_body(int bid, const typename Cx<CT>::T 
   __syncthreads();
   C2 *res = lds_fft<C2>(a, b, p, rb, tw, true);
   for (int e = threadIdx.x; e < rb * n; e += blockDim.x) {
-    const int r = e / n, x = e - r * n;
+    const int r = fdiv(e, mn), x = e - r * n;
     const int ra = 2 * (r0 + r), rbw = ra + 1;
     if (ra < nrows) phi[(size_t)ra * n + x] = res[e].x;
     if (rbw < nrows) phi[(size_t)rbw * n + x] = res[e].y;
@@ -663,20 +692,21 @@ __device__ __forceinline__ void ffty_body(int bid, typename Cx<CT>::T *__restric
   const int nchunk = (H + W - 1) / W;
   const int z = bid / nchunk, kx0 = (bid % nchunk) * W;
   const int wc = min(W, H - kx0);
+  const unsigned mw = magic_dev(W);
   staged_copy<C2>(
       W * n, threadIdx.x, blockDim.x,
       [&](int e) {
-        const int y = e / W, c = e - y * W;
+        const int y = fdiv(e, mw), c = e - y * W;
         return c < wc ? data[((size_t)z * K1 + y) * H + kx0 + c] : C2{0, 0};
       },
       [&](int e, C2 v) {
-        const int y = e / W, c = e - y * W;
+        const int y = fdiv(e, mw), c = e - y * W;
         if (c < wc) a[c * n + y] = v;
       });
   __syncthreads();
   C2 *res = lds_fft<C2>(a, b, p, wc, tw, inv != 0);
   for (int e = threadIdx.x; e < W * n; e += blockDim.x) {
-    const int y = e / W, c = e - y * W;
+    const int y = fdiv(e, mw), c = e - y * W;
     if (c < wc) data[((size_t)z * K1 + y) * H + kx0 + c] = res[c * n + y];
   }
 }
@@ -724,14 +754,15 @@ __device__ __forceinline__ void fftz_conv_body(int bid, typename Cx<CT>::T *__re
   const int y = bid / nchunk, kx0 = (bid % nchunk) * W;
   const int wc = min(W, H - kx0);
   const size_t zs = (size_t)c.K1 * H;
+  const unsigned mw = magic_dev(W), mn = magic_dev(n);
   staged_copy<C2>(
       W * n, threadIdx.x, blockDim.x,
       [&](int e) {
-        const int z = e / W, col = e - z * W;
+        const int z = fdiv(e, mw), col = e - z * W;
         return col < wc ? data[(size_t)z * zs + (size_t)y * H + kx0 + col] : C2{0, 0};
       },
       [&](int e, C2 v) {
-        const int z = e / W, col = e - z * W;
+        const int z = fdiv(e, mw), col = e - z * W;
         if (col < wc) a[col * n + z] = v;
       });
   __syncthreads();
@@ -739,7 +770,7 @@ __device__ __forceinline__ void fftz_conv_body(int bid, typename Cx<CT>::T *__re
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
   const int nf0 = c.K0 / 2 + (c.K0 % 2), nf1 = c.K1 / 2 + (c.K1 % 2), nf2 = c.K2 / 2 + (c.K2 % 2);
   for (int e = threadIdx.x; e < wc * n; e += blockDim.x) {
-    const int col = e / n, z = e - col * n;
+    const int col = fdiv(e, mn), z = e - col * n;
     const int kx = kx0 + col;
     C2 q = res[e];
     if (kx == 0 && y == 0 && z == 0) {
@@ -778,7 +809,7 @@ __device__ __forceinline__ void fftz_conv_body(int bid, typename Cx<CT>::T *__re
   C2 *other = res == a ? b : a;
   C2 *out = lds_fft<C2>(res, other, p, wc, tw, true);
   for (int e = threadIdx.x; e < W * n; e += blockDim.x) {
-    const int z = e / W, col = e - z * W;
+    const int z = fdiv(e, mw), col = e - z * W;
     if (col < wc) data[(size_t)z * zs + (size_t)y * H + kx0 + col] = out[col * n + z];
   }
   if (CALC_EV) {

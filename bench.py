@@ -585,7 +585,8 @@ def main():
             xyz, vel = st["xyz"], st["vel"]
             equil = {"steps": a.equil_md_steps, "friction_per_ps": EQUIL_FRICTION,
                      "T_end_K": round(float(T_eq[1]), 2), "wall_s": round(t_eq, 3),
-                     "launch_setting": launch["setting"]}
+                     "launch_setting": launch["setting"],
+                     "launch_trial_cycle_ms": [round(v, 4) for v in launch["trial_ms"]]}
         lt = LangevinThermostatIntegrator(DT_PS, friction=FRICTION, temperature=TEMP, seed=SEED)
         lt.setNonbondedListUpdateFrequency(REBUILD_EVERY)
         lt.setListPruneFrequency(a.prune_every)
@@ -695,4 +696,7 @@ def main():
 
 
 if __name__ == "__main__":
+    # a crash in native code prints the Python stack of every thread to stderr
+    import faulthandler
+    faulthandler.enable()
     main()
