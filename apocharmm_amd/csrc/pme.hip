@@ -273,14 +273,18 @@ constexpr int kSpreadThreads = APO_SPREAD_THREADS;  // threads per slab block
 __host__ __device__ __forceinline__ int slab_stride(int K0) { return K0 + ((36 - K0 % 32) % 32); }
 // Atoms whose stencils a wave walks are staged as records in LDS, kSpreadRec
 // per wave at a time: the twelve FP64 B-spline weights (the charge folded into
-// the x weights) and the packed stencil base.
+// the x weights) and the packed stencil base, followed by the list of the
+// records' stencil rows (y, z) that lie inside the slab (16-bit entries:
+// record, y row, z row).
 constexpr int kSpreadRec = 32;
 // Register budget of the spread kernels: five waves per SIMD, so a slab
 // block still finds room beside the pair kernel's waves
 #ifndef APO_SPREAD_WAVES_ATTR
 #define APO_SPREAD_WAVES_ATTR __attribute__((amdgpu_waves_per_eu(5)))
 #endif
-constexpr size_t kSpreadRecBytes = (size_t)kSpreadRec * (12 * sizeof(double) + sizeof(int));
+constexpr size_t kSpreadRecBytes =
+    (size_t)kSpreadRec * (12 * sizeof(double) + sizeof(int) + 16 * sizeof(unsigned short));
+static_assert(kSpreadRec <= 32 && kSpreadRecBytes % 8 == 0, "record index and alignment");
 template <typename C2> static size_t slab_lds(int K0, int by, int bz) {
   const size_t rows = (size_t)by * bz;
   return rows * slab_stride(K0) * sizeof(unsigned long long) + (size_t)K0 * sizeof(C2) +
@@ -367,6 +371,7 @@ __device__ __forceinline__ void spread_fftx_body(int bid, const int4 *__restrict
                 (size_t)wave * (kSpreadRecBytes / sizeof(double));
   double *rty = rtx + 4 * kSpreadRec, *rtz = rty + 4 * kSpreadRec;
   int *rpos = reinterpret_cast<int *>(rtz + 4 * kSpreadRec);
+  unsigned short *rlist = reinterpret_cast<unsigned short *>(rpos + kSpreadRec);
   const int byi = bid % g.nby, bzi = bid / g.nby;
   const int y0 = byi * by, z0 = bzi * bz;
   for (int k = tid; k < nrow * ss; k += kSpreadThreads) slab[k] = 0ull;
@@ -378,9 +383,6 @@ __device__ __forceinline__ void spread_fftx_body(int bid, const int4 *__restrict
   const float K1 = (float)g.K1, K2 = (float)g.K2;
   const float iK1 = 1.f / K1, iK2 = 1.f / K2;
   const float gy = K1 / g.L1, gz = K2 / g.L2;  // grid points per A
-  // this lane's stencil point and its weight polynomials
-  const int sa = lane & 3, sb = (lane >> 2) & 3, sc = lane >> 4;
-  const Theta6 cx = theta6_coef(sa), cy = theta6_coef(sb), cz = theta6_coef(sc);
   // cells are tested kSlabCand at a time, four per thread with their loads in
   // flight together; the atoms of the cells that pass are walked by the four
   // waves, each prefetching its next atom while it adds the current hits
@@ -447,45 +449,63 @@ __device__ __forceinline__ void spread_fftx_body(int bid, const int4 *__restrict
       const int nhit = __popcll(hit);
       const int rank = __popcll(hit & ((1ull << lane) - 1ull));
       // the hits' weights are formed once per atom (by the atom's own lane)
-      // and staged in LDS; each lane then reads the three weights of its
-      // stencil point, so the product is (q wx) wy wz in the same order as a
-      // lane forming it itself
-      auto contrib = [&](int r, int &at) -> long long {
-        const int pos = rpos[r];
-        int px = (pos & 0xffff) + sa;
-        const int py = ((pos >> 16) & 0xff) - 3 + sb;
-        const int pz = (int)((unsigned)pos >> 24) - 3 + sc;
-        if (px >= n) px -= n;
-        at = -1;
-        if (py < 0 || py >= by || pz < 0 || pz >= bz) return 0;
-        at = (pz * by + py) * ss + px;
-        return rint_i64(rtx[4 * r + sa] * rty[4 * r + sb] * rtz[4 * r + sc]);
-      };
+      // and staged in LDS, with the list of the atom's stencil rows (y, z)
+      // inside the slab; a lane then takes one listed row and adds its four x
+      // points, each product formed as (q wx) wy wz, the order in which a
+      // lane forming it itself would
       for (int h0 = 0; h0 < nhit; h0 += kSpreadRec) {
-        if (ishit && rank >= h0 && rank < h0 + kSpreadRec) {
-          const int r = rank - h0;
+        const bool mine = ishit && rank >= h0 && rank < h0 + kSpreadRec;
+        const int r = rank - h0;
+        unsigned ymask = 0, zmask = 0;
+        if (mine) {
 #pragma unroll
           for (int t = 0; t < 4; t++) {
             const Theta6 c = theta6_coef(t);
             rtx[4 * r + t] = qs * theta6(c, wx);
             rty[4 * r + t] = theta6(c, wy);
             rtz[4 * r + t] = theta6(c, wz);
+            if (dy + t >= 0 && dy + t < by) ymask |= 1u << t;
+            if (dz + t >= 0 && dz + t < bz) zmask |= 1u << t;
           }
           rpos[r] = ix | ((dy + 3) << 16) | ((dz + 3) << 24);
+        }
+        const int cnt = __popc(ymask) * __popc(zmask);
+        int incl = cnt;  // inclusive prefix sum over the wave
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const int v = __shfl_up(incl, o, 64);
+          if (lane >= o) incl += v;
+        }
+        const int total = __shfl(incl, 63, 64);
+        if (cnt) {
+          int at = incl - cnt;
+#pragma unroll
+          for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+              if (((zmask >> t) & (ymask >> u) & 1u) != 0u)
+                rlist[at++] = (unsigned short)(r | (u << 5) | (t << 7));
         }
         // the records are this wave's own: in-order LDS within a wave, no
         // block barrier (waves walk different numbers of chunks)
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        const int nr = min(kSpreadRec, nhit - h0);
-        // two atoms per pass: their weight products are independent chains
-        for (int r = 0; r < nr; r += 2) {
-          const bool two = r + 1 < nr;
-          int at1, at2 = -1;
-          const long long iv1 = contrib(r, at1);
-          const long long iv2 = two ? contrib(r + 1, at2) : 0ll;
-          if (iv1) atomicAdd(&slab[at1], (unsigned long long)iv1);
-          if (two && iv2) atomicAdd(&slab[at2], (unsigned long long)iv2);
+        for (int e = lane; e < total; e += 64) {
+          const int ent = rlist[e];
+          const int rr = ent & 31, sy = (ent >> 5) & 3, sz = ent >> 7;
+          const int pos = rpos[rr];
+          const int px0 = pos & 0xffff;
+          const int py = ((pos >> 16) & 0xff) - 3 + sy;
+          const int pz = (int)((unsigned)pos >> 24) - 3 + sz;
+          unsigned long long *row = slab + (size_t)(pz * by + py) * ss;
+          const double wyv = rty[4 * rr + sy], wzv = rtz[4 * rr + sz];
+#pragma unroll
+          for (int a = 0; a < 4; a++) {
+            int px = px0 + a;
+            if (px >= n) px -= n;
+            const long long iv = rint_i64(rtx[4 * rr + a] * wyv * wzv);
+            if (iv) atomicAdd(&row[px], (unsigned long long)iv);
+          }
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
