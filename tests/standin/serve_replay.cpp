@@ -3,7 +3,7 @@
 // created, advanced in chunks on their own streams, read back and destroyed
 // (the stream server), then jobs one after another (the plain run), then
 // jobs in fused batches re-formed at every membership change (the batch
-// server).  With APO_STANDIN_GROW=1 every list build reports more entries
+// server), then systems in the many-system benchmark's order.  With APO_STANDIN_GROW=1 every list build reports more entries
 // than the last, so capacities grow and captured graphs are recaptured.
 // Exit status 0 = every call succeeded and nothing leaked; ASAN aborts on a
 // memory error.
@@ -257,6 +257,31 @@ int main(int argc, char **argv) {
     }
     if (batch) CHECK(apo_lt_batch_destroy(batch));
     std::printf("[standin] batch server: %d jobs, %d batches formed\n", served, reforms);
+  }
+  // 4. the many-system benchmark's order: every system created first, each
+  // warmed and then advanced alone (launch trial included), then all of them
+  // in one fused batch with its launch-shape autotuning, then destroyed
+  {
+    std::vector<Job> jobs = make_jobs(16, 53);
+    for (Job &j : jobs) start(j, false);
+    for (Job &j : jobs) CHECK(apo_lt_propagate(j.lt, 30, nullptr));
+    for (Job &j : jobs) CHECK(apo_lt_propagate(j.lt, 400, nullptr));
+    std::vector<apo_lt_integrator *> lts;
+    for (Job &j : jobs) {
+      j.done = 430;
+      lts.push_back(j.lt);
+    }
+    apo_lt_batch *batch = nullptr;
+    CHECK(apo_lt_batch_create(lts.data(), (int)lts.size(), 4, &batch));
+    double report[16] = {0};
+    CHECK(apo_lt_batch_autotune(batch, nullptr, report));
+    CHECK(apo_lt_batch_propagate(batch, 100, nullptr));
+    CHECK(apo_lt_batch_destroy(batch));
+    for (Job &j : jobs) {
+      j.done += 100;
+      finish(j);
+    }
+    std::printf("[standin] many-system bench order: %zu systems\n", jobs.size());
   }
   const size_t allocs = apo_standin_live_device_allocs(), handles = apo_standin_live_handles();
   size_t c[4];

@@ -6,7 +6,9 @@ memory is host memory, kernels are no-ops except the list builds' count
 export, graph capture is recorded and replayed, destroyed handles are caught)
 and runs serve_replay.cpp: 40 water-box jobs through the stream server's
 create / chunked propagate / read back / destroy sequence, six more one after
-another, 20 through fused batches re-formed at every membership change.  With
+another, 20 through fused batches re-formed at every membership change, and 16
+in the many-system benchmark's order (all created, each warmed and advanced
+alone through the launch trial, then one autotuned fused batch).  With
 APO_STANDIN_GROW=1 every list build reports more entries than the last, which
 drives list_export_check -> list_reserve -> new generation -> graph
 recapture.  Every call must succeed, ASAN must stay silent, and no device
@@ -37,7 +39,7 @@ def replay():
 
 def _run(exe, grow):
     env = dict(os.environ, APO_STANDIN_GROW="1" if grow else "0",
-               ASAN_OPTIONS="detect_leaks=1:abort_on_error=1")
+               ASAN_OPTIONS="detect_leaks=1:abort_on_error=1:detect_stack_use_after_return=1")
     r = subprocess.run([exe, "40"], capture_output=True, text=True, timeout=900, env=env)
     out = r.stdout + r.stderr
     print(out[-2000:])
@@ -52,6 +54,7 @@ def test_serving_sequences_clean_under_asan(replay):
     inst, allocs, out = _run(replay, grow=False)
     assert "stream server: 40 jobs" in out and "plain run: 6 jobs" in out
     assert "batch server: 20 jobs" in out
+    assert "many-system bench order: 16 systems" in out
 
 
 def test_list_growth_path_clean_under_asan(replay):
