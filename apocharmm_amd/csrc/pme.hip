@@ -40,6 +40,14 @@ __device__ __forceinline__ unsigned magic_dev(int d) {
   return d > 1 ? 0xffffffffu / (unsigned)d + 1u : 0u;
 }
 
+// Whether a transform length needs a radix above 5 (the FFT kernels' variant
+// with the radix-7, 11 and 13 stages).
+static bool plan_big(const FftDim &d) {
+  for (int i = 0; i < d.nrad; i++)
+    if (d.rad[i] > 5) return true;
+  return false;
+}
+
 // Radix plan of one transform length, with the divisors each Stockham stage
 // needs (butterflies per row, and the span Ns of the stage) as magic numbers.
 struct FftPlan {
@@ -149,7 +157,11 @@ __device__ __forceinline__ void fft_stage(const C2 *in, C2 *out, int n, int nrow
 // Returns the buffer holding the result.
 // Always inlined: as a called function it makes every caller allocate the
 // callee's worst-case register budget (262 VGPRs, one wave per SIMD).
-template <typename C2>
+// BIG: the plan may hold radices 7, 11 or 13.  Their generic DFT sets the
+// register budget of every kernel that contains it (the y pass needs 94
+// VGPRs with them, 44 without), so the launchers pick the variant without
+// them whenever every radix of the plans is at most 5 (plan_big).
+template <typename C2, bool BIG>
 __device__ __forceinline__ C2 *lds_fft(C2 *a, C2 *b, const FftPlan &p, int nrow, const C2 *tw,
                                        bool inv) {
   int Ns = 1;
@@ -161,9 +173,13 @@ __device__ __forceinline__ C2 *lds_fft(C2 *a, C2 *b, const FftPlan &p, int nrow,
       case 3: fft_stage<C2, 3>(a, b, p.n, nrow, Ns, mb, mn, tw, inv); break;
       case 4: fft_stage<C2, 4>(a, b, p.n, nrow, Ns, mb, mn, tw, inv); break;
       case 5: fft_stage<C2, 5>(a, b, p.n, nrow, Ns, mb, mn, tw, inv); break;
-      case 7: fft_stage<C2, 7>(a, b, p.n, nrow, Ns, mb, mn, tw, inv); break;
-      case 11: fft_stage<C2, 11>(a, b, p.n, nrow, Ns, mb, mn, tw, inv); break;
-      default: fft_stage<C2, 13>(a, b, p.n, nrow, Ns, mb, mn, tw, inv); break;
+      default:
+        if constexpr (BIG) {
+          if (R == 7) fft_stage<C2, 7>(a, b, p.n, nrow, Ns, mb, mn, tw, inv);
+          else if (R == 11) fft_stage<C2, 11>(a, b, p.n, nrow, Ns, mb, mn, tw, inv);
+          else fft_stage<C2, 13>(a, b, p.n, nrow, Ns, mb, mn, tw, inv);
+        }
+        break;
     }
     __syncthreads();
     C2 *t = a;
@@ -347,7 +363,7 @@ __device__ __forceinline__ double readlane_f64(double v, int k) {
   return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
-template <typename CT>
+template <typename CT, bool BIG>
 __device__ __forceinline__ void spread_fftx_body(int bid, const int4 *__restrict__ posq,
                                                  const float4 *__restrict__ bb,
                                                  const int *__restrict__ meta, const BrickGeom &g,
@@ -529,7 +545,7 @@ __device__ __forceinline__ void spread_fftx_body(int bid, const int4 *__restrict
     a[e] = v;
   }
   __syncthreads();
-  C2 *res = lds_fft<C2>(a, b, p, ncx, tw, false);
+  C2 *res = lds_fft<C2, BIG>(a, b, p, ncx, tw, false);
   for (int e = tid; e < ncx * H; e += kSpreadThreads) {
     const int r = fdiv(e, mh), k = e - r * H;
     const int pz = fdiv(r, mhb), py = 2 * (r - pz * hb);
@@ -545,7 +561,7 @@ __device__ __forceinline__ void spread_fftx_body(int bid, const int4 *__restrict
     if (z < g.K2 && y + 1 < g.K1) out[o + H] = B;
   }
 }
-template <typename CT, bool PRIO>
+template <typename CT, bool PRIO, bool BIG>
 __global__ __launch_bounds__(kSpreadThreads) APO_SPREAD_WAVES_ATTR
 void k_spread_fftx(const int4 *__restrict__ posq,
                                                      const float4 *__restrict__ bb,
@@ -554,7 +570,7 @@ void k_spread_fftx(const int4 *__restrict__ posq,
                                                      const typename Cx<CT>::T *__restrict__ twg,
                                                      typename Cx<CT>::T *__restrict__ out) {
   if constexpr (PRIO) pme_wave_priority();
-  spread_fftx_body<CT>(blockIdx.x, posq, bb, meta, g, qunscale, p, twg, out);
+  spread_fftx_body<CT, BIG>(blockIdx.x, posq, bb, meta, g, qunscale, p, twg, out);
 }
 
 // Gather in cell order: 16 lanes per atom; the gradient goes straight into
@@ -628,7 +644,7 @@ __global__ __launch_bounds__(256) void k_gather_sorted(const int4 *__restrict__ 
 
 // x inverse: complex half rows -> real rows (C2R semantics: the imaginary
 // parts of the k = 0 and Nyquist bins are ignored).
-template <typename CT>
+template <typename CT, bool BIG>
 __device__ __forceinline__ void fftx_inv_body(int bid, const typename Cx<CT>::T *__restrict__ in,
                                               int nrows, const FftPlan &p, int rb,
                                               const typename Cx<CT>::T *__restrict__ twg,
@@ -678,7 +694,7 @@ __device__ __forceinline__ void fftx_inv_body(int bid, const typename Cx<CT>::T 
         a[e] = Z;
       });
   __syncthreads();
-  C2 *res = lds_fft<C2>(a, b, p, rb, tw, true);
+  C2 *res = lds_fft<C2, BIG>(a, b, p, rb, tw, true);
   for (int e = threadIdx.x; e < rb * n; e += blockDim.x) {
     const int r = fdiv(e, mn), x = e - r * n;
     const int ra = 2 * (r0 + r), rbw = ra + 1;
@@ -686,18 +702,18 @@ __device__ __forceinline__ void fftx_inv_body(int bid, const typename Cx<CT>::T 
     if (rbw < nrows) phi[(size_t)rbw * n + x] = res[e].y;
   }
 }
-template <typename CT, bool PRIO>
+template <typename CT, bool PRIO, bool BIG>
 __global__ __launch_bounds__(256) void k_fftx_inv(const typename Cx<CT>::T *__restrict__ in,
                                                   int nrows, FftPlan p, int rb,
                                                   const typename Cx<CT>::T *__restrict__ twg,
                                                   CT *__restrict__ phi) {
   if constexpr (PRIO) pme_wave_priority();
-  fftx_inv_body<CT>(blockIdx.x, in, nrows, p, rb, twg, phi);
+  fftx_inv_body<CT, BIG>(blockIdx.x, in, nrows, p, rb, twg, phi);
 }
 
 
 // y pass (either direction): block = (z, chunk of W kx columns).
-template <typename CT>
+template <typename CT, bool BIG>
 __device__ __forceinline__ void ffty_body(int bid, typename Cx<CT>::T *__restrict__ data, int H,
                                           int K1, int W, const FftPlan &p,
                                           const typename Cx<CT>::T *__restrict__ twg, int inv) {
@@ -724,19 +740,19 @@ __device__ __forceinline__ void ffty_body(int bid, typename Cx<CT>::T *__restric
         if (c < wc) a[c * n + y] = v;
       });
   __syncthreads();
-  C2 *res = lds_fft<C2>(a, b, p, wc, tw, inv != 0);
+  C2 *res = lds_fft<C2, BIG>(a, b, p, wc, tw, inv != 0);
   for (int e = threadIdx.x; e < W * n; e += blockDim.x) {
     const int y = fdiv(e, mw), c = e - y * W;
     if (c < wc) data[((size_t)z * K1 + y) * H + kx0 + c] = res[c * n + y];
   }
 }
-template <typename CT, bool PRIO>
+template <typename CT, bool PRIO, bool BIG>
 __global__ __launch_bounds__(256) void k_ffty(typename Cx<CT>::T *__restrict__ data, int H, int K1,
                                               int W, FftPlan p,
                                               const typename Cx<CT>::T *__restrict__ twg,
                                               int inv) {
   if constexpr (PRIO) pme_wave_priority();
-  ffty_body<CT>(blockIdx.x, data, H, K1, W, p, twg, inv);
+  ffty_body<CT, BIG>(blockIdx.x, data, H, K1, W, p, twg, inv);
 }
 
 
@@ -752,7 +768,7 @@ template <> __device__ __forceinline__ float exp_k<float>(float x) { return expf
 template <> __device__ __forceinline__ double exp_k<double>(double x) { return exp(x); }
 
 // z pass with the k-space convolution in the middle: block = (y, W columns).
-template <typename CT, bool CALC_EV>
+template <typename CT, bool CALC_EV, bool BIG>
 __device__ __forceinline__ void fftz_conv_body(int bid, typename Cx<CT>::T *__restrict__ data,
                                                const ConvArgs &c, const FftPlan &p,
                                                const typename Cx<CT>::T *__restrict__ twg,
@@ -786,7 +802,7 @@ __device__ __forceinline__ void fftz_conv_body(int bid, typename Cx<CT>::T *__re
         if (col < wc) a[col * n + z] = v;
       });
   __syncthreads();
-  C2 *res = lds_fft<C2>(a, b, p, wc, tw, false);
+  C2 *res = lds_fft<C2, BIG>(a, b, p, wc, tw, false);
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
   const int nf0 = c.K0 / 2 + (c.K0 % 2), nf1 = c.K1 / 2 + (c.K1 % 2), nf2 = c.K2 / 2 + (c.K2 % 2);
   for (int e = threadIdx.x; e < wc * n; e += blockDim.x) {
@@ -827,7 +843,7 @@ __device__ __forceinline__ void fftz_conv_body(int bid, typename Cx<CT>::T *__re
   }
   __syncthreads();
   C2 *other = res == a ? b : a;
-  C2 *out = lds_fft<C2>(res, other, p, wc, tw, true);
+  C2 *out = lds_fft<C2, BIG>(res, other, p, wc, tw, true);
   for (int e = threadIdx.x; e < W * n; e += blockDim.x) {
     const int z = fdiv(e, mw), col = e - z * W;
     if (col < wc) data[(size_t)z * zs + (size_t)y * H + kx0 + col] = out[col * n + z];
@@ -848,14 +864,14 @@ __device__ __forceinline__ void fftz_conv_body(int bid, typename Cx<CT>::T *__re
     }
   }
 }
-template <typename CT, bool CALC_EV, bool PRIO>
+template <typename CT, bool CALC_EV, bool PRIO, bool BIG>
 __global__ __launch_bounds__(256) void k_fftz_conv(typename Cx<CT>::T *__restrict__ data, ConvArgs c,
                                                    FftPlan p,
                                                    const typename Cx<CT>::T *__restrict__ twg,
                                                    const CT *__restrict__ prefac,
                                                    double *__restrict__ part) {
   if constexpr (PRIO) pme_wave_priority();
-  fftz_conv_body<CT, CALC_EV>(blockIdx.x, data, c, p, twg, prefac, part);
+  fftz_conv_body<CT, CALC_EV, BIG>(blockIdx.x, data, c, p, twg, prefac, part);
 }
 
 
@@ -923,7 +939,7 @@ static PmeShapes<CT> pme_shapes(const NbSystem &S, int rows = kFftRowsPerBlock,
   return P;
 }
 
-template <typename CT, bool PRIO>
+template <typename CT, bool PRIO, bool BIG>
 static void recip_t(NbSystem &S, const float4 *x, int64_t *force, int stride, bool EV,
                     typename Cx<CT>::T *cgrid, CT *phi, const typename Cx<CT>::T *twx,
                     const typename Cx<CT>::T *twy, const typename Cx<CT>::T *twz, const CT *pre,
@@ -934,27 +950,27 @@ static void recip_t(NbSystem &S, const float4 *x, int64_t *force, int stride, bo
   const double qunscale = 1.0;  // sorted coordinates carry the raw charge
   hipEvent_t t0 = nullptr;
   S.tic(1, s, t0);
-  k_spread_fftx<CT, PRIO><<<P.nslabs, kSpreadThreads, P.shs, s>>>(
+  k_spread_fftx<CT, PRIO, BIG><<<P.nslabs, kSpreadThreads, P.shs, s>>>(
       S.posq.get(), S.cell_bb.get(), S.meta.get(), P.bg, qunscale, plan_of(S.fx), twx, cgrid);
   APO_LAUNCH_CHECK();
   S.toc(1, s, t0);
   S.tic(2, s, t0);
-  k_ffty<CT, PRIO><<<K2 * P.ncy, P.ft, P.shy, s>>>(cgrid, H, K1, P.wy, plan_of(S.fy), twy, 0);
+  k_ffty<CT, PRIO, BIG><<<K2 * P.ncy, P.ft, P.shy, s>>>(cgrid, H, K1, P.wy, plan_of(S.fy), twy, 0);
   APO_LAUNCH_CHECK();
   if (K1 * P.ncz > S.part.recip_n) throw RuntimeError("reciprocal partial buffer too small");
   double *part = S.parts_ptr() + S.part.recip_off;
   if (EV) {
     launch_fill_f64(part, (size_t)S.part.recip_n * kRecipPart, s);
-    k_fftz_conv<CT, true, PRIO><<<K1 * P.ncz, P.ft, P.shz, s>>>(cgrid, P.c, plan_of(S.fz), twz,
+    k_fftz_conv<CT, true, PRIO, BIG><<<K1 * P.ncz, P.ft, P.shz, s>>>(cgrid, P.c, plan_of(S.fz), twz,
                                                                 pre, part);
   } else {
-    k_fftz_conv<CT, false, PRIO><<<K1 * P.ncz, P.ft, P.shz, s>>>(cgrid, P.c, plan_of(S.fz), twz,
+    k_fftz_conv<CT, false, PRIO, BIG><<<K1 * P.ncz, P.ft, P.shz, s>>>(cgrid, P.c, plan_of(S.fz), twz,
                                                                  pre, part);
   }
   APO_LAUNCH_CHECK();
-  k_ffty<CT, PRIO><<<K2 * P.ncy, P.ft, P.shy, s>>>(cgrid, H, K1, P.wy, plan_of(S.fy), twy, 1);
+  k_ffty<CT, PRIO, BIG><<<K2 * P.ncy, P.ft, P.shy, s>>>(cgrid, H, K1, P.wy, plan_of(S.fy), twy, 1);
   APO_LAUNCH_CHECK();
-  k_fftx_inv<CT, PRIO><<<(P.npair + P.rb - 1) / P.rb, P.ft, P.shx, s>>>(
+  k_fftx_inv<CT, PRIO, BIG><<<(P.npair + P.rb - 1) / P.rb, P.ft, P.shx, s>>>(
       cgrid, P.nrows, plan_of(S.fx), P.rb, twx, phi);
   APO_LAUNCH_CHECK();
   S.toc(2, s, t0);
@@ -970,17 +986,24 @@ void recip_launch(NbSystem &S, const float4 *x, int64_t *force, int stride, bool
                   hipStream_t s) {
   // the issue priority is a template parameter: as a kernel argument, every
   // wave waited for its load before issuing the kernel's own first loads
-  auto run = [&](auto prio) {
-    constexpr bool P = decltype(prio)::value;
+  auto run = [&](auto prio, auto big) {
+    constexpr bool P = decltype(prio)::value, B = decltype(big)::value;
     if (S.precision == 1)
-      recip_t<double, P>(S, x, force, stride, E || V, S.cgrid_d.get(), S.phi_d.get(),
-                         S.fx.tw64.get(), S.fy.tw64.get(), S.fz.tw64.get(), S.prefac_d.get(), s);
+      recip_t<double, P, B>(S, x, force, stride, E || V, S.cgrid_d.get(), S.phi_d.get(),
+                            S.fx.tw64.get(), S.fy.tw64.get(), S.fz.tw64.get(), S.prefac_d.get(),
+                            s);
     else
-      recip_t<float, P>(S, x, force, stride, E || V, S.cgrid.get(), S.phi.get(), S.fx.tw.get(),
-                        S.fy.tw.get(), S.fz.tw.get(), S.prefac.get(), s);
+      recip_t<float, P, B>(S, x, force, stride, E || V, S.cgrid.get(), S.phi.get(),
+                           S.fx.tw.get(), S.fy.tw.get(), S.fz.tw.get(), S.prefac.get(), s);
   };
-  if (S.pme_prio) run(std::true_type{});
-  else run(std::false_type{});
+  const bool big = plan_big(S.fx) || plan_big(S.fy) || plan_big(S.fz);
+  if (S.pme_prio) {
+    if (big) run(std::true_type{}, std::true_type{});
+    else run(std::true_type{}, std::false_type{});
+  } else {
+    if (big) run(std::false_type{}, std::true_type{});
+    else run(std::false_type{}, std::false_type{});
+  }
 }
 
 // ---- fused launches over several systems (mixed precision, no energy) ------
@@ -1023,31 +1046,36 @@ struct GatherArgs {
   int stride;
 };
 
+template <bool BIG>
 __global__ __launch_bounds__(kSpreadThreads) APO_SPREAD_WAVES_ATTR
 void kb_spread_fftx(const SpreadArgs *__restrict__ tab,
                                                       const int *__restrict__ bstart, int nsys) {
   const int s = batch_system(bstart, nsys);
   const SpreadArgs &a = tab[s];
-  spread_fftx_body<float>(blockIdx.x - bstart[s], a.posq, a.bb, a.meta, a.g, 1.0, a.p, a.tw,
+  spread_fftx_body<float, BIG>(blockIdx.x - bstart[s], a.posq, a.bb, a.meta, a.g, 1.0, a.p, a.tw,
                           a.cout);
 }
+template <bool BIG>
 __global__ __launch_bounds__(256) void kb_fftx_inv(const FftxArgs *__restrict__ tab,
                                                    const int *__restrict__ bstart, int nsys) {
   const int s = batch_system(bstart, nsys);
   const FftxArgs &a = tab[s];
-  fftx_inv_body<float>(blockIdx.x - bstart[s], a.cin, a.nrows, a.p, a.rb, a.tw, a.phi);
+  fftx_inv_body<float, BIG>(blockIdx.x - bstart[s], a.cin, a.nrows, a.p, a.rb, a.tw, a.phi);
 }
+template <bool BIG>
 __global__ __launch_bounds__(256) void kb_ffty(const FftyArgs *__restrict__ tab,
                                                const int *__restrict__ bstart, int nsys, int inv) {
   const int s = batch_system(bstart, nsys);
   const FftyArgs &a = tab[s];
-  ffty_body<float>(blockIdx.x - bstart[s], a.data, a.H, a.K1, a.W, a.p, a.tw, inv);
+  ffty_body<float, BIG>(blockIdx.x - bstart[s], a.data, a.H, a.K1, a.W, a.p, a.tw, inv);
 }
+template <bool BIG>
 __global__ __launch_bounds__(256) void kb_fftz_conv(const FftzArgs *__restrict__ tab,
                                                     const int *__restrict__ bstart, int nsys) {
   const int s = batch_system(bstart, nsys);
   const FftzArgs &a = tab[s];
-  fftz_conv_body<float, false>(blockIdx.x - bstart[s], a.data, a.c, a.p, a.tw, a.prefac, nullptr);
+  fftz_conv_body<float, false, BIG>(blockIdx.x - bstart[s], a.data, a.c, a.p, a.tw, a.prefac,
+                                    nullptr);
 }
 __global__ __launch_bounds__(256) void kb_gather(const GatherArgs *__restrict__ tab,
                                                  const int *__restrict__ bstart, int nsys) {
@@ -1064,6 +1092,7 @@ struct PmeBatch {
   BatchLaunch<FftzArgs> fz;
   BatchLaunch<GatherArgs> gather;
   int ft = kFftThreads;  // threads per block of the y, z and x-inverse passes
+  bool big = false;      // some system's plans need radices above 5 (plan_big)
 };
 
 PmeBatch *pme_batch_create() { return new PmeBatch(); }
@@ -1074,6 +1103,7 @@ void pme_batch_clear(PmeBatch &b) {
   b.fy.clear();
   b.fz.clear();
   b.gather.clear();
+  b.big = false;
 }
 
 void pme_batch_add(PmeBatch &b, NbSystem &S, int64_t *force, int stride, int fft_rows,
@@ -1081,6 +1111,7 @@ void pme_batch_add(PmeBatch &b, NbSystem &S, int64_t *force, int stride, int fft
   if (S.precision != 0) throw InvalidArgument("fused batches run the mixed-precision path");
   const PmeShapes<float> P = pme_shapes<float>(S, fft_rows, fft_cols, fft_threads, true);
   b.ft = P.ft;
+  b.big = b.big || plan_big(S.fx) || plan_big(S.fy) || plan_big(S.fz);
   if (S.K[1] * P.ncz > S.part.recip_n) throw RuntimeError("reciprocal partial buffer too small");
   const int H = S.K[0] / 2 + 1;
   b.spread.add(SpreadArgs{S.posq.get(), S.cell_bb.get(), S.meta.get(), P.bg, plan_of(S.fx),
@@ -1108,22 +1139,26 @@ void pme_batch_upload(PmeBatch &b, hipStream_t s) {
 // spread + x FFT, the other four FFT passes with the convolution, gather: one
 // launch each for all systems (cell boxes must be current).  Without the gather the
 // launches write only the batch's grids (the autotuner times them so).
-void pme_batch_launch(PmeBatch &b, hipStream_t s, bool gather) {
-  if (!b.spread.nsys()) return;
-  kb_spread_fftx<<<b.spread.blocks(), kSpreadThreads, b.spread.lds, s>>>(
+template <bool BIG> static void pme_batch_launch_t(PmeBatch &b, hipStream_t s, bool gather) {
+  kb_spread_fftx<BIG><<<b.spread.blocks(), kSpreadThreads, b.spread.lds, s>>>(
       b.spread.d_args.get(), b.spread.d_start.get(), b.spread.nsys());
-  kb_ffty<<<b.fy.blocks(), b.ft, b.fy.lds, s>>>(b.fy.d_args.get(), b.fy.d_start.get(), b.fy.nsys(),
-                                               0);
-  kb_fftz_conv<<<b.fz.blocks(), b.ft, b.fz.lds, s>>>(b.fz.d_args.get(), b.fz.d_start.get(),
-                                                    b.fz.nsys());
-  kb_ffty<<<b.fy.blocks(), b.ft, b.fy.lds, s>>>(b.fy.d_args.get(), b.fy.d_start.get(), b.fy.nsys(),
-                                               1);
-  kb_fftx_inv<<<b.fx_inv.blocks(), b.ft, b.fx_inv.lds, s>>>(b.fx_inv.d_args.get(),
-                                                           b.fx_inv.d_start.get(), b.fx_inv.nsys());
+  kb_ffty<BIG><<<b.fy.blocks(), b.ft, b.fy.lds, s>>>(b.fy.d_args.get(), b.fy.d_start.get(),
+                                                    b.fy.nsys(), 0);
+  kb_fftz_conv<BIG><<<b.fz.blocks(), b.ft, b.fz.lds, s>>>(b.fz.d_args.get(), b.fz.d_start.get(),
+                                                         b.fz.nsys());
+  kb_ffty<BIG><<<b.fy.blocks(), b.ft, b.fy.lds, s>>>(b.fy.d_args.get(), b.fy.d_start.get(),
+                                                    b.fy.nsys(), 1);
+  kb_fftx_inv<BIG><<<b.fx_inv.blocks(), b.ft, b.fx_inv.lds, s>>>(
+      b.fx_inv.d_args.get(), b.fx_inv.d_start.get(), b.fx_inv.nsys());
   if (gather)
     kb_gather<<<b.gather.blocks(), 256, 0, s>>>(b.gather.d_args.get(), b.gather.d_start.get(),
                                                 b.gather.nsys());
   APO_LAUNCH_CHECK();
+}
+void pme_batch_launch(PmeBatch &b, hipStream_t s, bool gather) {
+  if (!b.spread.nsys()) return;
+  if (b.big) pme_batch_launch_t<true>(b, s, gather);
+  else pme_batch_launch_t<false>(b, s, gather);
 }
 
 }  // namespace apo

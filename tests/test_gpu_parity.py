@@ -225,6 +225,23 @@ def test_odd_pme_grid_against_oracle(oracle, n_waters):
         assert abs(got["energy"][k] - ref["energy"][k]) <= MIXED_ENERGY_RTOL * scale, k
 
 
+@pytest.mark.parametrize("n_waters,K", [(700, 28), (1200, 33), (600, 26)])
+def test_large_radix_pme_grid_against_oracle(oracle, n_waters, K):
+    """Grids whose FFT plans need radix 7, 11 or 13 (28 = 4 x 7, 33 = 3 x 11,
+    26 = 2 x 13): the FFT kernels' variant that holds those stages (the other
+    variant, for plans of radices up to 5, runs every other test)."""
+    L = (n_waters / systems.WATER_DENSITY) ** (1.0 / 3.0)
+    spec = systems.tip3p_box(n_waters, L, K, seed=6100 + K)
+    print("case radix grid %d waters L=%.2f K=%d" % (n_waters, L, K), flush=True)
+    got = gpu_eval(spec)
+    ref = oracle_eval(oracle, spec)
+    err = report("radix K=%d" % K, got, ref)
+    assert err <= MIXED_FORCE_RTOL
+    scale = energy_scale(ref)
+    for k in ("ewks", "elec", "vdw", "ewex"):
+        assert abs(got["energy"][k] - ref["energy"][k]) <= MIXED_ENERGY_RTOL * scale, k
+
+
 def test_config2_full_size_against_oracle(oracle):
     """98,304-atom TIP3P box, 100^3 grid: the benchmark workload at step 0."""
     spec = systems.config2()
