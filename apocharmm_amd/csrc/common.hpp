@@ -202,6 +202,7 @@ struct PairArgs {
   const int4 *posq;
   const int *types;
   const int4 *items;
+  const int *iorder;  // processing order of the items
   const uint32_t *ent;
   const int *cand_j, *meta;
   const float2 *vdwtab, *vdwsq;

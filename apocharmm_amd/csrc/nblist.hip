@@ -403,6 +403,62 @@ __global__ __launch_bounds__(64 * kGroupsPerCell) void k_grp_emit(
   }
 }
 
+// 10. Processing order of the work items for the pair kernel: by step count,
+// longest first, items of one length in index order (a stable counting sort
+// in one block).  The kernel's blocks take items in this order, so the items
+// still running when the launch drains are short ones (list scheduling,
+// longest processing time first).  order[p] = item index.
+constexpr int kOrderThreads = 1024;
+__global__ __launch_bounds__(kOrderThreads) void k_item_order(const int *__restrict__ meta,
+                                                              const int4 *__restrict__ items,
+                                                              int *__restrict__ order,
+                                                              int isteps) {
+  constexpr int kW = kOrderThreads / 64;
+  __shared__ int cnt[kItemSteps + 1], run[kItemSteps + 1];
+  __shared__ int wcnt[kW][kItemSteps + 1];
+  const int n = meta[3];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  auto len_of = [&](int i) {
+    const int4 it = items[i];
+    return min(max((it.z - it.y) >> 6, 1), isteps);
+  };
+  if (tid <= kItemSteps) cnt[tid] = 0;
+  __syncthreads();
+  for (int i = tid; i < n; i += kOrderThreads) atomicAdd(&cnt[len_of(i)], 1);
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int L = isteps; L >= 1; L--) {
+      run[L] = acc;
+      acc += cnt[L];
+    }
+  }
+  __syncthreads();
+  for (int c0 = 0; c0 < n; c0 += kOrderThreads) {
+    const int i = c0 + tid;
+    const int L = i < n ? len_of(i) : 0;
+    int rank = 0;
+    for (int b = 1; b <= isteps; b++) {
+      const unsigned long long m = __ballot(L == b);
+      if (L == b) rank = __popcll(m & ((1ull << lane) - 1ull));
+      if (lane == 0) wcnt[w][b] = __popcll(m);
+    }
+    __syncthreads();
+    if (i < n) {
+      int p = run[L] + rank;
+      for (int v = 0; v < w; v++) p += wcnt[v][L];
+      order[p] = i;
+    }
+    __syncthreads();
+    if (tid >= 1 && tid <= isteps) {
+      int t = 0;
+      for (int v = 0; v < kW; v++) t += wcnt[v][tid];
+      run[tid] += t;
+    }
+    __syncthreads();
+  }
+}
+
 // List pruning (no counterpart in the reference; the idea of GROMACS's
 // dynamic pruning).  Between builds, every K steps, each group's entries of
 // the build's list are tested against the current coordinates with the
@@ -479,7 +535,7 @@ void nblist_prune(NbSystem &S, hipStream_t s) {
   APO_LAUNCH_CHECK();
 }
 
-// 10. After the counts are known: any capacity overflow or input error
+// 11. After the counts are known: any capacity overflow or input error
 // leaves the list empty (no pair kernel work, no out-of-range write) and sets
 // meta[8]; then the build's counts and flags go to pinned host memory with
 // the running build number in slot 15 (meta[16] survives the per-build
@@ -548,6 +604,7 @@ void list_reserve(NbSystem &S, int ncand, int nent, int nitem) {
     S.item_cap = (size_t)(nitem * 1.5) + 64;
     S.items.reserve(S.item_cap);
     S.items_p.reserve(S.item_cap);
+    S.item_order.reserve(S.item_cap);
     S.generation++;
   }
 }
@@ -669,6 +726,8 @@ void nblist_build(NbSystem &S, const float4 *x, hipStream_t s, bool sizing) {
   k_grp_emit<<<S.ncell_cap, 64 * kGroupsPerCell, 0, s>>>(meta, S.icell_start.get(), S.cand_mask.get(),
                                         S.grp_start.get(), S.grp_itemstart.get(), S.ent.get(),
                                         S.items.get(), S.item_steps);
+  APO_LAUNCH_CHECK();
+  k_item_order<<<1, kOrderThreads, 0, s>>>(meta, S.items.get(), S.item_order.get(), S.item_steps);
   APO_LAUNCH_CHECK();
   S.builds_queued++;
   S.list_valid = true;

@@ -133,7 +133,7 @@ template <typename T2> static size_t pair_tab_lds_bytes(int mode, int ntypes) {
 template <typename CT, bool CALC_E, bool CALC_V, int TAB, bool ISO, bool PRUNE>
 __device__ __forceinline__ void pair_body(
     int bid, int nb, const int4 *__restrict__ posq, const int *__restrict__ types,
-    const int4 *__restrict__ items, const uint32_t *__restrict__ ent,
+    const int4 *__restrict__ items, const int *__restrict__ iorder, const uint32_t *__restrict__ ent,
     const int *__restrict__ cand_j, const int *__restrict__ meta,
     const typename Vec2<CT>::T *__restrict__ vdwtab, const typename Vec2<CT>::T *__restrict__ vdwsq,
     const float2 *__restrict__ ljtype, const float4 *__restrict__ vdwpqa, int ntypes,
@@ -166,7 +166,9 @@ __device__ __forceinline__ void pair_body(
 #pragma unroll
   for (int k = 0; k < kPairPart; k++) acc[k] = 0.0;
 
-  for (int item = bid; item < nitem; item += nb) {
+  // items in the list's processing order (longest first)
+  for (int p = bid; p < nitem; p += nb) {
+    const int item = iorder[p];
     const int4 it = items[item];
     const int ibase = it.x * kGroup;
     // i-data to SGPRs: lane k < kGroup loads slot ibase + k
@@ -403,13 +405,13 @@ __device__ __forceinline__ void pair_body(
 template <typename CT, bool CALC_E, bool CALC_V, int TAB, bool ISO, bool PRUNE>
 __global__ __launch_bounds__(64) void k_pair(
     const int4 *__restrict__ posq, const int *__restrict__ types, const int4 *__restrict__ items,
-    const uint32_t *__restrict__ ent, const int *__restrict__ cand_j, const int *__restrict__ meta,
+    const int *__restrict__ iorder, const uint32_t *__restrict__ ent, const int *__restrict__ cand_j, const int *__restrict__ meta,
     const typename Vec2<CT>::T *__restrict__ vdwtab, const typename Vec2<CT>::T *__restrict__ vdwsq,
     const float2 *__restrict__ ljtype, const float4 *__restrict__ vdwpqa, int ntypes,
     const PairConsts<CT> P, CT qfac, unsigned long long *__restrict__ fsort, int ss,
     double *__restrict__ part, const PairConsts<double> PE, const double2 *__restrict__ vdwtab_e,
     uint32_t *__restrict__ ent_out, int4 *__restrict__ items_out, CT rp2t) {
-  pair_body<CT, CALC_E, CALC_V, TAB, ISO, PRUNE>(blockIdx.x, gridDim.x, posq, types, items, ent,
+  pair_body<CT, CALC_E, CALC_V, TAB, ISO, PRUNE>(blockIdx.x, gridDim.x, posq, types, items, iorder, ent,
                                                  cand_j, meta, vdwtab, vdwsq, ljtype, vdwpqa,
                                                  ntypes, P, qfac, fsort, ss, part, PE, vdwtab_e,
                                                  ent_out, items_out, rp2t);
@@ -424,7 +426,7 @@ __global__ __launch_bounds__(64) void kb_pair(const PairArgs *__restrict__ tab,
   const PairArgs &a = tab[s];
   const PairConsts<double> no_energy{};
   pair_body<float, false, false, TAB, ISO, false>(
-      blockIdx.x - bstart[s], bstart[s + 1] - bstart[s], a.posq, a.types, a.items, a.ent,
+      blockIdx.x - bstart[s], bstart[s + 1] - bstart[s], a.posq, a.types, a.items, a.iorder, a.ent,
       a.cand_j, a.meta, a.vdwtab, a.vdwsq, a.ljtype, a.vdwpqa, a.ntypes, a.P, a.qfac, a.fsort,
       a.ss, nullptr, no_energy, nullptr, nullptr, nullptr, 0.0f);
 }
@@ -433,6 +435,7 @@ struct PairLaunch {
   const int4 *posq;
   const int *types;
   const int4 *items;
+  const int *iorder;
   const uint32_t *ent;
   const int *cand_j, *meta;
   const float2 *ljtype;
@@ -459,14 +462,15 @@ static void go_pair(const PairLaunch &L, const typename Vec2<CT>::T *tab,
   if constexpr (sizeof(CT) == sizeof(float) && !E && !V) {
     if (L.ent_out) {
       k_pair<CT, false, false, TAB, ISO, true><<<L.blocks, 64, L.lds, L.s>>>(
-          L.posq, L.types, L.items, L.ent, L.cand_j, L.meta, tab, sq, L.ljtype, L.vdwpqa,
+          L.posq, L.types, L.items, L.iorder, L.ent, L.cand_j, L.meta, tab, sq, L.ljtype, L.vdwpqa,
           L.ntypes, P, qfac, L.fsort, L.ss, L.part, PE, L.vdwtab_e, L.ent_out, L.items_out,
           L.rp2t);
       return;
     }
   }
   k_pair<CT, E, V, TAB, ISO, false><<<L.blocks, 64, L.lds, L.s>>>(
-      L.posq, L.types, L.items, L.ent, L.cand_j, L.meta, tab, sq, L.ljtype, L.vdwpqa, L.ntypes, P,
+      L.posq, L.types, L.items, L.iorder, L.ent, L.cand_j, L.meta, tab, sq, L.ljtype, L.vdwpqa,
+      L.ntypes, P,
       qfac, L.fsort, L.ss, L.part, PE, L.vdwtab_e, nullptr, nullptr, (CT)0);
 }
 template <typename CT, bool E, bool V, bool ISO>
@@ -508,6 +512,7 @@ static void launch_pair_t(NbSystem &S, const typename Vec2<CT>::T *tab,
   PairLaunch L{S.posq.get(),
                S.type_s.get(),
                pruned ? S.items_p.get() : S.items.get(),
+               S.item_order.get(),
                pruned ? S.ent_p.get() : S.ent.get(),
                S.cand_j.get(),
                S.meta.get(),
@@ -577,7 +582,8 @@ void direct_finalize(NbSystem &S, int64_t *force, int stride, hipStream_t s) {
 // ---- fused launches over several systems ------------------------------------
 void batch_add_pair(BatchLaunch<PairArgs> &b, NbSystem &S, int blocks) {
   if (S.precision != 0) throw InvalidArgument("fused batches run the mixed-precision path");
-  PairArgs a{S.posq.get(), S.type_s.get(), S.items.get(), S.ent.get(), S.cand_j.get(),
+  PairArgs a{S.posq.get(), S.type_s.get(), S.items.get(), S.item_order.get(), S.ent.get(),
+             S.cand_j.get(),
              S.meta.get(), S.vdwtab.get(), S.vdwsq.get(), S.ljtype.get(), S.vdwpqa.get(),
              S.ntypes, S.pc_f, (float)kCcelec,
              reinterpret_cast<unsigned long long *>(S.fsort.get()), S.ssort};

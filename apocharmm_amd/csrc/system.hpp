@@ -94,6 +94,7 @@ struct NbSystem {
   DevBuf<int> grp_cnt, grp_start, grp_nitem, grp_itemstart;
   DevBuf<uint32_t> ent;
   DevBuf<int4> items;
+  DevBuf<int> item_order;  // items by step count, longest first (pair-kernel order)
   // pruned copy of the list (same layout and regions), read by the pair
   // kernel in calc_force calls with kCalcUsePruned; pruned_build is the build
   // (builds_queued) it was last derived from, kept by its user
