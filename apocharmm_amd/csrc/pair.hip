@@ -411,6 +411,13 @@ __global__ __launch_bounds__(64) void k_pair(
     const PairConsts<CT> P, CT qfac, unsigned long long *__restrict__ fsort, int ss,
     double *__restrict__ part, const PairConsts<double> PE, const double2 *__restrict__ vdwtab_e,
     uint32_t *__restrict__ ent_out, int4 *__restrict__ items_out, CT rp2t) {
+  // Register floor of 112 VGPRs (an empty asm statement that names v111): a
+  // SIMD then holds at most four pair waves (448 of its 512 VGPRs) and keeps
+  // room for one wave of the PME passes (44-54 VGPRs), which otherwise waited
+  // for a pair wave to end (measured on the 98,304-atom box: z pass with the
+  // convolution 102 -> 86 us inside the step, step 705 -> 714 ns/day).  The
+  // kernel's own allocation is 92-107 VGPRs, so no spill follows.
+  asm volatile("" ::: "v111");
   pair_body<CT, CALC_E, CALC_V, TAB, ISO, PRUNE>(blockIdx.x, gridDim.x, posq, types, items, iorder, ent,
                                                  cand_j, meta, vdwtab, vdwsq, ljtype, vdwpqa,
                                                  ntypes, P, qfac, fsort, ss, part, PE, vdwtab_e,
