@@ -889,8 +889,9 @@ template <typename CT> struct PmeShapes {
 // rows: x-inverse row pairs per block; cols: y/z-pass columns per block (both
 // capped by 64 KiB of LDS); threads: block size of those passes (64..256).
 // The spread slab is kSlabBy x kSlabBz rows, halved (z first, then y down to
-// 2) while the grid gives fewer than two slab blocks per CU (small grids: a
-// 64^3 grid has 256 4x4 slabs, one per CU) or the LDS does not fit.  Any slab
+// 2) while the grid gives fewer slab blocks than CUs (small grids: a 32^3 grid
+// has 64 4x4 slabs) or the LDS does not fit; every halving doubles the atoms
+// each block walks, so one block per CU is kept, not more.  Any slab
 // shape sums every grid point exactly and pairs the same rows for the x FFT,
 // so the grid is bitwise the same.
 template <typename CT>
@@ -909,7 +910,7 @@ static PmeShapes<CT> pme_shapes(const NbSystem &S, int rows = kFftRowsPerBlock,
   const size_t lds_cap = 64 * 1024;
   int by = std::max(2, kSlabBy & ~1), bz = std::max(1, kSlabBz);
   auto slabs = [&](int y, int z) { return ((K1 + y - 1) / y) * ((K2 + z - 1) / z); };
-  while ((slab_lds<C2>(K0, by, bz) > lds_cap - 2048 || (!batched && slabs(by, bz) < 2 * S.ncu)) &&
+  while ((slab_lds<C2>(K0, by, bz) > lds_cap - 2048 || (!batched && slabs(by, bz) < S.ncu)) &&
          (bz > 1 || by > 2)) {
     if (bz >= by && bz > 1) bz /= 2;
     else by = std::max(2, (by / 2) & ~1);
