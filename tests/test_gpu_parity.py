@@ -1,6 +1,6 @@
 """GPU parity: the HIP force path (through the C ABI) against the FP64 oracle.
 
-Metrics (BASELINE: force RMS relative error <= 2e-6 for the mixed path):
+Metrics (mixed-precision bar: force RMS relative error <= 2e-6):
   force_rms_rel = sqrt(mean |g - g_ref|^2) / sqrt(mean |g_ref|^2)
 Energies are compared per term, the virial against max |W_ref|.  The all-FP64
 instantiation of the same kernels must agree to ~1e-9.  Fixed-point
