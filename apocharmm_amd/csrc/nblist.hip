@@ -281,8 +281,8 @@ __global__ __launch_bounds__(64) void k_tiles(const float4 *__restrict__ bb,
 // 7. 32x32 masks: bit t of lane l set <=> pair (i-cell atom t, j-cell atom l)
 // is skipped: padding, diagonal i >= j, topological exclusion (1-2/1-3/1-4),
 // or farther than the list cutoff at build time.  Four waves per i-cell walk
-// its candidate tiles two at a time (lanes 0-31 and 32-63); each wave loads
-// the i-cell's atoms once and reads them as wave-uniform values.
+// its candidate tiles two at a time (lanes 0-31 and 32-63); the i-cell's atoms
+// are staged in LDS once per block and read as wave-uniform values.
 // ISO (cubic box): the differences stay in fixed-point units and r^2 is
 // compared with rl2 in those units (rl2u), with no per-axis multiplies.
 constexpr int kMaskWaves = 4;
@@ -297,7 +297,11 @@ __global__ __launch_bounds__(64 * kMaskWaves) void k_tile_mask(
   if (ic >= meta[0]) return;
   const int lane = threadIdx.x & 63, l = lane & 31, half = lane >> 5;
   const int wave = threadIdx.x >> 6;
-  const int4 pil = posq[ic * kCell + l];
+  // the i-cell's coordinates, read by every wave as LDS broadcasts (one
+  // 16-byte read per i atom instead of three cross-lane reads)
+  __shared__ int4 sh_pi[kCell];
+  if (threadIdx.x < kCell) sh_pi[threadIdx.x] = posq[ic * kCell + threadIdx.x];
+  __syncthreads();
   const int gil = loc2glo[ic * kCell + l];
   uint32_t ipad = 0;  // bit t: i slot t is padding
   {
@@ -316,9 +320,8 @@ __global__ __launch_bounds__(64 * kMaskWaves) void k_tile_mask(
     if (tv && gj >= 0) m = ipad | (jc == ic ? ~((1u << l) - 1u) : 0u);
 #pragma unroll 8
     for (int t = 0; t < kCell; t++) {
-      const int px = __builtin_amdgcn_readlane(pil.x, t);
-      const int py = __builtin_amdgcn_readlane(pil.y, t);
-      const int pz = __builtin_amdgcn_readlane(pil.z, t);
+      const int4 pi = sh_pi[t];
+      const int px = pi.x, py = pi.y, pz = pi.z;
       const float dx = ISO ? (float)(int)(px - pj.x) : (float)(int)(px - pj.x) * unit.x;
       const float dy = ISO ? (float)(int)(py - pj.y) : (float)(int)(py - pj.y) * unit.y;
       const float dz = ISO ? (float)(int)(pz - pj.z) : (float)(int)(pz - pj.z) * unit.z;
